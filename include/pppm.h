@@ -1,0 +1,223 @@
+/*
+ * pppm.h -- C ABI of the MI355X progressive photon mapping library
+ * (libpppm_hip.so, built for gfx950).
+ *
+ * Every entry point returns an int status (PPPM_OK = 0); the message of the
+ * last failure on a context is pppm_last_error(ctx).  Nothing aborts across
+ * the ABI.  Host buffers are owned by the caller; device memory is owned by
+ * the context.  One context per host thread; calls are synchronous.
+ *
+ * Each declaration names the reference interface it replaces
+ * (tatsy/tatsy-pppm, paths relative to the reference root).
+ */
+#ifndef PPPM_H
+#define PPPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPPM_OK 0
+#define PPPM_ERR_INVALID 1  /* bad argument */
+#define PPPM_ERR_HIP 2      /* HIP runtime error (no device, launch failure ...) */
+#define PPPM_ERR_STATE 3    /* call out of order (no scene, no BVH, no camera ...) */
+#define PPPM_ERR_CAPACITY 4 /* photon buffer too small; stats report the size needed */
+#define PPPM_ERR_NOMEM 5
+
+/* Material flags: sources/bsdf.h:19-26 BsdfType, plus an emitter bit. */
+#define PPPM_BSDF_LAMBERTIAN 0x01
+#define PPPM_BSDF_SPECULAR 0x02
+#define PPPM_BSDF_PHONG 0x04 /* not supported: rejected by pppm_scene_upload */
+#define PPPM_BSDF_REFRACTION 0x08
+#define PPPM_BSDF_BSSRDF 0x10
+#define PPPM_BSDF_EMITTER 0x40
+
+/* One material (64 bytes).  Replaces the polymorphic BSDF handle of
+ * sources/bsdf.h:32-59 (reflectance() + type()) and the dipole parameters of
+ * sources/bssrdf.h:47-66. */
+typedef struct {
+    int32_t type;
+    float reflectance[3];
+    float emission[3]; /* radiance of an emitter's front face */
+    float ior;         /* refraction: 1.5 in the reference (sources/common.h:16) */
+    float sigma_a[3];
+    float sigmap_s[3];
+    float eta;
+    float pad;
+} pppm_bsdf;
+
+/* Parallelogram area light: corner + two edges; emits on the side of
+ * edge_u x edge_v.  (The reference's only emitter is Envmap::samplePhoton,
+ * sources/envmap.cc:62-86; this is the emission contract the configs need.) */
+typedef struct {
+    float corner[3];
+    float edge_u[3];
+    float edge_v[3];
+    float emission[3];
+} pppm_light;
+
+/* Pinhole camera (sources/perspective_camera.h:24-31): vertical field of view
+ * in degrees, orthonormal image basis, image row 0 at the top. */
+typedef struct {
+    double center[3];
+    double direction[3];
+    double up[3];
+    double fov_y_deg;
+    int32_t width;
+    int32_t height;
+} pppm_camera;
+
+/* Ray for batch queries (sources/ray.h:16-36) and its hit (ray.h:38-79):
+ * tri = triangle index in upload order, -1 on a miss. */
+typedef struct {
+    float origin[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} pppm_ray;
+
+typedef struct {
+    float t;
+    int32_t tri;
+    float u, v;
+} pppm_hit;
+
+typedef struct {
+    int64_t nodes;     /* 4-wide nodes */
+    int64_t leaves;
+    int32_t depth;     /* root = 1 */
+    int32_t max_leaf;  /* triangles per leaf limit */
+    double build_ms;
+} pppm_bvh_info;
+
+/* Per-render settings (RenderParameters, sources/render_parameters.h:4-44,
+ * plus the knobs the reference hard-codes: ALPHA, ppm_probabilistic.cc:9;
+ * initial radius, :55). */
+typedef struct {
+    int64_t photons;       /* emitted photons per pass (RenderParameters::photons) */
+    int32_t passes;        /* RenderParameters::spp: number of passes */
+    int32_t max_bounces;   /* bounceLimit, 64 in the reference */
+    double alpha;          /* 0.7 */
+    double radius1;        /* r_1, scene units */
+    double ray_eps;        /* continuation-ray offset along the normal */
+    uint64_t seed;
+    uint32_t scene_id;     /* RNG stream id of this scene */
+    int32_t hash_size;     /* 0: the number of stored photons of each pass */
+    double capacity_factor;/* photon buffer = photons * factor records (0: 8) */
+} pppm_render_params;
+
+/* Timings and counters of one pass. */
+typedef struct {
+    int32_t pass_index;
+    int32_t hash_size;
+    double radius;
+    int64_t photons_emitted;
+    int64_t photons_stored;
+    int64_t photon_segments;   /* ray queries of the photon pass */
+    int64_t eye_segments;      /* ray queries of the eye pass */
+    int64_t hitpoints;
+    int64_t gather_buckets;    /* sum over hit points of de-duplicated buckets */
+    int64_t gather_candidates; /* sum of photon records in them */
+    int64_t gather_inradius;
+    double photon_ms, order_ms, grid_ms, eye_ms, gather_ms, total_ms;
+} pppm_pass_stats;
+
+/* ---- context ---- */
+int pppm_create(void** ctx, int32_t device);
+int pppm_destroy(void* ctx);
+const char* pppm_last_error(const void* ctx);
+int pppm_device_info(void* ctx, char* name, int32_t name_len, int32_t* cu_count,
+                     int64_t* hbm_bytes);
+
+/* ---- scene: replaces Scene::add / setAccelerator (sources/scene.h:38-56,
+ * sources/scene.cc:35-38) and QBVHAccel::construct (sources/qbvh_accel.h:49) ---- */
+int pppm_scene_upload(void* ctx, const float* tri_xyz /* [n][9] */, const int32_t* bsdf_id,
+                      int64_t n, const pppm_bsdf* table, int32_t nb, const pppm_light* lights,
+                      int32_t nl);
+/* GPU BVH build: LBVH over Morton codes collapsed to 4-wide 128-byte nodes. */
+int pppm_bvh_build(void* ctx, pppm_bvh_info* info);
+/* Batch closest hit: replaces Scene::intersect (sources/scene.cc:40-47) and
+ * QBVHAccel::intersect (sources/qbvh_accel.cc:205-285). */
+int pppm_intersect(void* ctx, const pppm_ray* rays, int64_t n, pppm_hit* hits);
+
+/* ---- hash grid: replaces HashGrid<Ty> (sources/hash_grid.h:8-39,
+ * sources/hash_grid_detail.h:26-72) ---- */
+/* Point insertion add(i, p_i, p_i), i = 0..n-1: sorted_idx[cell_start[h] ..
+ * cell_end[h]) is bucket h in insertion order (empty bucket: start = end =
+ * number of items in lower buckets). */
+int pppm_hashgrid_build(void* ctx, const float* pos /* [n][3] */, int64_t n, int32_t hash_size,
+                        double hash_scale, const double bbox_min[3], int32_t* sorted_idx,
+                        int32_t* cell_start, int32_t* cell_end);
+/* Box insertion add(i, box_min_i, box_max_i): every (item, cell) pair in the
+ * reference's z, y, x loop order.  *n_pairs returns the pair count; fails
+ * with PPPM_ERR_CAPACITY if it exceeds cap. */
+int pppm_hashgrid_build_boxes(void* ctx, const double* box_min /* [n][3] */,
+                              const double* box_max, int64_t n, int32_t hash_size,
+                              double hash_scale, const double bbox_min[3], int32_t* sorted_items,
+                              int64_t cap, int64_t* n_pairs, int32_t* cell_start,
+                              int32_t* cell_end);
+
+/* ---- radius schedule (sources/ppm_probabilistic.cc:94), FP64 on the host ---- */
+/* literal = 0: Knaus-Zwicker r2_{i+1} = r2_i (i+alpha)/(i+1); literal = 1: the
+ * reference's r_{t+1} = (t+1)/(t+alpha) r_t.  out[k] = r_{k+1}. */
+int pppm_radius_schedule(double r1, double alpha, int32_t n, int32_t literal, double* out);
+
+/* ---- renderer: replaces ProgressivePhotonMappingProb::render
+ * (sources/ppm_probabilistic.h:35, ppm_probabilistic.cc:24-121) split into
+ * begin / pass / read so the caller owns the pass loop ---- */
+int pppm_camera_set(void* ctx, const pppm_camera* cam);
+int pppm_render_begin(void* ctx, const pppm_render_params* params);
+/* Runs pass i = (passes done) + 1 with r_i from the schedule. */
+int pppm_render_pass(void* ctx, pppm_pass_stats* stats);
+/* FP64 image [height][width][3], row 0 at the top, averaged over the passes
+ * done: (sum - compensation) / passes. */
+int pppm_image_read(void* ctx, double* rgb, int64_t n_values);
+/* Test hooks for the last pass: photons in (photon index, bounce) order,
+ * key = photon index << 6 | bounce, and the hash grid over them. */
+int pppm_last_photons(void* ctx, float* pos, float* flux, float* normal, uint32_t* key,
+                      int64_t cap, int64_t* n);
+int pppm_last_grid(void* ctx, int32_t* sorted_idx, int32_t* cell_start, int32_t* cell_end,
+                   int32_t hash_size, double bbox_min[3], double* hash_scale);
+/* Hit points of the last pass: pos, oriented normal, weight, pixel id. */
+int pppm_last_hitpoints(void* ctx, float* pos, float* normal, float* weight, int32_t* pixel,
+                        int64_t cap, int64_t* n);
+/* Gather only: given photons and hit points, the per-hit-point radiance of
+ * the fixed-radius cone-filtered estimate (sources/ppm_probabilistic.cc:300-334). */
+int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const float* ph_nrm,
+                int64_t n_ph, const float* hit_pos, const float* hit_nrm, const float* hit_w,
+                int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
+                float* out_rgb, pppm_pass_stats* stats);
+
+/* ---- procedural scenes (no reference counterpart: the reference loads PLY
+ * meshes, sources/trimesh.cc:125-217) ---- */
+#define PPPM_SCENE_BOX 1        /* box + two icospheres */
+#define PPPM_SCENE_BOX_CAUSTIC 2/* box + displaced icospheres + mirror + glass */
+#define PPPM_SCENE_RANDOM 3     /* one scene of a batch: sized by param */
+#define PPPM_SCENE_TWO_ROOMS 4  /* occluded light through a slit */
+#define PPPM_SCENE_TWO_ROOMS_SSS 5
+
+typedef struct {
+    int64_t n_tris;
+    float* tri_xyz;
+    int32_t* bsdf_id;
+    int32_t n_bsdf;
+    pppm_bsdf* bsdfs;
+    int32_t n_lights;
+    pppm_light* lights;
+    pppm_camera camera;
+    double bbox_min[3];
+    double bbox_max[3];
+} pppm_scene_desc;
+
+/* frequency: icosphere subdivision frequency (20 f^2 triangles per sphere);
+ * 0 picks the documented default of the scene kind. */
+int pppm_scenegen(int32_t kind, uint64_t seed, int32_t frequency, int32_t width, int32_t height,
+                  pppm_scene_desc* out);
+void pppm_scenegen_free(pppm_scene_desc* desc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,995 @@
+/*
+ * oracle.c -- CPU restatement of the tatsy-pppm probabilistic PPM pass (P-KZ).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
+ * baseline timed by bench.py.  Never linked into the product.
+ *
+ * Function                      restates                                   pinned by
+ * ----------------------------  -----------------------------------------  -----------------------------------------
+ * orc_philox4x32_10             Philox-4x32-10 (Salmon et al. 2011)         Random123 known-answer vectors
+ * tri_intersect                 sources/triangle.cc:53-78                  closed forms; compiled reference (_ref)
+ * qbvh_build / qbvh_intersect   sources/qbvh_accel.cc:139-285              brute force (tests/test_trimesh.cc:62-117
+ *                                                                          strategy); compiled reference (_ref)
+ * orc_hash / orc_hashgrid_*     sources/hash_grid_detail.h:26-72           compiled reference HashGrid<int> (_ref);
+ *                                                                          brute-force bucket lists
+ * orc_radius_schedule           sources/ppm_probabilistic.cc:94 (literal)  Gamma-function closed form
+ *                               + Knaus-Zwicker 2011 shrink
+ * on_hemisphere                 sources/sampler.cc:17-32                   compiled reference (_ref); cosine moments
+ * total_reflection              sources/reflectance.h:7-34                 compiled reference (_ref); Fresnel 0.04
+ * bsdf_sample                   sources/brdf.cc:23-26, 49-52, 117-142      compiled reference (_ref)
+ * camera_ray                    sources/perspective_camera.cc:42-46        centre/corner closed forms
+ * orc_trace_photons             sources/ppm_probabilistic.cc:166-251       flux conservation, mean-store closed form
+ * orc_eye_pass                  sources/ppm_probabilistic.cc:253-387       mirror-chain / direct-emission closed form
+ * orc_gather                    sources/ppm_probabilistic.cc:343-377       brute force; uniform-density closed form
+ * orc_render                    sources/ppm_probabilistic.cc:67-164        furnace-style energy check; convergence
+ *
+ * Deliberate differences from the literal reference (each is a quirk the
+ * reference cannot run with): fixed-radius gather instead of the swapped kNN,
+ * Knaus-Zwicker radius shrink, area-light emission, photon/hit-point normal
+ * agreement test, counter-based RNG, ray origins offset by ray_eps along the
+ * surface normal.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* sources/common.h:12-18 */
+#define INFTY 1.0e20
+#define EPS 1.0e-12
+#define IOR_OBJECT 1.5
+#define IOR_VACCUM 1.0
+#define REFLECT_PROBABILITY 0.5
+static const double PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------------ */
+/* Random numbers                                                            */
+/* ------------------------------------------------------------------------ */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Stream of uniforms: key = seed, counter = (index, scene, pass, kind<<24 | block).
+ * Uniform = top 24 bits * 2^-24, exact in FP32 and FP64. */
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int pos;
+    uint32_t block;
+} rng_t;
+
+static void rng_init(rng_t* r, uint64_t seed, uint32_t kind, uint32_t pass, uint32_t index,
+                     uint32_t scene) {
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->ctr[0] = index;
+    r->ctr[1] = scene;
+    r->ctr[2] = pass;
+    r->ctr[3] = kind << 24;
+    r->pos = 4;
+    r->block = 0;
+}
+
+static double rng_at(rng_t* r, int dim) {
+    uint32_t blk = (uint32_t)dim >> 2;
+    if (r->pos == 4 || blk != r->block) {
+        uint32_t c[4] = {r->ctr[0], r->ctr[1], r->ctr[2], r->ctr[3] | blk};
+        orc_philox4x32_10(c, r->key, r->buf);
+        r->block = blk;
+        r->pos = 0;
+    }
+    return (double)(r->buf[dim & 3] >> 8) * (1.0 / 16777216.0);
+}
+
+void orc_uniforms(uint64_t seed, uint32_t kind, uint32_t pass, uint32_t index, uint32_t scene,
+                  int32_t n, double* out) {
+    rng_t r;
+    rng_init(&r, seed, kind, pass, index, scene);
+    for (int i = 0; i < n; i++) out[i] = rng_at(&r, i);
+}
+
+#define KIND_PHOTON 1u
+#define KIND_EYE 2u
+/* photon dims: 0 light pick, 1-2 position, 3-4 direction, 5-7 reserved,
+ * bounce b uses 8+3b .. 10+3b.  eye dims: 0-1 jitter, bounce b 2+3b .. 4+3b. */
+#define PH_DIM_BOUNCE 8
+#define EYE_DIM_BOUNCE 2
+
+/* ------------------------------------------------------------------------ */
+/* Vector3D (sources/vector3d.cc)                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { double x, y, z; } v3;
+static inline v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 scl(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+/* vector3d.cc:44-53 */
+static inline double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 u, v3 v) {
+    return V(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+static inline double norm(v3 a) { return sqrt(dot(a, a)); }
+/* vector3d.cc:63-65: divides by norm + EPS (multiplying by 1/(n+EPS), :144-150) */
+static inline v3 normalized(v3 a) { return scl(a, 1.0 / (norm(a) + EPS)); }
+/* vector3d.cc:67-69 */
+static inline v3 reflect(v3 v, v3 n) { return sub(v, scl(n, 2.0 * dot(n, v))); }
+static inline double vmax3(v3 a) { return fmax(a.x, fmax(a.y, a.z)); }
+static inline double vget(v3 a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+
+/* ------------------------------------------------------------------------ */
+/* Scene + QBVH                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float lo[3][4], hi[3][4];   /* childBoxes[min|max][x|y|z][4] (qbvh_accel.h:17) */
+    int32_t child[4];           /* node index, or -1 */
+    int32_t tri_start, tri_count; /* leaf range in tri_order */
+    int8_t sep[3];
+    int8_t is_leaf;
+} qnode;
+
+struct orc_scene {
+    int64_t n;
+    v3* p0; v3* p1; v3* p2;
+    int32_t* mat;
+    orc_bsdf* bsdf;
+    int32_t nb;
+    orc_light* lights;
+    int32_t nl;
+    double* light_cdf;
+    double light_total;
+    qnode* nodes;
+    int64_t n_nodes, cap_nodes;
+    int32_t* tri_order;
+    int32_t depth;
+    v3 bmin, bmax;
+};
+
+/* triangle.cc:53-78 (Moller-Trumbore, FP64).  Returns t or -1. */
+static double tri_intersect(const orc_scene* s, int64_t i, v3 o, v3 d) {
+    v3 e1 = sub(s->p1[i], s->p0[i]);
+    v3 e2 = sub(s->p2[i], s->p0[i]);
+    v3 pv = cross(d, e2);
+    double det = dot(e1, pv);
+    if (det > -EPS && det < EPS) return -1.0;
+    double invdet = 1.0 / det;
+    v3 tv = sub(o, s->p0[i]);
+    double u = dot(tv, pv) * invdet;
+    if (u < 0.0 || u > 1.0) return -1.0;
+    v3 qv = cross(tv, e1);
+    double v = dot(d, qv) * invdet;
+    if (v < 0.0 || u + v > 1.0) return -1.0;
+    double t = dot(e2, qv) * invdet;
+    return t > EPS ? t : -1.0;
+}
+
+/* triangle.cc:45-47 */
+static v3 tri_normal(const orc_scene* s, int64_t i) {
+    return normalized(cross(sub(s->p1[i], s->p0[i]), sub(s->p2[i], s->p0[i])));
+}
+
+static double* g_sort_key; /* centroid coordinate for the comparator */
+static int cmp_centroid(const void* a, const void* b) {
+    int32_t ia = *(const int32_t*)a, ib = *(const int32_t*)b;
+    double ka = g_sort_key[ia], kb = g_sort_key[ib];
+    if (ka < kb) return -1;
+    if (ka > kb) return 1;
+    return ia < ib ? -1 : (ia > ib ? 1 : 0); /* deterministic ties (std::sort's are unspecified) */
+}
+
+static double* g_centroid[3];
+
+static int64_t qnode_alloc(orc_scene* s) {
+    if (s->n_nodes == s->cap_nodes) {
+        s->cap_nodes = s->cap_nodes ? s->cap_nodes * 2 : 1024;
+        s->nodes = (qnode*)realloc(s->nodes, sizeof(qnode) * (size_t)s->cap_nodes);
+    }
+    memset(&s->nodes[s->n_nodes], 0, sizeof(qnode));
+    return s->n_nodes++;
+}
+
+static void range_box(const orc_scene* s, const int32_t* idx, int64_t n, v3* lo, v3* hi) {
+    *lo = V(INFTY, INFTY, INFTY);
+    *hi = V(-INFTY, -INFTY, -INFTY);
+    for (int64_t k = 0; k < n; k++) {
+        int32_t i = idx[k];
+        v3 ps[3] = {s->p0[i], s->p1[i], s->p2[i]};
+        for (int j = 0; j < 3; j++) {
+            lo->x = fmin(lo->x, ps[j].x); lo->y = fmin(lo->y, ps[j].y); lo->z = fmin(lo->z, ps[j].z);
+            hi->x = fmax(hi->x, ps[j].x); hi->y = fmax(hi->y, ps[j].y); hi->z = fmax(hi->z, ps[j].z);
+        }
+    }
+}
+
+/* qbvh_accel.cc:150-203: 4-way object-median split, <= 3 triangles per leaf. */
+static int64_t qbvh_build(orc_scene* s, int32_t* idx, int64_t n, int dim, int depth) {
+    int64_t me = qnode_alloc(s);
+    if (depth > s->depth) s->depth = depth;
+    if (n <= 3) {
+        s->nodes[me].is_leaf = 1;
+        s->nodes[me].tri_start = (int32_t)(idx - s->tri_order);
+        s->nodes[me].tri_count = (int32_t)n;
+        for (int c = 0; c < 4; c++) s->nodes[me].child[c] = -1;
+        return me;
+    }
+    g_sort_key = g_centroid[dim];
+    qsort(idx, (size_t)n, sizeof(int32_t), cmp_centroid);
+    int64_t mid = n / 2;
+    g_sort_key = g_centroid[(dim + 1) % 3];
+    qsort(idx, (size_t)mid, sizeof(int32_t), cmp_centroid);
+    qsort(idx + mid, (size_t)(n - mid), sizeof(int32_t), cmp_centroid);
+    int64_t b[5] = {0, mid / 2, mid, mid + mid / 2, n};
+    for (int c = 0; c < 4; c++) {
+        v3 lo, hi;
+        range_box(s, idx + b[c], b[c + 1] - b[c], &lo, &hi);
+        /* boxes stored as float (qbvh_accel.cc:182-183) */
+        s->nodes[me].lo[0][c] = (float)lo.x; s->nodes[me].lo[1][c] = (float)lo.y; s->nodes[me].lo[2][c] = (float)lo.z;
+        s->nodes[me].hi[0][c] = (float)hi.x; s->nodes[me].hi[1][c] = (float)hi.y; s->nodes[me].hi[2][c] = (float)hi.z;
+    }
+    for (int c = 0; c < 4; c++) {
+        int64_t ch = qbvh_build(s, idx + b[c], b[c + 1] - b[c], (dim + 2) % 3, depth + 1);
+        s->nodes[me].child[c] = (int32_t)ch;
+    }
+    s->nodes[me].sep[0] = (int8_t)dim;
+    s->nodes[me].sep[1] = (int8_t)((dim + 1) % 3);
+    s->nodes[me].sep[2] = (int8_t)((dim + 1) % 3);
+    s->nodes[me].is_leaf = 0;
+    return me;
+}
+
+/* qbvh_accel.cc:205-285.  Near-first child order by entry distance replaces
+ * the sign-bit order table (it only changes which of two equal-t hits wins). */
+static int32_t qbvh_intersect(const orc_scene* s, v3 o, v3 d, double tmin_ray, double tmax_ray,
+                              double* t_out) {
+    float org[3] = {(float)o.x, (float)o.y, (float)o.z};
+    float idir[3];
+    for (int k = 0; k < 3; k++) {
+        double dk = vget(d, k);
+        idir[k] = dk == 0.0 ? (float)1.0e20 : (float)(1.0 / dk);
+    }
+    int sgn[3] = {idir[0] > 0.0f ? 0 : 1, idir[1] > 0.0f ? 0 : 1, idir[2] > 0.0f ? 0 : 1};
+    double best = tmax_ray;
+    int32_t hit = -1;
+    int32_t stack[256];
+    int sp = 0;
+    if (s->n_nodes == 0) { *t_out = best; return -1; }
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const qnode* nd = &s->nodes[stack[--sp]];
+        if (nd->is_leaf) {
+            for (int k = 0; k < nd->tri_count; k++) {
+                int32_t ti = s->tri_order[nd->tri_start + k];
+                double t = tri_intersect(s, ti, o, d);
+                if (t > tmin_ray && best > t) { best = t; hit = ti; }
+            }
+            continue;
+        }
+        float tn[4], tf[4];
+        for (int c = 0; c < 4; c++) { tn[c] = 0.0f; tf[c] = (float)best; }
+        for (int k = 0; k < 3; k++) {
+            for (int c = 0; c < 4; c++) {
+                float a = ((sgn[k] ? nd->hi[k][c] : nd->lo[k][c]) - org[k]) * idir[k];
+                float b = ((sgn[k] ? nd->lo[k][c] : nd->hi[k][c]) - org[k]) * idir[k];
+                tn[c] = fmaxf(tn[c], a);
+                tf[c] = fminf(tf[c], b);
+            }
+        }
+        int order[4], m = 0;
+        for (int c = 0; c < 4; c++) if (tf[c] >= tn[c]) order[m++] = c;
+        /* push far-to-near so that the nearest is popped first */
+        for (int a = 1; a < m; a++)
+            for (int b = a; b > 0 && tn[order[b]] > tn[order[b - 1]]; b--) {
+                int tmp = order[b]; order[b] = order[b - 1]; order[b - 1] = tmp;
+            }
+        for (int c = 0; c < m; c++) stack[sp++] = nd->child[order[c]];
+    }
+    *t_out = best;
+    return hit;
+}
+
+static int32_t brute_intersect(const orc_scene* s, v3 o, v3 d, double tmin_ray, double tmax_ray,
+                               double* t_out) {
+    double best = tmax_ray;
+    int32_t hit = -1;
+    for (int64_t i = 0; i < s->n; i++) {
+        double t = tri_intersect(s, i, o, d);
+        if (t > tmin_ray && best > t) { best = t; hit = (int32_t)i; }
+    }
+    *t_out = best;
+    return hit;
+}
+
+orc_scene* orc_scene_create(const float* tri, const int32_t* bsdf_id, int64_t n,
+                            const orc_bsdf* table, int32_t nb, const orc_light* lights, int32_t nl) {
+    orc_scene* s = (orc_scene*)calloc(1, sizeof(orc_scene));
+    s->n = n;
+    s->p0 = (v3*)malloc(sizeof(v3) * (size_t)(n ? n : 1));
+    s->p1 = (v3*)malloc(sizeof(v3) * (size_t)(n ? n : 1));
+    s->p2 = (v3*)malloc(sizeof(v3) * (size_t)(n ? n : 1));
+    s->mat = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+    s->bmin = V(INFTY, INFTY, INFTY);
+    s->bmax = V(-INFTY, -INFTY, -INFTY);
+    for (int64_t i = 0; i < n; i++) {
+        const float* t = tri + 9 * i;
+        s->p0[i] = V(t[0], t[1], t[2]);
+        s->p1[i] = V(t[3], t[4], t[5]);
+        s->p2[i] = V(t[6], t[7], t[8]);
+        s->mat[i] = bsdf_id ? bsdf_id[i] : 0;
+        for (int j = 0; j < 9; j++) {
+            double c = t[j];
+            int d = j % 3;
+            if (d == 0) { s->bmin.x = fmin(s->bmin.x, c); s->bmax.x = fmax(s->bmax.x, c); }
+            if (d == 1) { s->bmin.y = fmin(s->bmin.y, c); s->bmax.y = fmax(s->bmax.y, c); }
+            if (d == 2) { s->bmin.z = fmin(s->bmin.z, c); s->bmax.z = fmax(s->bmax.z, c); }
+        }
+    }
+    s->nb = nb;
+    s->bsdf = (orc_bsdf*)malloc(sizeof(orc_bsdf) * (size_t)(nb ? nb : 1));
+    if (nb) memcpy(s->bsdf, table, sizeof(orc_bsdf) * (size_t)nb);
+    s->nl = nl;
+    s->lights = (orc_light*)malloc(sizeof(orc_light) * (size_t)(nl ? nl : 1));
+    s->light_cdf = (double*)malloc(sizeof(double) * (size_t)(nl + 1));
+    if (nl) memcpy(s->lights, lights, sizeof(orc_light) * (size_t)nl);
+    s->light_cdf[0] = 0.0;
+    for (int l = 0; l < nl; l++) {
+        const orc_light* L = &s->lights[l];
+        v3 eu = V(L->edge_u[0], L->edge_u[1], L->edge_u[2]);
+        v3 ev = V(L->edge_v[0], L->edge_v[1], L->edge_v[2]);
+        double area = norm(cross(eu, ev));
+        double lum = (L->emission[0] + (double)L->emission[1] + L->emission[2]) / 3.0;
+        s->light_cdf[l + 1] = s->light_cdf[l] + PI * area * lum;
+    }
+    s->light_total = s->light_cdf[nl];
+    /* QBVH build */
+    s->tri_order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+    for (int d = 0; d < 3; d++) g_centroid[d] = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    for (int64_t i = 0; i < n; i++) {
+        s->tri_order[i] = (int32_t)i;
+        /* Triangle::gravity, triangle.cc:49-51 */
+        v3 g = scl(add(add(s->p0[i], s->p1[i]), s->p2[i]), 1.0 / 3.0);
+        g_centroid[0][i] = g.x; g_centroid[1][i] = g.y; g_centroid[2][i] = g.z;
+    }
+    if (n > 0) qbvh_build(s, s->tri_order, n, 0, 1);
+    for (int d = 0; d < 3; d++) { free(g_centroid[d]); g_centroid[d] = NULL; }
+    return s;
+}
+
+void orc_scene_destroy(orc_scene* s) {
+    if (!s) return;
+    free(s->p0); free(s->p1); free(s->p2); free(s->mat); free(s->bsdf);
+    free(s->lights); free(s->light_cdf); free(s->nodes); free(s->tri_order);
+    free(s);
+}
+
+int32_t orc_qbvh_stats(const orc_scene* s, int64_t* nodes, int64_t* leaves, int32_t* depth) {
+    int64_t nl = 0;
+    for (int64_t i = 0; i < s->n_nodes; i++) nl += s->nodes[i].is_leaf;
+    if (nodes) *nodes = s->n_nodes;
+    if (leaves) *leaves = nl;
+    if (depth) *depth = s->depth;
+    return 0;
+}
+
+static int32_t scene_intersect(const orc_scene* s, v3 o, v3 d, double* t) {
+    return qbvh_intersect(s, o, d, 0.0, INFTY, t);
+}
+
+void orc_intersect_batch(const orc_scene* s, const float* rays, int64_t n, int32_t brute,
+                         float* t_out, int32_t* tri_out) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        v3 o = V(r[0], r[1], r[2]);
+        v3 d = V(r[4], r[5], r[6]);
+        double t;
+        int32_t h = brute ? brute_intersect(s, o, d, r[3], r[7], &t)
+                          : qbvh_intersect(s, o, d, r[3], r[7], &t);
+        tri_out[i] = h;
+        t_out[i] = h >= 0 ? (float)t : r[7];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* HashGrid (hash_grid_detail.h)                                             */
+/* ------------------------------------------------------------------------ */
+
+/* hash_grid_detail.h:60-63.  The int32 products wrap (two's complement), as
+ * they do on every platform the reference targets; done here in uint32. */
+uint32_t orc_hash(int32_t ix, int32_t iy, int32_t iz, int32_t hash_size) {
+    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    return h % (uint32_t)hash_size;
+}
+
+/* hash_grid_detail.h:35-43: |trunc((x - min) * scale)| in FP64 */
+static inline int32_t cell_coord(double x, double mn, double scale) {
+    double b = (x - mn) * scale;
+    int32_t c = (int32_t)b;
+    return c < 0 ? -c : c;
+}
+
+static void buckets_to_ranges(const uint32_t* h, int64_t n, int32_t S, int32_t* sorted,
+                              int32_t* cell_start, int32_t* cell_end) {
+    /* stable counting sort by bucket: bucket h holds items in insertion order */
+    int64_t* cnt = (int64_t*)calloc((size_t)S + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) cnt[h[i] + 1]++;
+    for (int32_t b = 0; b < S; b++) cnt[b + 1] += cnt[b];
+    for (int32_t b = 0; b < S; b++) { cell_start[b] = (int32_t)cnt[b]; cell_end[b] = (int32_t)cnt[b + 1]; }
+    int64_t* pos = (int64_t*)malloc(sizeof(int64_t) * ((size_t)S + 1));
+    memcpy(pos, cnt, sizeof(int64_t) * ((size_t)S + 1));
+    for (int64_t i = 0; i < n; i++) sorted[pos[h[i]]++] = (int32_t)i;
+    free(pos);
+    free(cnt);
+}
+
+int32_t orc_hashgrid_points(const float* pos, int64_t n, int32_t S, double scale,
+                            const double bmin[3], int32_t* sorted, int32_t* cell_start,
+                            int32_t* cell_end) {
+    if (S <= 0) return -1;
+    uint32_t* h = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    for (int64_t i = 0; i < n; i++) {
+        int32_t ix = cell_coord((double)pos[3 * i + 0], bmin[0], scale);
+        int32_t iy = cell_coord((double)pos[3 * i + 1], bmin[1], scale);
+        int32_t iz = cell_coord((double)pos[3 * i + 2], bmin[2], scale);
+        h[i] = orc_hash(ix, iy, iz, S);
+    }
+    buckets_to_ranges(h, n, S, sorted, cell_start, cell_end);
+    free(h);
+    return 0;
+}
+
+int64_t orc_hashgrid_boxes(const double* box_min, const double* box_max, int64_t n, int32_t S,
+                           double scale, const double bmin[3], int32_t* sorted_items, int64_t cap,
+                           int32_t* cell_start, int32_t* cell_end) {
+    if (S <= 0) return -1;
+    /* hash_grid_detail.h:44-50: z outer, y, x inner, inclusive bounds */
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int32_t lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = cell_coord(box_min[3 * i + d], bmin[d], scale);
+            hi[d] = cell_coord(box_max[3 * i + d], bmin[d], scale);
+        }
+        if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2])
+            total += (int64_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    }
+    if (total > cap) return -1;
+    uint32_t* h = (uint32_t*)calloc((size_t)(total ? total : 1), sizeof(uint32_t));
+    int32_t* item = (int32_t*)malloc(sizeof(int32_t) * (size_t)(total ? total : 1));
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int32_t lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = cell_coord(box_min[3 * i + d], bmin[d], scale);
+            hi[d] = cell_coord(box_max[3 * i + d], bmin[d], scale);
+        }
+        for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
+            for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
+                for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
+                    h[k] = orc_hash(ix, iy, iz, S);
+                    item[k] = (int32_t)i;
+                    k++;
+                }
+    }
+    int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (size_t)(total ? total : 1));
+    buckets_to_ranges(h, total, S, perm, cell_start, cell_end);
+    for (int64_t j = 0; j < total; j++) sorted_items[j] = item[perm[j]];
+    free(perm); free(item); free(h);
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Radius schedule                                                           */
+/* ------------------------------------------------------------------------ */
+
+void orc_radius_schedule(double r1, double alpha, int32_t n, int32_t literal, double* out) {
+    if (n <= 0) return;
+    out[0] = r1;
+    if (literal) {
+        /* ppm_probabilistic.cc:94: _radius = (t + 1.0) / (t + ALPHA) * _radius */
+        double r = r1;
+        for (int t = 1; t < n; t++) { r = (t + 1.0) / (t + alpha) * r; out[t] = r; }
+    } else {
+        /* Knaus & Zwicker 2011: r2_{i+1} = r2_i * ((i + alpha) / (i + 1)) */
+        double r2 = r1 * r1;
+        for (int i = 1; i < n; i++) { r2 = r2 * ((i + alpha) / (i + 1.0)); out[i] = sqrt(r2); }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sampling and BSDFs                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* sampler.cc:17-32 */
+static v3 on_hemisphere(v3 w, double r1, double r2) {
+    v3 u;
+    if (fabs(w.x) > EPS) u = normalized(cross(V(0.0, 1.0, 0.0), w));
+    else u = normalized(cross(V(1.0, 0.0, 0.0), w));
+    v3 v = cross(w, u);
+    double t = 2.0 * PI * r1;
+    double z2s = sqrt(r2);
+    return normalized(add(add(scl(u, cos(t) * z2s), scl(v, sin(t) * z2s)), scl(w, sqrt(1.0 - r2))));
+}
+
+/* reflectance.h:7-34 */
+static int total_reflection(int into, v3 in, v3 n, v3 on, v3* refl, v3* refr, double* fre,
+                            double* ftr) {
+    *refl = reflect(in, n);
+    double nnt = into ? IOR_VACCUM / IOR_OBJECT : IOR_OBJECT / IOR_VACCUM;
+    double ddn = dot(in, on);
+    double cos2t = 1.0 - nnt * nnt * (1.0 - ddn * ddn);
+    if (cos2t < 0.0) {
+        *refr = V(0, 0, 0);
+        *fre = 1.0;
+        *ftr = 0.0;
+        return 1;
+    }
+    *refr = normalized(sub(scl(in, nnt), scl(n, (into ? 1.0 : -1.0) * (ddn * nnt + sqrt(cos2t)))));
+    double a = IOR_OBJECT - IOR_VACCUM, b = IOR_OBJECT + IOR_VACCUM;
+    double R0 = (a * a) / (b * b);
+    double c = 1.0 - (into ? -ddn : dot(*refr, neg(on)));
+    *fre = R0 + (1.0 - R0) * (c * c * c * c * c);
+    *ftr = 1.0 - *fre;
+    return 0;
+}
+
+/* brdf.cc:23-26 (Lambert), 49-52 (mirror), 117-142 (refraction).  pdf keeps
+ * the caller's 1.0 unless the BSDF sets it. */
+static v3 bsdf_sample(const orc_bsdf* b, v3 in, v3 n, double r1, double r2, double* pdf) {
+    if (b->type & ORC_LAMBERTIAN) {
+        v3 on = dot(in, n) < 0.0 ? n : neg(n);
+        return on_hemisphere(on, r1, r2);
+    }
+    if (b->type & ORC_SPECULAR) {
+        v3 on = dot(in, n) < 0.0 ? n : neg(n);
+        return reflect(in, on);
+    }
+    /* ORC_REFRACTION */
+    int into = 1;
+    v3 on = n;
+    if (dot(in, n) >= 0.0) { into = 0; on = neg(n); }
+    v3 rd, td;
+    double fre, ftr;
+    if (total_reflection(into, in, n, on, &rd, &td, &fre, &ftr)) {
+        *pdf = 1.0;
+        return rd;
+    }
+    double rp = 0.25 + 0.5 * REFLECT_PROBABILITY;
+    if (r1 < rp) { *pdf = rp / fre; return rd; }
+    *pdf = (1.0 - rp) / ftr;
+    return td;
+}
+
+static inline v3 refl_of(const orc_bsdf* b) {
+    return V(b->reflectance[0], b->reflectance[1], b->reflectance[2]);
+}
+
+/* Origin of a continuation ray: the hit point moved by eps to the side of the
+ * surface the new direction leaves into. */
+static inline v3 spawn(v3 p, v3 ng, v3 dir, double eps) {
+    return dot(dir, ng) >= 0.0 ? add(p, scl(ng, eps)) : sub(p, scl(ng, eps));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Photon pass (ppm_probabilistic.cc:166-251, area-light emission)           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float p[3], f[3], n[3]; int32_t bounce; } deposit;
+
+/* Emission: pick a light by power, uniform point on the parallelogram,
+ * cosine direction about its normal (sampler.cc:17-32); flux = P_total / N. */
+static void emit_photon(const orc_scene* s, rng_t* r, int64_t N, v3* pos, v3* dir, v3* flux,
+                        v3* nl_out) {
+    double ul = rng_at(r, 0);
+    int l = 0;
+    double target = ul * s->light_total;
+    while (l < s->nl - 1 && s->light_cdf[l + 1] <= target) l++;
+    const orc_light* L = &s->lights[l];
+    v3 c = V(L->corner[0], L->corner[1], L->corner[2]);
+    v3 eu = V(L->edge_u[0], L->edge_u[1], L->edge_u[2]);
+    v3 ev = V(L->edge_v[0], L->edge_v[1], L->edge_v[2]);
+    double u1 = rng_at(r, 1), u2 = rng_at(r, 2);
+    *pos = add(c, add(scl(eu, u1), scl(ev, u2)));
+    v3 nl = normalized(cross(eu, ev));
+    *nl_out = nl;
+    *dir = on_hemisphere(nl, rng_at(r, 3), rng_at(r, 4));
+    double pl = (s->light_cdf[l + 1] - s->light_cdf[l]) / s->light_total;
+    double area = norm(cross(eu, ev));
+    double k = PI * area / (pl * (double)N);
+    *flux = V(L->emission[0] * k, L->emission[1] * k, L->emission[2] * k);
+}
+
+static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, int64_t idx,
+                            deposit* out, int cap_out) {
+    rng_t r;
+    rng_init(&r, P->seed, KIND_PHOTON, (uint32_t)P->pass_index, (uint32_t)idx, P->scene_id);
+    v3 pos, dir, flux, nl;
+    emit_photon(s, &r, P->photons, &pos, &dir, &flux, &nl);
+    v3 o = add(pos, scl(nl, P->ray_eps));
+    int nd = 0;
+    for (int bounce = 0;; bounce++) {
+        if (bounce >= P->max_bounces || vmax3(flux) <= 0.0) break;
+        double t;
+        int32_t tri = scene_intersect(s, o, dir, &t);
+        if (tri < 0) break;
+        int dd = PH_DIM_BOUNCE + 3 * bounce;
+        double r0 = rng_at(&r, dd), r1 = rng_at(&r, dd + 1), r2 = rng_at(&r, dd + 2);
+        const orc_bsdf* b = &s->bsdf[s->mat[tri]];
+        v3 hp = add(o, scl(dir, t));
+        v3 ng = tri_normal(s, tri);
+        v3 on = dot(ng, dir) < 0.0 ? ng : neg(ng);
+        if (b->type & ORC_EMITTER) break;
+        if (b->type & ORC_LAMBERTIAN) {
+            if (nd < cap_out) {
+                deposit* d = &out[nd];
+                d->p[0] = (float)hp.x; d->p[1] = (float)hp.y; d->p[2] = (float)hp.z;
+                d->f[0] = (float)flux.x; d->f[1] = (float)flux.y; d->f[2] = (float)flux.z;
+                d->n[0] = (float)on.x; d->n[1] = (float)on.y; d->n[2] = (float)on.z;
+                d->bounce = bounce;
+            }
+            nd++;
+            v3 rho = refl_of(b);
+            double prob = (rho.x + rho.y + rho.z) / 3.0;
+            if (r0 < prob) {
+                double pdf = 1.0;
+                v3 nd_dir = bsdf_sample(b, dir, ng, r1, r2, &pdf);
+                o = spawn(hp, ng, nd_dir, P->ray_eps);
+                dir = nd_dir;
+                flux = scl(mul(flux, rho), 1.0 / prob);
+            } else {
+                break;
+            }
+        } else {
+            double pdf = 1.0;
+            /* the unoriented normal, as the eye pass passes it (ppm_probabilistic.cc:338);
+             * the photon pass's oriented normal (:188) would make every glass hit "into" */
+            v3 nd_dir = bsdf_sample(b, dir, ng, r0, r1, &pdf);
+            o = spawn(hp, ng, nd_dir, P->ray_eps);
+            dir = nd_dir;
+            flux = scl(mul(flux, refl_of(b)), 1.0 / pdf);
+        }
+    }
+    return nd;
+}
+
+#define MAX_DEP 64
+
+int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* P, float* pos, float* flux,
+                          float* nrm, uint32_t* key, int64_t cap) {
+    int64_t N = P->photons;
+    int32_t* counts = (int32_t*)calloc((size_t)(N + 1), sizeof(int32_t));
+    deposit* tmp = (deposit*)malloc(sizeof(deposit) * (size_t)(N ? N : 1) * 4);
+    int64_t tmp_cap = N * 4;
+    /* first pass: count and keep up to 4 deposits per photon; retrace the rest */
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < N; i++) {
+        deposit buf[MAX_DEP];
+        int nd = trace_one_photon(s, P, i, buf, MAX_DEP);
+        counts[i + 1] = nd;
+        for (int k = 0; k < nd && k < 4; k++) tmp[i * 4 + k] = buf[k];
+    }
+    (void)tmp_cap;
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
+    off[0] = 0;
+    for (int64_t i = 0; i < N; i++) off[i + 1] = off[i] + counts[i + 1];
+    int64_t total = off[N];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < N; i++) {
+        int nd = counts[i + 1];
+        deposit buf[MAX_DEP];
+        const deposit* src = &tmp[i * 4];
+        if (nd > 4) { trace_one_photon(s, P, i, buf, MAX_DEP); src = buf; }
+        for (int k = 0; k < nd && k < MAX_DEP; k++) {
+            int64_t j = off[i] + k;
+            if (j >= cap) break;
+            memcpy(pos + 3 * j, src[k].p, 12);
+            memcpy(flux + 3 * j, src[k].f, 12);
+            memcpy(nrm + 3 * j, src[k].n, 12);
+            if (key) key[j] = ((uint32_t)i << 6) | (uint32_t)src[k].bounce;
+        }
+    }
+    free(off); free(tmp); free(counts);
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eye pass (ppm_probabilistic.cc:253-344)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* perspective_camera.cc:42-46 with an orthonormal (U, V) basis and the field
+ * of view in degrees; pixel (x, row) with row 0 at the top of the image. */
+static void camera_ray(const orc_camera* c, double px, double py_up, v3* o, v3* d) {
+    v3 D = normalized(V(c->direction[0], c->direction[1], c->direction[2]));
+    v3 up = V(c->up[0], c->up[1], c->up[2]);
+    v3 U = normalized(cross(D, up));
+    v3 Vv = cross(U, D);
+    double th = tan(c->fov_y_deg * 0.5 * PI / 180.0);
+    double aspect = (double)c->width / (double)c->height;
+    v3 dir = add(D, add(scl(U, th * aspect * (2.0 * px / c->width - 1.0)),
+                        scl(Vv, th * (2.0 * py_up / c->height - 1.0))));
+    *o = V(c->center[0], c->center[1], c->center[2]);
+    *d = normalized(dir);
+}
+
+/* Returns 1 with a hit point, 0 with a direct contribution in *direct. */
+static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P,
+                    int64_t pix, v3* hpos, v3* hnrm, v3* hw, v3* direct) {
+    int W = cam->width, H = cam->height;
+    int x = (int)(pix % W), row = (int)(pix / W);
+    rng_t r;
+    rng_init(&r, P->seed, KIND_EYE, (uint32_t)P->pass_index, (uint32_t)pix, P->scene_id);
+    double px = x + rng_at(&r, 0);
+    double py = (H - 1 - row) + rng_at(&r, 1);
+    v3 o, d;
+    camera_ray(cam, px, py, &o, &d);
+    v3 thr = V(1, 1, 1);
+    *direct = V(0, 0, 0);
+    for (int bounce = 0;; bounce++) {
+        double t;
+        int32_t tri = bounce >= P->max_bounces ? -1 : scene_intersect(s, o, d, &t);
+        if (tri < 0) return 0; /* black environment */
+        int dd = EYE_DIM_BOUNCE + 3 * bounce;
+        double r0 = rng_at(&r, dd), r1 = rng_at(&r, dd + 1), r2 = rng_at(&r, dd + 2);
+        const orc_bsdf* b = &s->bsdf[s->mat[tri]];
+        v3 hp = add(o, scl(d, t));
+        v3 ng = tri_normal(s, tri);
+        if (b->type & ORC_EMITTER) {
+            if (dot(d, ng) < 0.0)
+                *direct = mul(thr, V(b->emission[0], b->emission[1], b->emission[2]));
+            return 0;
+        }
+        v3 rho = refl_of(b);
+        double roulette = vmax3(rho);
+        if (bounce >= 3) {
+            if (r0 > roulette) return 0;
+        } else {
+            roulette = 1.0;
+        }
+        v3 on = dot(ng, d) < 0.0 ? ng : neg(ng);
+        if (b->type & ORC_LAMBERTIAN) {
+            *hpos = hp;
+            *hnrm = on;
+            *hw = scl(mul(thr, rho), 1.0 / (PI * roulette));
+            return 1;
+        }
+        double pdf = 1.0;
+        v3 nd = bsdf_sample(b, d, ng, r1, r2, &pdf);
+        thr = scl(mul(thr, rho), 1.0 / (pdf * roulette));
+        o = spawn(hp, ng, nd, P->ray_eps);
+        d = nd;
+    }
+}
+
+void orc_eye_pass(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P,
+                  int32_t* has_hit, float* hit_pos, float* hit_nrm, float* hit_w,
+                  double* direct_rgb) {
+    int64_t np = (int64_t)cam->width * cam->height;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < np; i++) {
+        v3 hp = V(0, 0, 0), hn = V(0, 0, 0), hw = V(0, 0, 0), dr;
+        int h = eye_path(s, cam, P, i, &hp, &hn, &hw, &dr);
+        has_hit[i] = h;
+        hit_pos[3 * i] = (float)hp.x; hit_pos[3 * i + 1] = (float)hp.y; hit_pos[3 * i + 2] = (float)hp.z;
+        hit_nrm[3 * i] = (float)hn.x; hit_nrm[3 * i + 1] = (float)hn.y; hit_nrm[3 * i + 2] = (float)hn.z;
+        hit_w[3 * i] = (float)hw.x; hit_w[3 * i + 1] = (float)hw.y; hit_w[3 * i + 2] = (float)hw.z;
+        direct_rgb[3 * i] = dr.x; direct_rgb[3 * i + 1] = dr.y; direct_rgb[3 * i + 2] = dr.z;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gather (P-KZ form of ppm_probabilistic.cc:343-377)                        */
+/* ------------------------------------------------------------------------ */
+
+#define CONE_K 1.1
+
+static void gather_one(const float* ph_pos, const float* ph_flux, const float* ph_nrm,
+                       const int32_t* sorted, const int32_t* cs, const int32_t* ce, int32_t S,
+                       double scale, const double bmin[3], double r, v3 x, v3 nx, v3 w,
+                       double* out, int64_t* cnt) {
+    int32_t lo[3], hi[3];
+    double xs[3] = {x.x, x.y, x.z};
+    for (int d = 0; d < 3; d++) {
+        lo[d] = cell_coord(xs[d] - r, bmin[d], scale);
+        hi[d] = cell_coord(xs[d] + r, bmin[d], scale);
+    }
+    uint32_t seen[8];
+    int ns = 0;
+    double r2 = r * r;
+    v3 acc = V(0, 0, 0);
+    int64_t nb = 0, nc = 0, ni = 0;
+    for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
+        for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
+            for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
+                uint32_t h = orc_hash(ix, iy, iz, S);
+                int dup = 0;
+                for (int k = 0; k < ns; k++) if (seen[k] == h) dup = 1;
+                if (dup) continue;
+                if (ns < 8) seen[ns++] = h;
+                nb++;
+                for (int32_t j = cs[h]; j < ce[h]; j++) {
+                    int32_t pi = sorted[j];
+                    nc++;
+                    v3 p = V(ph_pos[3 * pi], ph_pos[3 * pi + 1], ph_pos[3 * pi + 2]);
+                    v3 df = sub(x, p);
+                    double d2 = dot(df, df);
+                    if (d2 > r2) continue;
+                    v3 np = V(ph_nrm[3 * pi], ph_nrm[3 * pi + 1], ph_nrm[3 * pi + 2]);
+                    if (dot(np, nx) <= 0.0) continue;
+                    ni++;
+                    double wt = 1.0 - sqrt(d2) / (CONE_K * r);
+                    acc = add(acc, scl(V(ph_flux[3 * pi], ph_flux[3 * pi + 1], ph_flux[3 * pi + 2]), wt));
+                }
+            }
+    double norm_ = PI * r2 * (1.0 - 2.0 / (3.0 * CONE_K));
+    out[0] = w.x * acc.x / norm_;
+    out[1] = w.y * acc.y / norm_;
+    out[2] = w.z * acc.z / norm_;
+    if (cnt) { cnt[0] = nb; cnt[1] = nc; cnt[2] = ni; }
+}
+
+void orc_gather(const float* ph_pos, const float* ph_flux, const float* ph_nrm, int64_t n_ph,
+                const float* hit_pos, const float* hit_nrm, const float* hit_w, int64_t n_hit,
+                double radius, int32_t S, const double bmin[3], double* out, int64_t* counters) {
+    double scale = 1.0 / (2.0 * radius);
+    int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_ph ? n_ph : 1));
+    int32_t* cs = (int32_t*)malloc(sizeof(int32_t) * (size_t)S);
+    int32_t* ce = (int32_t*)malloc(sizeof(int32_t) * (size_t)S);
+    orc_hashgrid_points(ph_pos, n_ph, S, scale, bmin, sorted, cs, ce);
+    int64_t tb = 0, tc = 0, ti = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : tb, tc, ti)
+    for (int64_t i = 0; i < n_hit; i++) {
+        v3 x = V(hit_pos[3 * i], hit_pos[3 * i + 1], hit_pos[3 * i + 2]);
+        v3 nx = V(hit_nrm[3 * i], hit_nrm[3 * i + 1], hit_nrm[3 * i + 2]);
+        v3 w = V(hit_w[3 * i], hit_w[3 * i + 1], hit_w[3 * i + 2]);
+        int64_t c[3];
+        gather_one(ph_pos, ph_flux, ph_nrm, sorted, cs, ce, S, scale, bmin, radius, x, nx, w,
+                   out + 3 * i, c);
+        tb += c[0]; tc += c[1]; ti += c[2];
+    }
+    if (counters) { counters[0] = tb; counters[1] = tc; counters[2] = ti; }
+    free(sorted); free(cs); free(ce);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pass loop (ppm_probabilistic.cc:67-164)                                   */
+/* ------------------------------------------------------------------------ */
+
+int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
+                   int32_t first_pass, double r1, double alpha, double ray_eps, uint64_t seed,
+                   uint32_t scene_id, int32_t hash_size, double* image, int64_t* stats) {
+    int64_t np = (int64_t)cam->width * cam->height;
+    double* radii = (double*)malloc(sizeof(double) * (size_t)(first_pass + n_passes));
+    orc_radius_schedule(r1, alpha, first_pass + n_passes, 0, radii);
+    double bmin[3] = {s->bmin.x - r1, s->bmin.y - r1, s->bmin.z - r1};
+    int64_t cap = photons * 8 + 1024;
+    float* pp = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
+    float* pf = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
+    float* pn = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
+    int32_t* has = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+    float* hp = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    float* hn = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    float* hw = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    double* dr = (double*)malloc(sizeof(double) * 3 * (size_t)np);
+    double* g = (double*)malloc(sizeof(double) * 3 * (size_t)np);
+    int64_t* hidx = (int64_t*)malloc(sizeof(int64_t) * (size_t)np);
+    float* cp = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    float* cn = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    float* cw = (float*)malloc(sizeof(float) * 3 * (size_t)np);
+    double* sum = (double*)calloc(3 * (size_t)np, sizeof(double));
+    int64_t tot_dep = 0, tot_hit = 0;
+    int32_t rc = 0;
+    for (int32_t k = 0; k < n_passes; k++) {
+        int32_t pass = first_pass + k;
+        orc_pass_params P;
+        memset(&P, 0, sizeof(P));
+        P.photons = photons;
+        P.pass_index = pass;
+        P.max_bounces = 64;
+        P.radius = radii[pass - 1];
+        P.ray_eps = ray_eps;
+        P.seed = seed;
+        P.scene_id = scene_id;
+        int64_t nd = orc_trace_photons(s, &P, pp, pf, pn, NULL, cap);
+        if (nd > cap) { rc = -1; break; }
+        tot_dep += nd;
+        orc_eye_pass(s, cam, &P, has, hp, hn, hw, dr);
+        int64_t nh = 0;
+        for (int64_t i = 0; i < np; i++)
+            if (has[i]) {
+                hidx[nh] = i;
+                memcpy(cp + 3 * nh, hp + 3 * i, 12);
+                memcpy(cn + 3 * nh, hn + 3 * i, 12);
+                memcpy(cw + 3 * nh, hw + 3 * i, 12);
+                nh++;
+            }
+        tot_hit += nh;
+        int32_t S = hash_size > 0 ? hash_size : (int32_t)(nd > 0 ? nd : 1);
+        orc_gather(pp, pf, pn, nd, cp, cn, cw, nh, P.radius, S, bmin, g, NULL);
+        for (int64_t i = 0; i < np; i++)
+            for (int c = 0; c < 3; c++) sum[3 * i + c] += dr[3 * i + c];
+        for (int64_t j = 0; j < nh; j++)
+            for (int c = 0; c < 3; c++) sum[3 * hidx[j] + c] += g[3 * j + c];
+    }
+    for (int64_t i = 0; i < 3 * np; i++) image[i] = sum[i] / (double)n_passes;
+    if (stats) { stats[0] = tot_dep; stats[1] = tot_hit; }
+    free(radii); free(pp); free(pf); free(pn); free(has); free(hp); free(hn); free(hw);
+    free(dr); free(g); free(hidx); free(cp); free(cn); free(cw); free(sum);
+    return rc;
+}
+
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exported single-sample entry points (pinned against the compiled          */
+/* reference in tests)                                                       */
+/* ------------------------------------------------------------------------ */
+
+void orc_on_hemisphere(const double n[3], double r1, double r2, double out[3]) {
+    v3 d = on_hemisphere(V(n[0], n[1], n[2]), r1, r2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+int32_t orc_total_reflection(int32_t into, const double in[3], const double n[3], const double on[3],
+                             double refl[3], double refr[3], double* fre, double* ftr) {
+    v3 rd, td;
+    int r = total_reflection(into, V(in[0], in[1], in[2]), V(n[0], n[1], n[2]),
+                             V(on[0], on[1], on[2]), &rd, &td, fre, ftr);
+    refl[0] = rd.x; refl[1] = rd.y; refl[2] = rd.z;
+    refr[0] = td.x; refr[1] = td.y; refr[2] = td.z;
+    return r;
+}
+
+void orc_bsdf_sample(int32_t type, const double rho[3], const double in[3], const double n[3],
+                     double r1, double r2, double out[3], double* pdf) {
+    orc_bsdf b;
+    memset(&b, 0, sizeof(b));
+    b.type = type;
+    b.reflectance[0] = (float)rho[0]; b.reflectance[1] = (float)rho[1]; b.reflectance[2] = (float)rho[2];
+    double p = 1.0;
+    v3 d = bsdf_sample(&b, V(in[0], in[1], in[2]), V(n[0], n[1], n[2]), r1, r2, &p);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+    *pdf = p;
+}
+
+void orc_camera_ray(const orc_camera* c, double px, double py_up, double o[3], double d[3]) {
+    v3 oo, dd;
+    camera_ray(c, px, py_up, &oo, &dd);
+    o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+}
+
+int32_t orc_tri_intersect(const double tri[9], const double o[3], const double d[3], double* t) {
+    orc_scene s;
+    memset(&s, 0, sizeof(s));
+    v3 p0 = V(tri[0], tri[1], tri[2]), p1 = V(tri[3], tri[4], tri[5]), p2 = V(tri[6], tri[7], tri[8]);
+    s.n = 1; s.p0 = &p0; s.p1 = &p1; s.p2 = &p2;
+    double tt = tri_intersect(&s, 0, V(o[0], o[1], o[2]), V(d[0], d[1], d[2]));
+    *t = tt;
+    return tt > 0.0 ? 1 : 0;
+}

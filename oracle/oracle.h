@@ -1,0 +1,161 @@
+/*
+ * oracle.h -- CPU restatement of the tatsy-pppm probabilistic PPM pass.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation and the CPU baseline that bench.py times.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path never links, loads or calls it.
+ *
+ * Arithmetic is FP64 (as in the reference's Vector3D, sources/vector3d.h),
+ * inputs are FP32 geometry widened exactly to FP64, and random numbers come
+ * from the same counter-based generator as the GPU path, so both paths draw
+ * identical uniforms for the same (seed, pass, stream, index, dimension).
+ *
+ * Each function cites the reference file:line it restates.  Parity status of
+ * every function is listed in the header of oracle.c.
+ */
+#ifndef PPPM_ORACLE_H
+#define PPPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material flags: the reference BsdfType bits (sources/bsdf.h:19-26) plus an
+ * emitter bit for the area light (the reference has no emissive material). */
+#define ORC_LAMBERTIAN 0x01
+#define ORC_SPECULAR   0x02
+#define ORC_PHONG      0x04
+#define ORC_REFRACTION 0x08
+#define ORC_BSSRDF     0x10
+#define ORC_EMITTER    0x40
+
+/* Layout-identical to pppm_bsdf in include/pppm.h (64 bytes). */
+typedef struct {
+    int32_t type;
+    float reflectance[3];
+    float emission[3];
+    float ior;
+    float sigma_a[3];
+    float sigmap_s[3];
+    float eta;
+    float pad;
+} orc_bsdf;
+
+/* Layout-identical to pppm_light: parallelogram corner + two edges. */
+typedef struct {
+    float corner[3];
+    float edge_u[3];
+    float edge_v[3];
+    float emission[3];
+} orc_light;
+
+/* Layout-identical to pppm_camera. */
+typedef struct {
+    double center[3];
+    double direction[3];
+    double up[3];
+    double fov_y_deg;
+    int32_t width;
+    int32_t height;
+} orc_camera;
+
+typedef struct orc_scene orc_scene;
+
+/* ---- random numbers (Philox-4x32-10, Salmon et al. SC'11) ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* n uniforms of the stream (seed, stream kind, pass, index, scene), dims 0..n-1 */
+void orc_uniforms(uint64_t seed, uint32_t kind, uint32_t pass, uint32_t index,
+                  uint32_t scene, int32_t n, double* out);
+
+/* ---- scene + QBVH (sources/scene.cc, sources/qbvh_accel.cc) ---- */
+orc_scene* orc_scene_create(const float* tri_xyz, const int32_t* bsdf_id, int64_t n,
+                            const orc_bsdf* table, int32_t nb,
+                            const orc_light* lights, int32_t nl);
+void orc_scene_destroy(orc_scene* s);
+/* rays: [n][8] = ox oy oz tmin dx dy dz tmax (FP32, as pppm_ray).  brute != 0
+ * intersects every triangle (the reference test's ground truth). */
+void orc_intersect_batch(const orc_scene* s, const float* rays, int64_t n, int32_t brute,
+                         float* t_out, int32_t* tri_out);
+int32_t orc_qbvh_stats(const orc_scene* s, int64_t* nodes, int64_t* leaves, int32_t* depth);
+
+/* ---- HashGrid<int> semantics (sources/hash_grid_detail.h) ---- */
+uint32_t orc_hash(int32_t ix, int32_t iy, int32_t iz, int32_t hash_size);
+/* Point insertion: add(i, p_i, p_i) for i = 0..n-1, then the concatenation
+ * of buckets 0..S-1 into sorted_idx with [cell_start, cell_end) ranges. */
+int32_t orc_hashgrid_points(const float* pos, int64_t n, int32_t hash_size, double hash_scale,
+                            const double bbox_min[3], int32_t* sorted_idx,
+                            int32_t* cell_start, int32_t* cell_end);
+/* Box insertion add(i, bmin_i, bmax_i); returns the number of (item, cell)
+ * pairs, or -1 if it exceeds cap. */
+int64_t orc_hashgrid_boxes(const double* box_min, const double* box_max, int64_t n,
+                           int32_t hash_size, double hash_scale, const double bbox_min[3],
+                           int32_t* sorted_items, int64_t cap,
+                           int32_t* cell_start, int32_t* cell_end);
+
+/* ---- radius schedule (sources/ppm_probabilistic.cc:94) ---- */
+/* literal=0: Knaus-Zwicker r2_{i+1} = r2_i (i+alpha)/(i+1);
+ * literal=1: the reference's r_{t+1} = (t+1)/(t+alpha) r_t.  out[i] = r_{i+1}. */
+void orc_radius_schedule(double r1, double alpha, int32_t n, int32_t literal, double* out);
+
+/* ---- PPM pass pieces (sources/ppm_probabilistic.cc) ---- */
+typedef struct {
+    int64_t photons;        /* emitted photons this pass */
+    int32_t pass_index;     /* 1-based pass number i */
+    int32_t max_bounces;    /* reference bounceLimit = 64 */
+    double radius;          /* r_i */
+    double ray_eps;         /* spawn offset along the surface normal */
+    uint64_t seed;
+    uint32_t scene_id;
+    int32_t pad;
+} orc_pass_params;
+
+/* Photon pass. Deposits in (photon index, bounce) order.  pos/flux/normal are
+ * [cap][3], key (may be NULL) = photon index << 6 | bounce.  Returns the
+ * deposit count (may exceed cap; then only cap are written). */
+int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* p,
+                          float* pos, float* flux, float* normal, uint32_t* key, int64_t cap);
+
+/* Eye pass: one path per pixel.  For each pixel writes either a hit point
+ * (has_hit=1: pos, oriented normal, weight = throughput * rho / pi) or a
+ * direct contribution (emission seen through specular chains). */
+void orc_eye_pass(const orc_scene* s, const orc_camera* cam, const orc_pass_params* p,
+                  int32_t* has_hit, float* hit_pos, float* hit_nrm, float* hit_w,
+                  double* direct_rgb);
+
+/* Fixed-radius cone-filtered gather (P-KZ form of ppm_probabilistic.cc:300-334)
+ * of n_hit hit points over n_ph photons through the hash grid. out_rgb [n_hit][3].
+ * counters (may be NULL): [0] buckets visited, [1] candidates, [2] in radius. */
+void orc_gather(const float* ph_pos, const float* ph_flux, const float* ph_nrm, int64_t n_ph,
+                const float* hit_pos, const float* hit_nrm, const float* hit_w, int64_t n_hit,
+                double radius, int32_t hash_size, const double bbox_min[3],
+                double* out_rgb, int64_t* counters);
+
+/* Full P-KZ render: passes 1..n_passes, radius r1 shrinking by alpha, FP64
+ * image (top row first) averaged over passes.  stats (may be NULL):
+ * [0] total deposits, [1] total hit points. */
+int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons_per_pass,
+                   int32_t n_passes, int32_t first_pass, double r1, double alpha,
+                   double ray_eps, uint64_t seed, uint32_t scene_id, int32_t hash_size,
+                   double* image, int64_t* stats);
+
+/* Single-sample pieces, exported to pin them against the compiled reference:
+ * sampler.cc:17-32, reflectance.h:7-34, brdf.cc, perspective_camera.cc:42-46,
+ * triangle.cc:53-78. */
+void orc_on_hemisphere(const double n[3], double r1, double r2, double out[3]);
+int32_t orc_total_reflection(int32_t into, const double in[3], const double n[3], const double on[3],
+                             double refl[3], double refr[3], double* fre, double* ftr);
+void orc_bsdf_sample(int32_t type, const double rho[3], const double in[3], const double n[3],
+                     double r1, double r2, double out[3], double* pdf);
+void orc_camera_ray(const orc_camera* c, double px, double py_up, double o[3], double d[3]);
+int32_t orc_tri_intersect(const double tri[9], const double o[3], const double d[3], double* t);
+
+/* Threads the OpenMP runtime will use. */
+int32_t orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,218 @@
+"""ctypes binding of the oracle (liboracle.so) and of the compiled reference
+(_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_P = C.c_void_p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class PassParams(C.Structure):
+    _fields_ = [("photons", C.c_int64), ("pass_index", C.c_int32), ("max_bounces", C.c_int32),
+                ("radius", C.c_double), ("ray_eps", C.c_double), ("seed", C.c_uint64),
+                ("scene_id", C.c_uint32), ("pad", C.c_int32)]
+
+
+_ODECL = {
+    "orc_philox4x32_10": (None, [_P, _P, _P]),
+    "orc_uniforms": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _P]),
+    "orc_scene_create": (_P, [_P, _P, C.c_int64, _P, C.c_int32, _P, C.c_int32]),
+    "orc_scene_destroy": (None, [_P]),
+    "orc_intersect_batch": (None, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
+    "orc_qbvh_stats": (C.c_int32, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "orc_hash": (C.c_uint32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "orc_hashgrid_points": (C.c_int32, [_P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
+    "orc_hashgrid_boxes": (C.c_int64, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, C.c_int64, _P, _P]),
+    "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_int32, C.c_int32, _P]),
+    "orc_trace_photons": (C.c_int64, [_P, C.POINTER(PassParams), _P, _P, _P, _P, C.c_int64]),
+    "orc_eye_pass": (None, [_P, _P, C.POINTER(PassParams), _P, _P, _P, _P, _P]),
+    "orc_gather": (None, [_P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double, C.c_int32, _P, _P, _P]),
+    "orc_render": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                               C.c_double, C.c_uint64, C.c_uint32, C.c_int32, _P, _P]),
+    "orc_num_threads": (C.c_int32, []),
+    "orc_on_hemisphere": (None, [_P, C.c_double, C.c_double, _P]),
+    "orc_total_reflection": (C.c_int32, [C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "orc_bsdf_sample": (None, [C.c_int32, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
+    "orc_camera_ray": (None, [_P, C.c_double, C.c_double, _P, _P]),
+    "orc_tri_intersect": (C.c_int32, [_P, _P, _P, _P]),
+}
+
+_RDECL = {
+    "ref_hashgrid_point_buckets": (C.c_int64, [_P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, C.c_int64, _P]),
+    "ref_hashgrid_box_buckets": (C.c_int64, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P,
+                                             C.c_int64, _P, C.c_int64, _P]),
+    "ref_qbvh_intersect": (None, [_P, C.c_int64, _P, C.c_int64, _P, _P]),
+    "ref_triangle_intersect": (C.c_int32, [_P, _P, _P, _P, _P]),
+    "ref_on_hemisphere": (None, [_P, C.c_double, C.c_double, _P]),
+    "ref_total_reflection": (C.c_int32, [C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "ref_bsdf_sample": (None, [C.c_int32, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
+    "ref_camera_ray": (None, [_P, _P, _P, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P]),
+    "ref_dipole": (None, [_P, _P, C.c_double, C.c_double, _P, _P]),
+}
+
+
+def _load(path, decl):
+    lib = C.CDLL(path)
+    for name, (res, args) in decl.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_orc = None
+_ref = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        _orc = _load(ORACLE_PATH, _ODECL)
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _load(REF_PATH, _RDECL)
+    return _ref
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    orc().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def uniforms(seed, kind, pass_index, index, scene, n):
+    out = np.zeros(n, np.float64)
+    orc().orc_uniforms(seed, kind, pass_index, index, scene, n, _ptr(out))
+    return out
+
+
+class OracleScene:
+    """The oracle's copy of a scene (FP64 triangles + restated QBVH)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self._tri = np.ascontiguousarray(scene.tri, np.float32)
+        self._ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
+        self._bsdf = scene.bsdf_array()
+        self._lights = scene.light_array()
+        self.h = orc().orc_scene_create(_ptr(self._tri), _ptr(self._ids), len(self._ids),
+                                        C.cast(self._bsdf, C.c_void_p), len(scene.bsdfs),
+                                        C.cast(self._lights, C.c_void_p), len(scene.lights))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            orc().orc_scene_destroy(self.h)
+            self.h = None
+
+    def intersect(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays)
+        t = np.zeros(len(rays), np.float32)
+        tri = np.zeros(len(rays), np.int32)
+        orc().orc_intersect_batch(self.h, _ptr(rays), len(rays), 1 if brute else 0, _ptr(t), _ptr(tri))
+        return t, tri
+
+    def qbvh_stats(self):
+        n, l, d = C.c_int64(), C.c_int64(), C.c_int32()
+        orc().orc_qbvh_stats(self.h, C.byref(n), C.byref(l), C.byref(d))
+        return n.value, l.value, d.value
+
+    def _params(self, photons, pass_index, radius, ray_eps, seed, scene_id=0, max_bounces=64):
+        return PassParams(photons=photons, pass_index=pass_index, max_bounces=max_bounces,
+                          radius=radius, ray_eps=ray_eps, seed=seed, scene_id=scene_id)
+
+    def trace_photons(self, photons, pass_index, ray_eps, seed, scene_id=0, cap=None):
+        cap = cap or photons * 8 + 1024
+        pos = np.zeros((cap, 3), np.float32)
+        flux = np.zeros((cap, 3), np.float32)
+        nrm = np.zeros((cap, 3), np.float32)
+        key = np.zeros(cap, np.uint32)
+        p = self._params(photons, pass_index, 0.0, ray_eps, seed, scene_id)
+        n = orc().orc_trace_photons(self.h, C.byref(p), _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), cap)
+        assert n <= cap
+        self.last_keys = key[:n]
+        return pos[:n], flux[:n], nrm[:n]
+
+    def eye_pass(self, cam, pass_index, ray_eps, seed, scene_id=0):
+        npix = cam.width * cam.height
+        has = np.zeros(npix, np.int32)
+        hp = np.zeros((npix, 3), np.float32)
+        hn = np.zeros((npix, 3), np.float32)
+        hw = np.zeros((npix, 3), np.float32)
+        dr = np.zeros((npix, 3), np.float64)
+        p = self._params(0, pass_index, 0.0, ray_eps, seed, scene_id)
+        orc().orc_eye_pass(self.h, C.byref(cam), C.byref(p), _ptr(has), _ptr(hp), _ptr(hn), _ptr(hw),
+                           _ptr(dr))
+        return has, hp, hn, hw, dr
+
+    def render(self, cam, photons, passes, r1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
+               hash_size=0, first_pass=1):
+        img = np.zeros((cam.height, cam.width, 3), np.float64)
+        stats = np.zeros(2, np.int64)
+        rc = orc().orc_render(self.h, C.byref(cam), photons, passes, first_pass, r1, alpha, ray_eps,
+                              seed, scene_id, hash_size, _ptr(img), _ptr(stats))
+        assert rc == 0
+        return img, stats
+
+
+def hashgrid_points(pos, hash_size, hash_scale, bbox_min):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = len(pos)
+    sidx = np.zeros(max(n, 1), np.int32)
+    cs = np.zeros(hash_size, np.int32)
+    ce = np.zeros(hash_size, np.int32)
+    bm = np.ascontiguousarray(bbox_min, np.float64)
+    orc().orc_hashgrid_points(_ptr(pos), n, hash_size, hash_scale, _ptr(bm), _ptr(sidx), _ptr(cs), _ptr(ce))
+    return sidx[:n], cs, ce
+
+
+def hashgrid_boxes(bmin, bmax, hash_size, hash_scale, bbox_min, cap):
+    bmin = np.ascontiguousarray(bmin, np.float64).reshape(-1, 3)
+    bmax = np.ascontiguousarray(bmax, np.float64).reshape(-1, 3)
+    items = np.zeros(max(cap, 1), np.int32)
+    cs = np.zeros(hash_size, np.int32)
+    ce = np.zeros(hash_size, np.int32)
+    bm = np.ascontiguousarray(bbox_min, np.float64)
+    n = orc().orc_hashgrid_boxes(_ptr(bmin), _ptr(bmax), len(bmin), hash_size, hash_scale, _ptr(bm),
+                                 _ptr(items), cap, _ptr(cs), _ptr(ce))
+    assert n >= 0
+    return items[:n], cs, ce
+
+
+def radius_schedule(r1, alpha, n, literal=False):
+    out = np.zeros(n, np.float64)
+    orc().orc_radius_schedule(r1, alpha, n, 1 if literal else 0, _ptr(out))
+    return out
+
+
+def gather(ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w, radius, hash_size, bbox_min):
+    a = [np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+         for x in (ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w)]
+    out = np.zeros((max(len(a[3]), 1), 3), np.float64)
+    cnt = np.zeros(3, np.int64)
+    bm = np.ascontiguousarray(bbox_min, np.float64)
+    orc().orc_gather(_ptr(a[0]), _ptr(a[1]), _ptr(a[2]), len(a[0]), _ptr(a[3]), _ptr(a[4]), _ptr(a[5]),
+                     len(a[3]), radius, hash_size, _ptr(bm), _ptr(out), _ptr(cnt))
+    return out[:len(a[3])], cnt

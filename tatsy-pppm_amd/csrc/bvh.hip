@@ -1,0 +1,387 @@
+// bvh.hip -- GPU construction of the 4-wide BVH and the batch intersect kernel.
+//
+// Replaces QBVHAccel::construct (sources/qbvh_accel.cc:139-203), whose serial
+// object-median build sorts at every level.  Here:
+//   1. 30-bit Morton codes of triangle centroids, stable radix sort;
+//   2. binary radix tree over the sorted codes (Karras 2012), every subtree a
+//      contiguous range of the sorted triangles;
+//   3. bottom-up boxes (one thread per leaf, the second arrival at a node
+//      merges its children);
+//   4. top-down collapse to 4-wide 128-byte nodes, one kernel per level:
+//      each node opens its largest child until it has four, subtrees of at
+//      most LEAF_MAX triangles become leaf ranges.
+#include "internal.h"
+#include "traverse.h"
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_centroid_bounds(const float* __restrict__ tri, int64_t n,
+                                  float* __restrict__ partial) {
+    __shared__ float s[6][256];
+    const int tid = threadIdx.x;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < n; i += (int64_t)gridDim.x * 256) {
+        const float* t = tri + 9 * i;
+        for (int d = 0; d < 3; d++) {
+            float c = (t[d] + t[3 + d] + t[6 + d]) * (1.0f / 3.0f);
+            lo[d] = fminf(lo[d], c);
+            hi[d] = fmaxf(hi[d], c);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        s[d][tid] = lo[d];
+        s[3 + d][tid] = hi[d];
+    }
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off)
+            for (int d = 0; d < 3; d++) {
+                s[d][tid] = fminf(s[d][tid], s[d][tid + off]);
+                s[3 + d][tid] = fmaxf(s[3 + d][tid], s[3 + d][tid + off]);
+            }
+        __syncthreads();
+    }
+    if (tid < 6) partial[blockIdx.x * 6 + tid] = s[tid][0];
+}
+
+__global__ void k_morton(const float* __restrict__ tri, int64_t n, const float* __restrict__ partial,
+                         int nparts, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int p = 0; p < nparts; p++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fminf(lo[d], partial[p * 6 + d]);
+            hi[d] = fmaxf(hi[d], partial[p * 6 + 3 + d]);
+        }
+    const float* t = tri + 9 * i;
+    uint32_t q[3];
+    for (int d = 0; d < 3; d++) {
+        float c = (t[d] + t[3 + d] + t[6 + d]) * (1.0f / 3.0f);
+        float ext = hi[d] - lo[d];
+        float u = ext > 0.0f ? (c - lo[d]) / ext : 0.5f;
+        q[d] = (uint32_t)fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    }
+    keys[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    vals[i] = (uint32_t)i;
+}
+
+// common-prefix length of sorted codes i and j; ties broken by index
+__device__ inline int delta(const uint32_t* codes, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = codes[i], b = codes[j];
+    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz(a ^ b);
+}
+
+// internal node i in [0, n-2]; child id < n-1: internal, else leaf (id - (n-1))
+__global__ void k_karras(const uint32_t* __restrict__ codes, int64_t n, int32_t* __restrict__ child,
+                         int32_t* __restrict__ parent, int32_t* __restrict__ range) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int dsign = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(codes, n, i, i - dsign);
+    int64_t lmax = 2;
+    while (delta(codes, n, i, i + lmax * dsign) > dmin) lmax <<= 1;
+    int64_t l = 0;
+    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(codes, n, i, i + (l + t) * dsign) > dmin) l += t;
+    int64_t j = i + l * dsign;
+    int dnode = delta(codes, n, i, j);
+    int64_t s = 0;
+    int64_t len = l;
+    for (int64_t t = (len + 1) >> 1;; t = (t + 1) >> 1) {
+        if (s + t < len + 1 && delta(codes, n, i, i + (s + t) * dsign) > dnode) s += t;
+        if (t == 1) break;
+    }
+    int64_t gamma = i + s * dsign + (dsign < 0 ? -1 : 0);
+    int64_t first = dsign > 0 ? i : j, last = dsign > 0 ? j : i;
+    int32_t left = (first == gamma) ? (int32_t)(n - 1 + gamma) : (int32_t)gamma;
+    int32_t right = (last == gamma + 1) ? (int32_t)(n - 1 + gamma + 1) : (int32_t)(gamma + 1);
+    child[2 * i] = left;
+    child[2 * i + 1] = right;
+    parent[left] = (int32_t)i;
+    parent[right] = (int32_t)i;
+    range[2 * i] = (int32_t)first;
+    range[2 * i + 1] = (int32_t)last;
+}
+
+__device__ inline void store_box(float* boxes, int64_t id, const Box& b) {
+    float4* p = reinterpret_cast<float4*>(boxes + 8 * id);
+    p[0] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+    p[1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+}
+__device__ inline Box load_box_coherent(float* boxes, int64_t id) {
+    Box b;
+    uint32_t* p = reinterpret_cast<uint32_t*>(boxes + 8 * id);
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = __uint_as_float(__hip_atomic_load(p + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        b.hi[d] = __uint_as_float(__hip_atomic_load(p + 4 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    return b;
+}
+__device__ inline Box load_box(const float* boxes, int64_t id) {
+    Box b;
+    const float4* p = reinterpret_cast<const float4*>(boxes + 8 * id);
+    float4 a = p[0], c = p[1];
+    b.lo[0] = a.x; b.lo[1] = a.y; b.lo[2] = a.z;
+    b.hi[0] = c.x; b.hi[1] = c.y; b.hi[2] = c.z;
+    return b;
+}
+
+// leaf boxes, leaf-ordered triangle records, then the bottom-up merge
+__global__ void k_leaves_refit(const float* __restrict__ tri, int64_t n,
+                               const uint32_t* __restrict__ order, const int32_t* __restrict__ child,
+                               const int32_t* __restrict__ parent, uint32_t* __restrict__ flags,
+                               float* __restrict__ boxes, float4* __restrict__ tris4) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t src = order[k];
+    const float* t = tri + 9 * (int64_t)src;
+    Box b;
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = fminf(fminf(t[d], t[3 + d]), t[6 + d]);
+        b.hi[d] = fmaxf(fmaxf(t[d], t[3 + d]), t[6 + d]);
+    }
+    tris4[3 * k + 0] = make_float4(t[0], t[1], t[2], __int_as_float((int32_t)src));
+    tris4[3 * k + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+    tris4[3 * k + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+    int64_t id = n - 1 + k;
+    store_box(boxes, id, b);
+    if (n == 1) return;
+    while (true) {
+        int32_t p = parent[id];
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;  // the sibling's thread merges later
+        __threadfence();
+        Box l = load_box_coherent(boxes, child[2 * p]);
+        Box r = load_box_coherent(boxes, child[2 * p + 1]);
+        Box m;
+        for (int d = 0; d < 3; d++) {
+            m.lo[d] = fminf(l.lo[d], r.lo[d]);
+            m.hi[d] = fmaxf(l.hi[d], r.hi[d]);
+        }
+        store_box(boxes, p, m);
+        if (p == 0) return;
+        id = p;
+    }
+}
+
+struct QItem {
+    int32_t bin;   // binary node id
+    int32_t node;  // 4-wide node index
+};
+
+__device__ inline float half_area(const Box& b) {
+    float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+    return ex * ey + ey * ez + ez * ex;
+}
+
+__global__ void k_collapse(const QItem* __restrict__ qin, const uint32_t* __restrict__ qin_count,
+                           QItem* __restrict__ qout, uint32_t* __restrict__ qout_count,
+                           uint32_t* __restrict__ node_count, int64_t n,
+                           const int32_t* __restrict__ child, const int32_t* __restrict__ range,
+                           const float* __restrict__ boxes, Node4* __restrict__ nodes) {
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= *qin_count) return;
+    QItem it = qin[qi];
+    int32_t cand[4];
+    int cnt = 2;
+    cand[0] = child[2 * it.bin];
+    cand[1] = child[2 * it.bin + 1];
+    auto size_of = [&](int32_t b) -> int32_t {
+        return b >= n - 1 ? 1 : range[2 * b + 1] - range[2 * b] + 1;
+    };
+    while (cnt < 4) {
+        int best = -1;
+        float best_a = -1.0f;
+        for (int c = 0; c < cnt; c++) {
+            int32_t b = cand[c];
+            if (b < n - 1 && size_of(b) > LEAF_MAX) {
+                float a = half_area(load_box(boxes, b));
+                if (a > best_a) { best_a = a; best = c; }
+            }
+        }
+        if (best < 0) break;
+        int32_t b = cand[best];
+        cand[best] = child[2 * b];
+        cand[cnt++] = child[2 * b + 1];
+    }
+    Node4 nd;
+    for (int c = 0; c < 4; c++) {
+        if (c < cnt) {
+            int32_t b = cand[c];
+            Box bx = load_box(boxes, b);
+            nd.lox[c] = bx.lo[0]; nd.hix[c] = bx.hi[0];
+            nd.loy[c] = bx.lo[1]; nd.hiy[c] = bx.hi[1];
+            nd.loz[c] = bx.lo[2]; nd.hiz[c] = bx.hi[2];
+            int32_t sz = size_of(b);
+            if (b >= n - 1 || sz <= LEAF_MAX) {
+                int32_t first = b >= n - 1 ? (int32_t)(b - (n - 1)) : range[2 * b];
+                nd.child[c] = leaf_ref(first, sz);
+            } else {
+                uint32_t idx = atomicAdd(node_count, 1u);
+                uint32_t slot = atomicAdd(qout_count, 1u);
+                qout[slot] = QItem{b, (int32_t)idx};
+                nd.child[c] = (int32_t)idx;
+            }
+        } else {
+            const float INF = __builtin_inff();
+            nd.lox[c] = INF; nd.hix[c] = INF;
+            nd.loy[c] = INF; nd.hiy[c] = INF;
+            nd.loz[c] = INF; nd.hiz[c] = INF;
+            nd.child[c] = EMPTY_REF;
+        }
+        nd.pad[c] = 0;
+    }
+    nodes[it.node] = nd;
+}
+
+__global__ void k_init_queue(QItem* q, uint32_t* qcount, uint32_t* node_count) {
+    q[0] = QItem{0, 0};
+    *qcount = 1;
+    *node_count = 1;
+}
+
+__global__ __launch_bounds__(256) void k_intersect(SceneDev sc, const pppm_ray* __restrict__ rays,
+                                                   int64_t n, pppm_hit* __restrict__ hits) {
+    __shared__ int32_t s_stack[STACK_LDS * 256];
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    pppm_ray r = rays[i];
+    bool ovf = false;
+    Hit h = trace_closest<256>(sc, mk(r.origin[0], r.origin[1], r.origin[2]),
+                               mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
+                               s_stack + threadIdx.x, &ovf);
+    pppm_hit o;
+    o.t = h.t;
+    o.tri = h.tri >= 0 ? tri_orig_id(sc, h.tri) : -1;
+    o.u = h.u;
+    o.v = h.v;
+    if (ovf) o.tri = -2;
+    hits[i] = o;
+}
+
+}  // namespace
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define HIPCHK(x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) return std::string(#x ": ") + hipGetErrorString(e_); \
+    } while (0)
+
+std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, BvhBuild* out) {
+    out->nodes = nullptr;
+    out->tris4 = nullptr;
+    out->root = EMPTY_REF;
+    out->n_nodes = 0;
+    out->n_leaves = 0;
+    out->depth = 0;
+    if (n <= 0) return "";
+    if (n > (1 << 27)) return "too many triangles for the leaf encoding";
+    const int nparts = 256;
+    float* partial = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *hist = nullptr;
+    int32_t *child = nullptr, *parent = nullptr, *range = nullptr;
+    uint32_t* flags = nullptr;
+    float* boxes = nullptr;
+    QItem *qa = nullptr, *qb = nullptr;
+    uint32_t* counts = nullptr;  // [0] qa count, [1] qb count, [2] node count
+    std::string err;
+    auto cleanup = [&]() {
+        (void)hipFree(partial); (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(keys2); (void)hipFree(vals2);
+        (void)hipFree(hist); (void)hipFree(child); (void)hipFree(parent); (void)hipFree(range); (void)hipFree(flags);
+        (void)hipFree(boxes); (void)hipFree(qa); (void)hipFree(qb); (void)hipFree(counts);
+    };
+    auto run = [&]() -> std::string {
+        HIPCHK(hipMalloc(&partial, sizeof(float) * 6 * nparts));
+        HIPCHK(hipMalloc(&keys, sizeof(uint32_t) * n));
+        HIPCHK(hipMalloc(&vals, sizeof(uint32_t) * n));
+        HIPCHK(hipMalloc(&keys2, sizeof(uint32_t) * n));
+        HIPCHK(hipMalloc(&vals2, sizeof(uint32_t) * n));
+        HIPCHK(hipMalloc(&hist, sizeof(uint32_t) * radix_hist_words(n)));
+        int64_t ni = n > 1 ? n - 1 : 1;
+        HIPCHK(hipMalloc(&child, sizeof(int32_t) * 2 * ni));
+        HIPCHK(hipMalloc(&parent, sizeof(int32_t) * (2 * n)));
+        HIPCHK(hipMalloc(&range, sizeof(int32_t) * 2 * ni));
+        HIPCHK(hipMalloc(&flags, sizeof(uint32_t) * ni));
+        HIPCHK(hipMalloc(&boxes, sizeof(float) * 8 * (2 * n)));
+        HIPCHK(hipMalloc(&qa, sizeof(QItem) * ni));
+        HIPCHK(hipMalloc(&qb, sizeof(QItem) * ni));
+        HIPCHK(hipMalloc(&counts, sizeof(uint32_t) * 4));
+        HIPCHK(hipMalloc(&out->tris4, sizeof(float4) * 3 * n));
+        HIPCHK(hipMalloc(&out->nodes, sizeof(Node4) * ni));
+
+        hipLaunchKernelGGL(k_centroid_bounds, dim3(nparts), dim3(256), 0, s, tri9, n, partial);
+        hipLaunchKernelGGL(k_morton, dim3(grid1(n, 256)), dim3(256), 0, s, tri9, n, partial, nparts,
+                           keys, vals);
+        bool alt = radix_sort_pairs(s, keys, vals, keys2, vals2, hist, n, 30);
+        uint32_t* codes = alt ? keys2 : keys;
+        uint32_t* order = alt ? vals2 : vals;
+        if (n > 1)
+            hipLaunchKernelGGL(k_karras, dim3(grid1(n - 1, 256)), dim3(256), 0, s, codes, n, child,
+                               parent, range);
+        launch_zero_u32(s, flags, ni);
+        hipLaunchKernelGGL(k_leaves_refit, dim3(grid1(n, 256)), dim3(256), 0, s, tri9, n, order,
+                           child, parent, flags, boxes, out->tris4);
+        HIPCHK(hipGetLastError());
+        if (n <= LEAF_MAX) {
+            out->root = leaf_ref(0, (int32_t)n);
+            out->n_leaves = 1;
+            out->depth = 0;
+            HIPCHK(hipStreamSynchronize(s));
+            return "";
+        }
+        hipLaunchKernelGGL(k_init_queue, dim3(1), dim3(1), 0, s, qa, counts + 0, counts + 2);
+        uint32_t nin = 1;
+        int depth = 0;
+        QItem *qi = qa, *qo = qb;
+        uint32_t *ci = counts + 0, *co = counts + 1;
+        while (nin > 0) {
+            launch_zero_u32(s, co, 1);
+            hipLaunchKernelGGL(k_collapse, dim3(grid1(nin, 256)), dim3(256), 0, s, qi, ci, qo, co,
+                               counts + 2, n, child, range, boxes, out->nodes);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(&nin, co, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            depth++;
+            QItem* t = qi; qi = qo; qo = t;
+            uint32_t* tc = ci; ci = co; co = tc;
+        }
+        uint32_t nn = 0;
+        HIPCHK(hipMemcpy(&nn, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        out->root = 0;
+        out->n_nodes = nn;
+        out->depth = depth;
+        return "";
+    };
+    err = run();
+    cleanup();
+    if (!err.empty()) {
+        (void)hipFree(out->nodes);
+        (void)hipFree(out->tris4);
+        out->nodes = nullptr;
+        out->tris4 = nullptr;
+    }
+    return err;
+}
+
+void launch_intersect(hipStream_t s, SceneDev sc, const pppm_ray* rays, int64_t n, pppm_hit* hits) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_intersect, dim3(grid1(n, 256)), dim3(256), 0, s, sc, rays, n, hits);
+}

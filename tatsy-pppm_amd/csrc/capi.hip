@@ -1,0 +1,878 @@
+// capi.hip -- the C ABI of libpppm_hip.so (declared in include/pppm.h).
+//
+// Owns the device state of one renderer: scene, 4-wide BVH, per-pass photon /
+// hit-point buffers and the Kahan-compensated FP32 image.  The pass loop of
+// ProgressivePhotonMappingProb::render (sources/ppm_probabilistic.cc:24-121)
+// is split into pppm_render_begin / pppm_render_pass / pppm_image_read.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "traverse.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu = 0;
+    std::string name;
+    size_t hbm = 0;
+    // scene
+    bool has_scene = false;
+    int64_t n_tris = 0;
+    DBuf tri9, tri_mat, mats, lights;
+    std::vector<pppm_bsdf> h_bsdf;
+    std::vector<pppm_light> h_lights;
+    double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    // BVH
+    bool has_bvh = false;
+    Node4* nodes = nullptr;
+    float4* tris4 = nullptr;
+    int32_t root = EMPTY_REF;
+    pppm_bvh_info binfo{};
+    // camera
+    bool has_cam = false;
+    pppm_camera cam{};
+    // render
+    bool rendering = false;
+    pppm_render_params rp{};
+    int32_t passes_done = 0;
+    std::vector<double> radii;
+    int64_t cap = 0;
+    double grid_bmin[3] = {0, 0, 0};
+    DBuf ph_pos, ph_flux, ph_pos_s, ph_flux_s, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp,
+        counters, scratch;
+    // last pass
+    bool last_valid = false;
+    int64_t last_n = 0;
+    int32_t last_S = 0;
+    double last_scale = 0.0;
+    uint32_t* last_perm1 = nullptr;
+    uint32_t* last_sorted = nullptr;
+    hipEvent_t ev[8] = {};
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    SceneDev scene_dev() const {
+        SceneDev s;
+        s.nodes = nodes;
+        s.tris = tris4;
+        s.tri_mat = tri_mat.as<int32_t>();
+        s.mats = mats.as<Mat>();
+        s.lights = lights.as<LightDev>();
+        s.n_lights = (int32_t)h_lights.size();
+        s.root = root;
+        s.n_tris = n_tris;
+        return s;
+    }
+};
+
+#define TRY_HIP(ctx, x)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return (ctx)->fail(PPPM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_CTX(c)                                   \
+    Ctx* c = reinterpret_cast<Ctx*>(c##_raw);          \
+    if (!c) return PPPM_ERR_INVALID;                   \
+    c->err.clear()
+
+int bits_for(uint64_t max_value) {
+    int b = 0;
+    while (b < 64 && (max_value >> b) != 0) b++;
+    return b;
+}
+
+// host copy of the device octahedral decode (pppm_device.h)
+void oct_decode_host(uint32_t e, float out[3]) {
+    float x = (float)(int16_t)(e & 0xFFFF) * (1.0f / 32767.0f);
+    float y = (float)(int16_t)(e >> 16) * (1.0f / 32767.0f);
+    float z = 1.0f - std::fabs(x) - std::fabs(y);
+    if (z < 0.0f) {
+        float ox = (1.0f - std::fabs(y)) * (x < 0.0f ? -1.0f : 1.0f);
+        float oy = (1.0f - std::fabs(x)) * (y < 0.0f ? -1.0f : 1.0f);
+        x = ox;
+        y = oy;
+    }
+    float n = std::sqrt(x * x + y * y + z * z);
+    out[0] = x / n;
+    out[1] = y / n;
+    out[2] = z / n;
+}
+
+void cross3(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Builds the grid (sort by bucket of (pos4 or pos3) through perm) into the
+// context's sort buffers, avoiding the buffer `keep` (index into u[], or -1).
+// Returns the buffer holding the sorted values (grid sorted_idx).
+uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32_t* perm, int64_t n,
+                     const GridParams& g, int keep) {
+    uint32_t* b[4];
+    int k = 0;
+    for (int i = 0; i < 5 && k < 4; i++)
+        if (i != keep) b[k++] = c->u[i].as<uint32_t>();
+    launch_point_keys(c->stream, pos4, pos3, perm, n, g, b[0], b[1]);
+    int bits = bits_for(g.size > 0 ? (uint64_t)(g.size - 1) : 0);
+    bool alt = radix_sort_pairs(c->stream, b[0], b[1], b[2], b[3], c->hist.as<uint32_t>(), n, bits);
+    uint32_t* keys = alt ? b[2] : b[0];
+    uint32_t* vals = alt ? b[3] : b[1];
+    launch_bucket_offsets(c->stream, keys, n, g.size, c->off.as<uint32_t>());
+    return vals;
+}
+
+hipError_t ensure_sort_buffers(Ctx* c, int64_t n, uint32_t S) {
+    for (int i = 0; i < 5; i++) {
+        hipError_t e = c->u[i].ensure(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = c->hist.ensure(sizeof(uint32_t) * (size_t)radix_hist_words(n > 0 ? n : 1));
+    if (e != hipSuccess) return e;
+    return c->off.ensure(sizeof(uint32_t) * ((size_t)S + 1));
+}
+
+void copy_ranges(const std::vector<uint32_t>& off, int32_t S, int32_t* cs, int32_t* ce) {
+    for (int32_t h = 0; h < S; h++) {
+        if (cs) cs[h] = (int32_t)off[h];
+        if (ce) ce[h] = (int32_t)off[h + 1];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pppm_create(void** out, int32_t device) {
+    if (!out) return PPPM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        g_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return PPPM_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        g_create_error = "device index out of range";
+        return PPPM_ERR_INVALID;
+    }
+    Ctx* c = new Ctx();
+    c->device = device;
+    (void)hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->name = prop.gcnArchName;
+        c->cu = prop.multiProcessorCount;
+        c->hbm = prop.totalGlobalMem;
+    }
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete c;
+        return PPPM_ERR_HIP;
+    }
+    for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+    *out = c;
+    return PPPM_OK;
+}
+
+int pppm_destroy(void* ctx_raw) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx_raw);
+    if (!c) return PPPM_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->tri9.release(); c->tri_mat.release(); c->mats.release(); c->lights.release();
+    (void)hipFree(c->nodes); (void)hipFree(c->tris4);
+    c->ph_pos.release(); c->ph_flux.release(); c->ph_pos_s.release(); c->ph_flux_s.release();
+    for (auto& b : c->u) b.release();
+    c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
+    c->img_sum.release(); c->img_comp.release(); c->counters.release(); c->scratch.release();
+    for (auto& ev : c->ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PPPM_OK;
+}
+
+const char* pppm_last_error(const void* ctx_raw) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(ctx_raw);
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int pppm_device_info(void* ctx_raw, char* name, int32_t name_len, int32_t* cu_count,
+                     int64_t* hbm_bytes) {
+    CHECK_CTX(ctx);
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s", ctx->name.c_str());
+    }
+    if (cu_count) *cu_count = ctx->cu;
+    if (hbm_bytes) *hbm_bytes = (int64_t)ctx->hbm;
+    return PPPM_OK;
+}
+
+int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_id, int64_t n,
+                      const pppm_bsdf* table, int32_t nb, const pppm_light* lights, int32_t nl) {
+    CHECK_CTX(ctx);
+    if (n < 0 || (n > 0 && (!tri_xyz || !bsdf_id)) || nb <= 0 || !table || nl <= 0 || !lights)
+        return ctx->fail(PPPM_ERR_INVALID, "scene needs triangles, bsdf ids, a bsdf table and >= 1 light");
+    for (int32_t i = 0; i < nb; i++) {
+        int t = table[i].type;
+        if (t & PPPM_BSDF_PHONG) return ctx->fail(PPPM_ERR_INVALID, "Phong BRDF is not supported");
+        if (!(t & (PPPM_BSDF_LAMBERTIAN | PPPM_BSDF_SPECULAR | PPPM_BSDF_REFRACTION | PPPM_BSDF_EMITTER)))
+            return ctx->fail(PPPM_ERR_INVALID, "bsdf " + std::to_string(i) + " has no supported type");
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (bsdf_id[i] < 0 || bsdf_id[i] >= nb)
+            return ctx->fail(PPPM_ERR_INVALID, "bsdf id out of range at triangle " + std::to_string(i));
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->has_scene = false;
+    ctx->has_bvh = false;
+    ctx->rendering = false;
+    TRY_HIP(ctx, ctx->tri9.ensure(sizeof(float) * 9 * (size_t)n));
+    TRY_HIP(ctx, ctx->tri_mat.ensure(sizeof(int32_t) * (size_t)n));
+    if (n > 0) {
+        TRY_HIP(ctx, hipMemcpy(ctx->tri9.p, tri_xyz, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice));
+        TRY_HIP(ctx, hipMemcpy(ctx->tri_mat.p, bsdf_id, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    }
+    ctx->h_bsdf.assign(table, table + nb);
+    ctx->h_lights.assign(lights, lights + nl);
+    std::vector<Mat> m(nb);
+    for (int32_t i = 0; i < nb; i++) {
+        const pppm_bsdf& b = table[i];
+        std::memset(&m[i], 0, sizeof(Mat));
+        m[i].type = b.type;
+        for (int k = 0; k < 3; k++) {
+            m[i].rho[k] = b.reflectance[k];
+            m[i].le[k] = b.emission[k];
+        }
+        m[i].rr_mean = (float)(((double)b.reflectance[0] + (double)b.reflectance[1] + (double)b.reflectance[2]) / 3.0);
+        m[i].rr_max = std::fmax(b.reflectance[0], std::fmax(b.reflectance[1], b.reflectance[2]));
+        m[i].ior = b.ior > 0.0f ? b.ior : 1.5f;
+    }
+    TRY_HIP(ctx, ctx->mats.ensure(sizeof(Mat) * (size_t)nb));
+    TRY_HIP(ctx, hipMemcpy(ctx->mats.p, m.data(), sizeof(Mat) * (size_t)nb, hipMemcpyHostToDevice));
+    TRY_HIP(ctx, ctx->lights.ensure(sizeof(LightDev) * (size_t)nl));
+    for (int d = 0; d < 3; d++) {
+        ctx->bmin[d] = 1e300;
+        ctx->bmax[d] = -1e300;
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < 9; j++) {
+            double v = tri_xyz[9 * i + j];
+            ctx->bmin[j % 3] = std::fmin(ctx->bmin[j % 3], v);
+            ctx->bmax[j % 3] = std::fmax(ctx->bmax[j % 3], v);
+        }
+    ctx->n_tris = n;
+    ctx->has_scene = true;
+    return PPPM_OK;
+}
+
+int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
+    CHECK_CTX(ctx);
+    if (!ctx->has_scene) return ctx->fail(PPPM_ERR_STATE, "no scene uploaded");
+    (void)hipSetDevice(ctx->device);
+    (void)hipFree(ctx->nodes);
+    (void)hipFree(ctx->tris4);
+    ctx->nodes = nullptr;
+    ctx->tris4 = nullptr;
+    ctx->has_bvh = false;
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    BvhBuild b;
+    std::string e = bvh_build(ctx->stream, ctx->tri9.as<float>(), ctx->n_tris, &b);
+    if (!e.empty()) return ctx->fail(PPPM_ERR_HIP, "BVH build: " + e);
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    TRY_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->nodes = b.nodes;
+    ctx->tris4 = b.tris4;
+    ctx->root = b.root;
+    // the traversal stack holds at most 3 entries per level below the root
+    if (3 * (int64_t)b.depth + 1 > STACK_MAX)
+        return ctx->fail(PPPM_ERR_INVALID, "BVH deeper than the traversal stack allows");
+    ctx->binfo.nodes = b.n_nodes;
+    ctx->binfo.leaves = b.n_leaves;
+    ctx->binfo.depth = b.depth;
+    ctx->binfo.max_leaf = LEAF_MAX;
+    ctx->binfo.build_ms = ms;
+    ctx->has_bvh = true;
+    if (info) *info = ctx->binfo;
+    return PPPM_OK;
+}
+
+int pppm_intersect(void* ctx_raw, const pppm_ray* rays, int64_t n, pppm_hit* hits) {
+    CHECK_CTX(ctx);
+    if (!ctx->has_bvh) return ctx->fail(PPPM_ERR_STATE, "no BVH built");
+    if (n < 0 || (n > 0 && (!rays || !hits))) return ctx->fail(PPPM_ERR_INVALID, "bad ray arrays");
+    if (n == 0) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    // placeholder lights: intersection needs none, but SceneDev reads n_lights only
+    TRY_HIP(ctx, ctx->scratch.ensure((sizeof(pppm_ray) + sizeof(pppm_hit)) * (size_t)n));
+    pppm_ray* dr = ctx->scratch.as<pppm_ray>();
+    pppm_hit* dh = reinterpret_cast<pppm_hit*>(dr + n);
+    TRY_HIP(ctx, hipMemcpyAsync(dr, rays, sizeof(pppm_ray) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    launch_intersect(ctx->stream, ctx->scene_dev(), dr, n, dh);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipMemcpyAsync(hits, dh, sizeof(pppm_hit) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < n; i++)
+        if (hits[i].tri == -2) return ctx->fail(PPPM_ERR_HIP, "traversal stack overflow");
+    return PPPM_OK;
+}
+
+int pppm_hashgrid_build(void* ctx_raw, const float* pos, int64_t n, int32_t hash_size,
+                        double hash_scale, const double bbox_min[3], int32_t* sorted_idx,
+                        int32_t* cell_start, int32_t* cell_end) {
+    CHECK_CTX(ctx);
+    if (n < 0 || (n > 0 && !pos) || hash_size <= 0 || !bbox_min || n > 0x7FFFFFFFLL)
+        return ctx->fail(PPPM_ERR_INVALID, "bad hash grid arguments");
+    (void)hipSetDevice(ctx->device);
+    TRY_HIP(ctx, ensure_sort_buffers(ctx, n, (uint32_t)hash_size));
+    TRY_HIP(ctx, ctx->scratch.ensure(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1)));
+    if (n > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(ctx->scratch.p, pos, sizeof(float) * 3 * (size_t)n,
+                                    hipMemcpyHostToDevice, ctx->stream));
+    GridParams g;
+    for (int d = 0; d < 3; d++) g.bmin[d] = bbox_min[d];
+    g.scale = hash_scale;
+    g.size = (uint32_t)hash_size;
+    uint32_t* vals = build_grid(ctx, nullptr, ctx->scratch.as<float>(), nullptr, n, g, -1);
+    TRY_HIP(ctx, hipGetLastError());
+    std::vector<uint32_t> off((size_t)hash_size + 1);
+    if (n > 0 && sorted_idx)
+        TRY_HIP(ctx, hipMemcpyAsync(sorted_idx, vals, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->off.p, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    copy_ranges(off, hash_size, cell_start, cell_end);
+    return PPPM_OK;
+}
+
+int pppm_hashgrid_build_boxes(void* ctx_raw, const double* box_min, const double* box_max, int64_t n,
+                              int32_t hash_size, double hash_scale, const double bbox_min[3],
+                              int32_t* sorted_items, int64_t cap, int64_t* n_pairs,
+                              int32_t* cell_start, int32_t* cell_end) {
+    CHECK_CTX(ctx);
+    if (n < 0 || (n > 0 && (!box_min || !box_max)) || hash_size <= 0 || !bbox_min || !n_pairs)
+        return ctx->fail(PPPM_ERR_INVALID, "bad hash grid arguments");
+    (void)hipSetDevice(ctx->device);
+    GridParams g;
+    for (int d = 0; d < 3; d++) g.bmin[d] = bbox_min[d];
+    g.scale = hash_scale;
+    g.size = (uint32_t)hash_size;
+    int64_t nn = n > 0 ? n : 1;
+    int64_t scan_words = nn + nn / 1024 + 64;
+    TRY_HIP(ctx, ctx->scratch.ensure(sizeof(double) * 6 * (size_t)nn + sizeof(uint64_t) * (size_t)(nn + scan_words)));
+    double* dmin = ctx->scratch.as<double>();
+    double* dmax = dmin + 3 * nn;
+    uint64_t* counts = reinterpret_cast<uint64_t*>(dmax + 3 * nn);
+    uint64_t* bsum = counts + nn;
+    if (n > 0) {
+        TRY_HIP(ctx, hipMemcpyAsync(dmin, box_min, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+        TRY_HIP(ctx, hipMemcpyAsync(dmax, box_max, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    launch_box_counts(ctx->stream, dmin, dmax, n, g, counts);
+    uint64_t last_count = 0;
+    if (n > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(&last_count, counts + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    launch_exclusive_scan_u64(ctx->stream, counts, n, bsum);
+    uint64_t last_off = 0;
+    if (n > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(&last_off, counts + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t total = (int64_t)(last_off + last_count);
+    *n_pairs = total;
+    if (total > cap) return ctx->fail(PPPM_ERR_CAPACITY, "pair count exceeds cap");
+    if (total > 0x7FFFFFFFLL) return ctx->fail(PPPM_ERR_INVALID, "too many (item, cell) pairs");
+    TRY_HIP(ctx, ensure_sort_buffers(ctx, total, (uint32_t)hash_size));
+    uint32_t* b[4] = {ctx->u[0].as<uint32_t>(), ctx->u[1].as<uint32_t>(), ctx->u[2].as<uint32_t>(),
+                      ctx->u[3].as<uint32_t>()};
+    launch_box_pairs(ctx->stream, dmin, dmax, n, g, counts, b[0], b[1]);
+    bool alt = radix_sort_pairs(ctx->stream, b[0], b[1], b[2], b[3], ctx->hist.as<uint32_t>(), total,
+                                bits_for((uint64_t)(hash_size - 1)));
+    uint32_t* keys = alt ? b[2] : b[0];
+    uint32_t* vals = alt ? b[3] : b[1];
+    launch_bucket_offsets(ctx->stream, keys, total, g.size, ctx->off.as<uint32_t>());
+    TRY_HIP(ctx, hipGetLastError());
+    std::vector<uint32_t> off((size_t)hash_size + 1);
+    if (total > 0 && sorted_items)
+        TRY_HIP(ctx, hipMemcpyAsync(sorted_items, vals, sizeof(uint32_t) * (size_t)total, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->off.p, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    copy_ranges(off, hash_size, cell_start, cell_end);
+    return PPPM_OK;
+}
+
+int pppm_radius_schedule(double r1, double alpha, int32_t n, int32_t literal, double* out) {
+    if (n < 0 || (n > 0 && !out) || !(r1 > 0.0) || !(alpha > 0.0)) return PPPM_ERR_INVALID;
+    if (n == 0) return PPPM_OK;
+    out[0] = r1;
+    if (literal) {
+        double r = r1;
+        for (int t = 1; t < n; t++) {
+            r = (t + 1.0) / (t + alpha) * r;  // ppm_probabilistic.cc:94
+            out[t] = r;
+        }
+    } else {
+        double r2 = r1 * r1;
+        for (int i = 1; i < n; i++) {
+            r2 = r2 * ((i + alpha) / (i + 1.0));  // Knaus & Zwicker 2011
+            out[i] = std::sqrt(r2);
+        }
+    }
+    return PPPM_OK;
+}
+
+int pppm_camera_set(void* ctx_raw, const pppm_camera* cam) {
+    CHECK_CTX(ctx);
+    if (!cam || cam->width <= 0 || cam->height <= 0 || !(cam->fov_y_deg > 0.0 && cam->fov_y_deg < 180.0))
+        return ctx->fail(PPPM_ERR_INVALID, "bad camera");
+    ctx->cam = *cam;
+    ctx->has_cam = true;
+    ctx->rendering = false;
+    return PPPM_OK;
+}
+
+int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
+    CHECK_CTX(ctx);
+    if (!ctx->has_bvh) return ctx->fail(PPPM_ERR_STATE, "no BVH built");
+    if (!ctx->has_cam) return ctx->fail(PPPM_ERR_STATE, "no camera set");
+    if (!p || p->photons <= 0 || p->passes <= 0 || !(p->alpha > 0.0 && p->alpha < 1.0) ||
+        !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0)
+        return ctx->fail(PPPM_ERR_INVALID, "bad render parameters");
+    if (p->photons > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "at most 2^26 photons per pass");
+    (void)hipSetDevice(ctx->device);
+    ctx->rp = *p;
+    ctx->radii.assign((size_t)p->passes, 0.0);
+    pppm_radius_schedule(p->radius1, p->alpha, p->passes, 0, ctx->radii.data());
+    double factor = p->capacity_factor > 0.0 ? p->capacity_factor : 8.0;
+    ctx->cap = (int64_t)std::ceil((double)p->photons * factor);
+    if (ctx->cap > 0x7FFFFFFFLL) ctx->cap = 0x7FFFFFFFLL;
+    for (int d = 0; d < 3; d++) ctx->grid_bmin[d] = ctx->bmin[d] - p->radius1;
+    // lights: selection by power, photon flux = P_total / N (FP64 on the host)
+    int nl = (int)ctx->h_lights.size();
+    std::vector<double> power(nl), area(nl);
+    double total = 0.0;
+    for (int l = 0; l < nl; l++) {
+        const pppm_light& L = ctx->h_lights[l];
+        double eu[3] = {L.edge_u[0], L.edge_u[1], L.edge_u[2]}, ev[3] = {L.edge_v[0], L.edge_v[1], L.edge_v[2]};
+        double cr[3];
+        cross3(eu, ev, cr);
+        area[l] = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        double lum = ((double)L.emission[0] + (double)L.emission[1] + (double)L.emission[2]) / 3.0;
+        power[l] = M_PI * area[l] * lum;
+        total += power[l];
+    }
+    if (!(total > 0.0)) return ctx->fail(PPPM_ERR_INVALID, "lights emit no power");
+    std::vector<LightDev> ld(nl);
+    double cum = 0.0;
+    for (int l = 0; l < nl; l++) {
+        const pppm_light& L = ctx->h_lights[l];
+        double eu[3] = {L.edge_u[0], L.edge_u[1], L.edge_u[2]}, ev[3] = {L.edge_v[0], L.edge_v[1], L.edge_v[2]};
+        double cr[3];
+        cross3(eu, ev, cr);
+        double inv = 1.0 / (area[l] + 1e-300);
+        LightDev& o = ld[l];
+        o.corner = mk(L.corner[0], L.corner[1], L.corner[2]);
+        o.eu = mk(L.edge_u[0], L.edge_u[1], L.edge_u[2]);
+        o.ev = mk(L.edge_v[0], L.edge_v[1], L.edge_v[2]);
+        o.normal = mk((float)(cr[0] * inv), (float)(cr[1] * inv), (float)(cr[2] * inv));
+        double pl = power[l] / total;
+        double k = M_PI * area[l] / (pl * (double)p->photons);
+        o.flux = mk((float)(L.emission[0] * k), (float)(L.emission[1] * k), (float)(L.emission[2] * k));
+        cum += power[l];
+        o.cdf_hi = (l == nl - 1) ? 2.0f : (float)(cum / total);
+    }
+    TRY_HIP(ctx, hipMemcpy(ctx->lights.p, ld.data(), sizeof(LightDev) * (size_t)nl, hipMemcpyHostToDevice));
+    const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
+    const int64_t cap = ctx->cap;
+    TRY_HIP(ctx, ctx->ph_pos.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_flux.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_pos_s.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_flux_s.ensure(sizeof(float4) * (size_t)cap));
+    uint32_t Smax = p->hash_size > 0 ? (uint32_t)p->hash_size : (uint32_t)cap;
+    TRY_HIP(ctx, ensure_sort_buffers(ctx, cap, Smax));
+    TRY_HIP(ctx, ctx->hp_pos.ensure(sizeof(float4) * (size_t)npix));
+    TRY_HIP(ctx, ctx->hp_w.ensure(sizeof(float4) * (size_t)npix));
+    TRY_HIP(ctx, ctx->img_sum.ensure(sizeof(float) * 3 * (size_t)npix));
+    TRY_HIP(ctx, ctx->img_comp.ensure(sizeof(float) * 3 * (size_t)npix));
+    TRY_HIP(ctx, ctx->counters.ensure(sizeof(Counters)));
+    launch_zero_u32(ctx->stream, ctx->img_sum.p, 3 * npix);
+    launch_zero_u32(ctx->stream, ctx->img_comp.p, 3 * npix);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->passes_done = 0;
+    ctx->rendering = true;
+    ctx->last_valid = false;
+    return PPPM_OK;
+}
+
+int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
+    CHECK_CTX(ctx);
+    if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "pppm_render_begin not called");
+    if (ctx->passes_done >= ctx->rp.passes) return ctx->fail(PPPM_ERR_STATE, "all passes done");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int32_t pass = ctx->passes_done + 1;
+    const double radius = ctx->radii[(size_t)pass - 1];
+    const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
+    const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    Counters* dc = ctx->counters.as<Counters>();
+    ctx->last_valid = false;
+
+    launch_zero_u32(s, dc, sizeof(Counters) / 4);
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    PhotonPassArgs pa;
+    pa.sc = ctx->scene_dev();
+    pa.n_photons = ctx->rp.photons;
+    pa.cap = ctx->cap;
+    pa.seed = ctx->rp.seed;
+    pa.pass = (uint32_t)pass;
+    pa.scene_id = ctx->rp.scene_id;
+    pa.max_bounces = ctx->rp.max_bounces;
+    pa.ray_eps = (float)ctx->rp.ray_eps;
+    pa.ph_pos = ctx->ph_pos.as<float4>();
+    pa.ph_flux = ctx->ph_flux.as<float4>();
+    pa.counters = dc;
+    launch_photon_pass(s, pa, grid_limit);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    Counters hc;
+    TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    TRY_HIP(ctx, hipStreamSynchronize(s));
+    const int64_t n = (int64_t)hc.photons_stored;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->pass_index = pass;
+        st->radius = radius;
+        st->photons_emitted = ctx->rp.photons;
+        st->photons_stored = n;
+    }
+    if (n > ctx->cap)
+        return ctx->fail(PPPM_ERR_CAPACITY, "photon buffer holds " + std::to_string(ctx->cap) +
+                                                " records, the pass stored " + std::to_string(n));
+    // deterministic photon order: stable sort by (photon index, bounce)
+    uint32_t* u0 = ctx->u[0].as<uint32_t>();
+    uint32_t* u1 = ctx->u[1].as<uint32_t>();
+    uint32_t* u2 = ctx->u[2].as<uint32_t>();
+    uint32_t* u3 = ctx->u[3].as<uint32_t>();
+    launch_origin_keys(s, ctx->ph_flux.as<float4>(), n, u0, u1);
+    int obits = bits_for(((uint64_t)(ctx->rp.photons - 1) << 6) | 63u);
+    bool alt = radix_sort_pairs(s, u0, u1, u2, u3, ctx->hist.as<uint32_t>(), n, obits);
+    int keep = alt ? 3 : 1;
+    uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    // hash grid over the ordered photons
+    uint32_t S = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
+    GridParams g;
+    for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
+    g.scale = 1.0 / (2.0 * radius);
+    g.size = S;
+    TRY_HIP(ctx, ctx->off.ensure(sizeof(uint32_t) * ((size_t)S + 1)));
+    uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
+    launch_permute2(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
+                    ctx->ph_pos_s.as<float4>(), ctx->ph_flux_s.as<float4>());
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+    // eye pass
+    EyePassArgs ea;
+    ea.sc = ctx->scene_dev();
+    ea.cam = ctx->cam;
+    {
+        const pppm_camera& c = ctx->cam;
+        double D[3] = {c.direction[0], c.direction[1], c.direction[2]};
+        double dn = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+        for (double& v : D) v /= dn;
+        double up[3] = {c.up[0], c.up[1], c.up[2]};
+        double U[3], V[3];
+        cross3(D, up, U);
+        double un = std::sqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+        for (double& v : U) v /= un;
+        cross3(U, D, V);
+        double th = std::tan(c.fov_y_deg * 0.5 * M_PI / 180.0);
+        double aspect = (double)c.width / (double)c.height;
+        for (int k = 0; k < 3; k++) {
+            ea.cam_basis[k] = (float)c.center[k];
+            ea.cam_basis[3 + k] = (float)D[k];
+            ea.cam_basis[6 + k] = (float)(U[k] * th * aspect);
+            ea.cam_basis[9 + k] = (float)(V[k] * th);
+        }
+    }
+    ea.seed = ctx->rp.seed;
+    ea.pass = (uint32_t)pass;
+    ea.scene_id = ctx->rp.scene_id;
+    ea.max_bounces = ctx->rp.max_bounces;
+    ea.ray_eps = (float)ctx->rp.ray_eps;
+    ea.hp_pos = ctx->hp_pos.as<float4>();
+    ea.hp_w = ctx->hp_w.as<float4>();
+    ea.img_sum = ctx->img_sum.as<float>();
+    ea.img_comp = ctx->img_comp.as<float>();
+    ea.counters = dc;
+    launch_eye_pass(s, ea, grid_limit);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    // gather
+    GatherArgs ga;
+    ga.ph_pos = ctx->ph_pos_s.as<float4>();
+    ga.ph_flux = ctx->ph_flux_s.as<float4>();
+    ga.off = ctx->off.as<uint32_t>();
+    ga.hp_pos = ctx->hp_pos.as<float4>();
+    ga.hp_w = ctx->hp_w.as<float4>();
+    ga.hp_count_src = dc;
+    ga.n_hit = npix;
+    ga.g = g;
+    ga.radius = radius;
+    ga.img_sum = ctx->img_sum.as<float>();
+    ga.img_comp = ctx->img_comp.as<float>();
+    ga.out_rgb = nullptr;
+    ga.counters = dc;
+    launch_gather(s, ga);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+    TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    TRY_HIP(ctx, hipStreamSynchronize(s));
+    if (hc.stack_overflow) return ctx->fail(PPPM_ERR_HIP, "traversal stack overflow");
+    ctx->passes_done = pass;
+    ctx->last_valid = true;
+    ctx->last_n = n;
+    ctx->last_S = (int32_t)S;
+    ctx->last_scale = g.scale;
+    ctx->last_perm1 = perm1;
+    ctx->last_sorted = sorted;
+    if (st) {
+        float t[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < 5; k++) (void)hipEventElapsedTime(&t[k], ctx->ev[k], ctx->ev[k + 1]);
+        float tot = 0.0f;
+        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]);
+        st->hash_size = (int32_t)S;
+        st->photon_segments = (int64_t)hc.photon_segments;
+        st->eye_segments = (int64_t)hc.eye_segments;
+        st->hitpoints = (int64_t)hc.hitpoints;
+        st->gather_buckets = (int64_t)hc.gather_buckets;
+        st->gather_candidates = (int64_t)hc.gather_candidates;
+        st->gather_inradius = (int64_t)hc.gather_inradius;
+        st->photon_ms = t[0];
+        st->order_ms = t[1];
+        st->grid_ms = t[2];
+        st->eye_ms = t[3];
+        st->gather_ms = t[4];
+        st->total_ms = tot;
+    }
+    return PPPM_OK;
+}
+
+int pppm_image_read(void* ctx_raw, double* rgb, int64_t n_values) {
+    CHECK_CTX(ctx);
+    if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "no render in progress");
+    const int64_t nv = 3 * (int64_t)ctx->cam.width * ctx->cam.height;
+    if (!rgb || n_values < nv) return ctx->fail(PPPM_ERR_INVALID, "image buffer too small");
+    (void)hipSetDevice(ctx->device);
+    std::vector<float> s((size_t)nv), c((size_t)nv);
+    TRY_HIP(ctx, hipMemcpyAsync(s.data(), ctx->img_sum.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(c.data(), ctx->img_comp.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double inv = ctx->passes_done > 0 ? 1.0 / (double)ctx->passes_done : 0.0;
+    for (int64_t i = 0; i < nv; i++) rgb[i] = ((double)s[(size_t)i] - (double)c[(size_t)i]) * inv;
+    return PPPM_OK;
+}
+
+int pppm_last_photons(void* ctx_raw, float* pos, float* flux, float* normal, uint32_t* key,
+                      int64_t cap, int64_t* n) {
+    CHECK_CTX(ctx);
+    if (!ctx->last_valid) return ctx->fail(PPPM_ERR_STATE, "no completed pass");
+    if (n) *n = ctx->last_n;
+    const int64_t m = ctx->last_n < cap ? ctx->last_n : cap;
+    if (m <= 0) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    TRY_HIP(ctx, ctx->scratch.ensure(sizeof(float4) * 2 * (size_t)m));
+    float4* a = ctx->scratch.as<float4>();
+    float4* b = a + m;
+    launch_permute2(ctx->stream, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), ctx->last_perm1,
+                    nullptr, m, a, b);
+    std::vector<float4> ha((size_t)m), hb((size_t)m);
+    TRY_HIP(ctx, hipMemcpyAsync(ha.data(), a, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(hb.data(), b, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < m; i++) {
+        if (pos) { pos[3 * i] = ha[i].x; pos[3 * i + 1] = ha[i].y; pos[3 * i + 2] = ha[i].z; }
+        if (flux) { flux[3 * i] = hb[i].x; flux[3 * i + 1] = hb[i].y; flux[3 * i + 2] = hb[i].z; }
+        if (key) std::memcpy(&key[i], &hb[i].w, 4);
+        if (normal) {
+            uint32_t e;
+            std::memcpy(&e, &ha[i].w, 4);
+            oct_decode_host(e, normal + 3 * i);
+        }
+    }
+    return PPPM_OK;
+}
+
+int pppm_last_grid(void* ctx_raw, int32_t* sorted_idx, int32_t* cell_start, int32_t* cell_end,
+                   int32_t hash_size, double bbox_min[3], double* hash_scale) {
+    CHECK_CTX(ctx);
+    if (!ctx->last_valid) return ctx->fail(PPPM_ERR_STATE, "no completed pass");
+    if (hash_size != ctx->last_S) return ctx->fail(PPPM_ERR_INVALID, "hash_size must equal the pass's");
+    (void)hipSetDevice(ctx->device);
+    std::vector<uint32_t> off((size_t)hash_size + 1);
+    if (sorted_idx && ctx->last_n > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(sorted_idx, ctx->last_sorted, sizeof(uint32_t) * (size_t)ctx->last_n, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->off.p, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    copy_ranges(off, hash_size, cell_start, cell_end);
+    if (bbox_min)
+        for (int d = 0; d < 3; d++) bbox_min[d] = ctx->grid_bmin[d];
+    if (hash_scale) *hash_scale = ctx->last_scale;
+    return PPPM_OK;
+}
+
+int pppm_last_hitpoints(void* ctx_raw, float* pos, float* normal, float* weight, int32_t* pixel,
+                        int64_t cap, int64_t* n) {
+    CHECK_CTX(ctx);
+    if (!ctx->last_valid) return ctx->fail(PPPM_ERR_STATE, "no completed pass");
+    (void)hipSetDevice(ctx->device);
+    Counters hc;
+    TRY_HIP(ctx, hipMemcpy(&hc, ctx->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
+    const int64_t nh = (int64_t)hc.hitpoints;
+    if (n) *n = nh;
+    const int64_t m = nh < cap ? nh : cap;
+    if (m <= 0) return PPPM_OK;
+    std::vector<float4> a((size_t)m), b((size_t)m);
+    TRY_HIP(ctx, hipMemcpy(a.data(), ctx->hp_pos.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    TRY_HIP(ctx, hipMemcpy(b.data(), ctx->hp_w.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; i++) {
+        if (pos) { pos[3 * i] = a[i].x; pos[3 * i + 1] = a[i].y; pos[3 * i + 2] = a[i].z; }
+        if (normal) {
+            uint32_t e;
+            std::memcpy(&e, &a[i].w, 4);
+            oct_decode_host(e, normal + 3 * i);
+        }
+        if (weight) { weight[3 * i] = b[i].x; weight[3 * i + 1] = b[i].y; weight[3 * i + 2] = b[i].z; }
+        if (pixel) std::memcpy(&pixel[i], &b[i].w, 4);
+    }
+    return PPPM_OK;
+}
+
+int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const float* ph_nrm,
+                int64_t n_ph, const float* hit_pos, const float* hit_nrm, const float* hit_w,
+                int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
+                float* out_rgb, pppm_pass_stats* st) {
+    CHECK_CTX(ctx);
+    if (n_ph < 0 || n_hit < 0 || hash_size <= 0 || !(radius > 0.0) || !bbox_min ||
+        (n_ph > 0 && (!ph_pos || !ph_flux || !ph_nrm)) ||
+        (n_hit > 0 && (!hit_pos || !hit_nrm || !hit_w || !out_rgb)) || n_ph > 0x7FFFFFFFLL)
+        return ctx->fail(PPPM_ERR_INVALID, "bad gather arguments");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int64_t np = n_ph > 0 ? n_ph : 1, nh = n_hit > 0 ? n_hit : 1;
+    // staging: photons [3 x 3 floats] + hit points [3 x 3 floats] + outputs
+    size_t bytes = sizeof(float) * 9 * (size_t)(np + nh) + sizeof(float4) * 4 * (size_t)np +
+                   sizeof(float4) * 2 * (size_t)nh + sizeof(float) * 3 * (size_t)nh;
+    TRY_HIP(ctx, ctx->scratch.ensure(bytes));
+    float* f = ctx->scratch.as<float>();
+    float* dpp = f; float* dpf = dpp + 3 * np; float* dpn = dpf + 3 * np;
+    float* dhp = dpn + 3 * np; float* dhn = dhp + 3 * nh; float* dhw = dhn + 3 * nh;
+    float4* p4 = reinterpret_cast<float4*>(dhw + 3 * nh);
+    float4* f4 = p4 + np;
+    float4* p4s = f4 + np;
+    float4* f4s = p4s + np;
+    float4* h4 = f4s + np;
+    float4* w4 = h4 + nh;
+    float* dout = reinterpret_cast<float*>(w4 + nh);
+    if (n_ph > 0) {
+        TRY_HIP(ctx, hipMemcpyAsync(dpp, ph_pos, sizeof(float) * 3 * (size_t)n_ph, hipMemcpyHostToDevice, s));
+        TRY_HIP(ctx, hipMemcpyAsync(dpf, ph_flux, sizeof(float) * 3 * (size_t)n_ph, hipMemcpyHostToDevice, s));
+        TRY_HIP(ctx, hipMemcpyAsync(dpn, ph_nrm, sizeof(float) * 3 * (size_t)n_ph, hipMemcpyHostToDevice, s));
+    }
+    if (n_hit > 0) {
+        TRY_HIP(ctx, hipMemcpyAsync(dhp, hit_pos, sizeof(float) * 3 * (size_t)n_hit, hipMemcpyHostToDevice, s));
+        TRY_HIP(ctx, hipMemcpyAsync(dhn, hit_nrm, sizeof(float) * 3 * (size_t)n_hit, hipMemcpyHostToDevice, s));
+        TRY_HIP(ctx, hipMemcpyAsync(dhw, hit_w, sizeof(float) * 3 * (size_t)n_hit, hipMemcpyHostToDevice, s));
+    }
+    launch_pack_points(s, dpp, dpf, dpn, n_ph, p4, f4, 1);
+    launch_pack_points(s, dhp, dhw, dhn, n_hit, h4, w4, 1);
+    TRY_HIP(ctx, ensure_sort_buffers(ctx, n_ph, (uint32_t)hash_size));
+    TRY_HIP(ctx, ctx->counters.ensure(sizeof(Counters)));
+    Counters* dc = ctx->counters.as<Counters>();
+    launch_zero_u32(s, dc, sizeof(Counters) / 4);
+    GridParams g;
+    for (int d = 0; d < 3; d++) g.bmin[d] = bbox_min[d];
+    g.scale = 1.0 / (2.0 * radius);
+    g.size = (uint32_t)hash_size;
+    uint32_t* sorted = build_grid(ctx, p4, nullptr, nullptr, n_ph, g, -1);
+    launch_permute2(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s);
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    GatherArgs ga;
+    ga.ph_pos = p4s;
+    ga.ph_flux = f4s;
+    ga.off = ctx->off.as<uint32_t>();
+    ga.hp_pos = h4;
+    ga.hp_w = w4;
+    ga.hp_count_src = nullptr;
+    ga.n_hit = n_hit;
+    ga.g = g;
+    ga.radius = radius;
+    ga.img_sum = nullptr;
+    ga.img_comp = nullptr;
+    ga.out_rgb = dout;
+    ga.counters = dc;
+    launch_gather(s, ga);
+    TRY_HIP(ctx, hipGetLastError());
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    if (n_hit > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(out_rgb, dout, sizeof(float) * 3 * (size_t)n_hit, hipMemcpyDeviceToHost, s));
+    Counters hc;
+    TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    TRY_HIP(ctx, hipStreamSynchronize(s));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        st->gather_ms = ms;
+        st->radius = radius;
+        st->hash_size = hash_size;
+        st->photons_stored = n_ph;
+        st->hitpoints = n_hit;
+        st->gather_buckets = (int64_t)hc.gather_buckets;
+        st->gather_candidates = (int64_t)hc.gather_candidates;
+        st->gather_inradius = (int64_t)hc.gather_inradius;
+    }
+    return PPPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// hash_grid.hip -- GPU build of the reference HashGrid (sources/hash_grid_detail.h).
+//
+// Bucket of a point: h = ((ix*73856093) ^ (iy*19349663) ^ (iz*83492791)) mod S
+// with i* = |trunc((x - bbox_min) * scale)| computed in FP64
+// (hash_grid_detail.h:35-43, 60-62).  The int32 products of the reference
+// wrap; uint32 products give the same bits.  A stable radix sort of
+// (h, insertion index) then lays every bucket out in insertion order, which
+// is exactly HashGrid<int>::_data[h] after the same add() calls.
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ inline int32_t cell_of(double x, double mn, double scale) {
+    double b = (x - mn) * scale;
+    int32_t c = (int32_t)b;  // truncation toward zero, as static_cast<int>
+    return c < 0 ? -c : c;   // std::abs
+}
+
+__device__ inline uint32_t hash3(int32_t ix, int32_t iy, int32_t iz, uint32_t size) {
+    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    return h % size;
+}
+
+__global__ void k_point_keys(const float4* __restrict__ pos4, const float* __restrict__ pos3,
+                             const uint32_t* __restrict__ perm, int64_t n, GridParams g,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t j = perm ? (int64_t)perm[i] : i;
+    double x, y, z;
+    if (pos4) {
+        float4 p = pos4[j];
+        x = p.x; y = p.y; z = p.z;
+    } else {
+        x = pos3[3 * j]; y = pos3[3 * j + 1]; z = pos3[3 * j + 2];
+    }
+    keys[i] = hash3(cell_of(x, g.bmin[0], g.scale), cell_of(y, g.bmin[1], g.scale),
+                    cell_of(z, g.bmin[2], g.scale), g.size);
+    vals[i] = (uint32_t)i;
+}
+
+// off[h] = lower_bound(sorted_keys, h) for h in [0, size]; each key boundary
+// fills the empty buckets up to it.
+__global__ void k_bucket_offsets(const uint32_t* __restrict__ keys, int64_t n, uint32_t size,
+                                 uint32_t* __restrict__ off) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) return;
+    int64_t lo = (j == 0) ? 0 : (int64_t)keys[j - 1] + 1;
+    int64_t hi = (j == n) ? (int64_t)size : (int64_t)keys[j];
+    for (int64_t h = lo; h <= hi; h++) off[h] = (uint32_t)j;
+}
+
+__device__ inline void box_cells(const double* bmin_items, const double* bmax_items, int64_t i,
+                                 const GridParams& g, int32_t lo[3], int32_t hi[3]) {
+    for (int d = 0; d < 3; d++) {
+        lo[d] = cell_of(bmin_items[3 * i + d], g.bmin[d], g.scale);
+        hi[d] = cell_of(bmax_items[3 * i + d], g.bmin[d], g.scale);
+    }
+}
+
+__global__ void k_box_counts(const double* __restrict__ bmin_items,
+                             const double* __restrict__ bmax_items, int64_t n, GridParams g,
+                             uint64_t* __restrict__ counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t lo[3], hi[3];
+    box_cells(bmin_items, bmax_items, i, g, lo, hi);
+    uint64_t c = 0;
+    if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2])
+        c = (uint64_t)(hi[0] - lo[0] + 1) * (uint64_t)(hi[1] - lo[1] + 1) * (uint64_t)(hi[2] - lo[2] + 1);
+    counts[i] = c;
+}
+
+// hash_grid_detail.h:44-50: z outer, then y, then x, bounds inclusive
+__global__ void k_box_pairs(const double* __restrict__ bmin_items,
+                            const double* __restrict__ bmax_items, int64_t n, GridParams g,
+                            const uint64_t* __restrict__ offsets, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t lo[3], hi[3];
+    box_cells(bmin_items, bmax_items, i, g, lo, hi);
+    uint64_t k = offsets[i];
+    for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
+        for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
+            for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
+                keys[k] = hash3(ix, iy, iz, g.size);
+                vals[k] = (uint32_t)i;
+                k++;
+            }
+}
+
+// exclusive scan of u64, 1024 items per block, two levels
+__global__ __launch_bounds__(256) void k_scan_u64_block(uint64_t* data, int64_t n,
+                                                        uint64_t* block_sums) {
+    __shared__ uint64_t s[256];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * 1024 + tid * 4;
+    uint64_t v[4], sum = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = base + k < n ? data[base + k] : 0;
+        sum += v[k];
+    }
+    s[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint64_t t = tid >= off ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += t;
+        __syncthreads();
+    }
+    uint64_t run = s[tid] - sum;
+    for (int k = 0; k < 4; k++) {
+        if (base + k < n) data[base + k] = run;
+        run += v[k];
+    }
+    if (tid == 255) block_sums[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_u64_add(uint64_t* data, int64_t n, const uint64_t* block_prefix) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) data[i] += block_prefix[i / 1024];
+}
+
+}  // namespace
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
+                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_point_keys, dim3(grid1(n, 256)), dim3(256), 0, s, pos4, pos3, perm, n, g,
+                       keys, vals);
+}
+
+void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
+                           uint32_t* off) {
+    hipLaunchKernelGGL(k_bucket_offsets, dim3(grid1(n + 1, 256)), dim3(256), 0, s, sorted_keys, n,
+                       size, off);
+}
+
+void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,
+                       int64_t n, GridParams g, uint64_t* counts) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_box_counts, dim3(grid1(n, 256)), dim3(256), 0, s, bmin_items, bmax_items,
+                       n, g, counts);
+}
+
+void launch_box_pairs(hipStream_t s, const double* bmin_items, const double* bmax_items,
+                      int64_t n, GridParams g, const uint64_t* offsets, uint32_t* keys,
+                      uint32_t* vals) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_box_pairs, dim3(grid1(n, 256)), dim3(256), 0, s, bmin_items, bmax_items,
+                       n, g, offsets, keys, vals);
+}
+
+// block_sums needs ceil(n/1024) + ceil(ceil(n/1024)/1024) + ... words
+void launch_exclusive_scan_u64(hipStream_t s, uint64_t* data, int64_t n, uint64_t* block_sums) {
+    if (n <= 0) return;
+    int64_t nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_u64_block, dim3((unsigned)nb), dim3(256), 0, s, data, n, block_sums);
+    if (nb > 1) {
+        launch_exclusive_scan_u64(s, block_sums, nb, block_sums + nb);
+        hipLaunchKernelGGL(k_scan_u64_add, dim3(grid1(n, 256)), dim3(256), 0, s, data, n,
+                           block_sums);
+    }
+}

@@ -1,0 +1,140 @@
+// internal.h -- host-side declarations shared by the translation units of
+// libpppm_hip.so (context, device buffers, kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "pppm_device.h"
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits.
+// ---------------------------------------------------------------------------
+#define RADIX_BLOCK 256
+#define RADIX_ITEMS 8
+#define RADIX_TILE (RADIX_BLOCK * RADIX_ITEMS)
+
+inline int64_t radix_blocks(int64_t n) { return (n + RADIX_TILE - 1) / RADIX_TILE; }
+// words of the histogram workspace for n keys (256 per tile + 256 totals)
+inline int64_t radix_hist_words(int64_t n) { return 256 * (radix_blocks(n) + 1) + 256; }
+
+// Sorts keys[0..n) by bits [0, bits), carrying vals.  Ping-pongs with the
+// *_alt buffers; returns true if the result ended in the *_alt buffers.
+bool radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                      uint32_t* vals_alt, uint32_t* hist, int64_t n, int bits);
+
+// ---------------------------------------------------------------------------
+// Hash grid kernels (hash_grid.hip)
+// ---------------------------------------------------------------------------
+struct GridParams {
+    double bmin[3];
+    double scale;
+    uint32_t size;
+};
+// keys[i] = hash(cell(pos[perm ? perm[i] : i])), vals[i] = i
+void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
+                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals);
+// off[h] = number of sorted keys < h, for h = 0..size
+void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
+                           uint32_t* off);
+// box insertion: per-item pair counts, then pairs in z, y, x order
+void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,
+                       int64_t n, GridParams g, uint64_t* counts);
+void launch_box_pairs(hipStream_t s, const double* bmin_items, const double* bmax_items,
+                      int64_t n, GridParams g, const uint64_t* offsets, uint32_t* keys,
+                      uint32_t* vals);
+void launch_exclusive_scan_u64(hipStream_t s, uint64_t* data, int64_t n, uint64_t* block_sums);
+
+// ---------------------------------------------------------------------------
+// BVH (bvh.hip)
+// ---------------------------------------------------------------------------
+struct BvhBuild {
+    // outputs
+    Node4* nodes;
+    float4* tris4;
+    int32_t root;
+    int64_t n_nodes, n_leaves;
+    int32_t depth;
+};
+// Builds the 4-wide BVH of n triangles (tri9: [n][9] floats on the device).
+// Allocates nodes / tris4 with hipMalloc (caller frees).  Returns an error
+// message, empty on success.
+std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, BvhBuild* out);
+void launch_intersect(hipStream_t s, SceneDev sc, const pppm_ray* rays, int64_t n, pppm_hit* hits);
+
+// ---------------------------------------------------------------------------
+// Render kernels (render.hip)
+// ---------------------------------------------------------------------------
+struct Counters {
+    unsigned long long photons_stored;
+    unsigned long long photon_segments;
+    unsigned long long eye_segments;
+    unsigned long long hitpoints;
+    unsigned long long gather_buckets;
+    unsigned long long gather_candidates;
+    unsigned long long gather_inradius;
+    unsigned long long stack_overflow;
+};
+
+struct PhotonPassArgs {
+    SceneDev sc;
+    int64_t n_photons;
+    int64_t cap;
+    uint64_t seed;
+    uint32_t pass;
+    uint32_t scene_id;
+    int32_t max_bounces;
+    float ray_eps;
+    float4* ph_pos;   // xyz, oct normal bits
+    float4* ph_flux;  // rgb, origin key bits (photon << 6 | bounce)
+    Counters* counters;
+};
+void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
+
+struct EyePassArgs {
+    SceneDev sc;
+    pppm_camera cam;
+    float cam_basis[12];  // center, D, U * tanh * aspect, V * tanh
+    uint64_t seed;
+    uint32_t pass;
+    uint32_t scene_id;
+    int32_t max_bounces;
+    float ray_eps;
+    float4* hp_pos;  // xyz, oct normal bits
+    float4* hp_w;    // weight rgb, pixel id bits
+    float* img_sum;
+    float* img_comp;
+    Counters* counters;
+};
+void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
+
+struct GatherArgs {
+    const float4* ph_pos;   // cell order
+    const float4* ph_flux;
+    const uint32_t* off;    // [S + 1]
+    const float4* hp_pos;
+    const float4* hp_w;
+    const Counters* hp_count_src;  // device hit point count (counters->hitpoints) or null
+    int64_t n_hit;                 // used when hp_count_src is null
+    GridParams g;
+    double radius;
+    float* img_sum;   // Kahan-accumulated image (null: write out_rgb)
+    float* img_comp;
+    float* out_rgb;   // [n_hit][3] raw estimates (test hook)
+    Counters* counters;
+};
+void launch_gather(hipStream_t s, const GatherArgs& a);
+
+// origin keys of the stored photons, and the permutation applied to the records
+void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
+                        uint32_t* vals);
+void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,
+                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out);
+void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,
+                    uint32_t* out);
+void launch_zero_u32(hipStream_t s, void* p, int64_t words);
+void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const float* n3,
+                        int64_t n, float4* pos4, float4* b4, int normals);

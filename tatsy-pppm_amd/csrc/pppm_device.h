@@ -1,0 +1,234 @@
+// pppm_device.h -- device-side types and helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pppm.h"
+
+#define PPPM_PI 3.14159265358979323846f
+#define PPPM_INV_PI 0.31830988618379067154f
+
+// ---------------------------------------------------------------------------
+// float3 arithmetic (explicit, so that FP contraction is visible at each site)
+// ---------------------------------------------------------------------------
+struct v3f {
+    float x, y, z;
+};
+__host__ __device__ inline v3f mk(float x, float y, float z) { return v3f{x, y, z}; }
+__host__ __device__ inline v3f operator+(v3f a, v3f b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline v3f operator-(v3f a, v3f b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline v3f operator*(v3f a, v3f b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__host__ __device__ inline v3f operator*(v3f a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline v3f operator-(v3f a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ inline float dot(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline v3f cross(v3f u, v3f v) {
+    return mk(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+__host__ __device__ inline float vmax3(v3f a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ inline v3f normalize(v3f a) {
+    float n2 = dot(a, a);
+    return n2 > 0.0f ? a * rsqrtf(n2) : a;
+}
+__device__ inline v3f xyz(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ inline float get(v3f a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon et al. 2011) counter-based RNG.
+// key = seed, counter = (index, scene, pass, kind << 24 | block); one block
+// gives four 24-bit uniforms.  Identical to the CPU oracle's stream.
+// ---------------------------------------------------------------------------
+__device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k.x += 0x9E3779B9u;
+            k.y += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+enum : uint32_t { KIND_PHOTON = 1u, KIND_EYE = 2u };
+// photon dims: 0 light pick, 1-2 position, 3-4 direction, 5-7 reserved,
+// bounce b: 8+3b .. 10+3b.  eye dims: 0-1 jitter, bounce b: 2+3b .. 4+3b.
+#define PH_DIM_BOUNCE 8
+#define EYE_DIM_BOUNCE 2
+
+struct Rng {
+    uint2 key;
+    uint4 ctr;
+    uint4 buf;
+    uint32_t block;
+
+    __device__ inline void init(uint64_t seed, uint32_t kind, uint32_t pass, uint32_t index,
+                                uint32_t scene) {
+        key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
+        ctr = make_uint4(index, scene, pass, kind << 24);
+        block = 0xFFFFFFFFu;
+    }
+    // uniform of dimension `dim` in [0, 1): top 24 bits * 2^-24
+    __device__ inline float at(int dim) {
+        uint32_t blk = (uint32_t)dim >> 2;
+        if (blk != block) {
+            uint4 c = ctr;
+            c.w |= blk;
+            buf = philox4x32_10(c, key);
+            block = blk;
+        }
+        uint32_t w = (dim & 3) == 0 ? buf.x : ((dim & 3) == 1 ? buf.y : ((dim & 3) == 2 ? buf.z : buf.w));
+        return (float)(w >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Octahedral unit-vector encoding, 2 x snorm16 packed in 32 bits.
+// ---------------------------------------------------------------------------
+__device__ inline float sgnf(float v) { return v < 0.0f ? -1.0f : 1.0f; }
+__device__ inline uint32_t oct_encode(v3f n) {
+    float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float x = n.x / s, y = n.y / s;
+    if (n.z < 0.0f) {
+        float ox = (1.0f - fabsf(y)) * sgnf(x);
+        float oy = (1.0f - fabsf(x)) * sgnf(y);
+        x = ox;
+        y = oy;
+    }
+    int qx = (int)rintf(fminf(fmaxf(x, -1.0f), 1.0f) * 32767.0f);
+    int qy = (int)rintf(fminf(fmaxf(y, -1.0f), 1.0f) * 32767.0f);
+    return ((uint32_t)(qx & 0xFFFF)) | ((uint32_t)(qy & 0xFFFF) << 16);
+}
+__device__ inline v3f oct_decode(uint32_t e) {
+    float x = (float)(int16_t)(e & 0xFFFF) * (1.0f / 32767.0f);
+    float y = (float)(int16_t)(e >> 16) * (1.0f / 32767.0f);
+    float z = 1.0f - fabsf(x) - fabsf(y);
+    if (z < 0.0f) {
+        float ox = (1.0f - fabsf(y)) * sgnf(x);
+        float oy = (1.0f - fabsf(x)) * sgnf(y);
+        x = ox;
+        y = oy;
+    }
+    return normalize(mk(x, y, z));
+}
+
+// ---------------------------------------------------------------------------
+// Device scene layout
+// ---------------------------------------------------------------------------
+// 4-wide BVH node, 128 bytes: child boxes in SoA FP32 (96 B) + 4 child refs.
+// ref >= 0: inner node index.  ref < 0: leaf, ~ref = (first << 3) | (count - 1)
+// over the leaf-ordered triangle array.  Empty slots have an inverted box.
+struct alignas(128) Node4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+
+#define LEAF_MAX 4
+#define EMPTY_REF 0x7FFFFFFF
+
+__host__ __device__ inline int32_t leaf_ref(int32_t first, int32_t count) {
+    return ~((first << 3) | (count - 1));
+}
+
+// Material in device form (48 bytes).
+struct Mat {
+    int32_t type;
+    float rho[3];
+    float le[3];
+    float rr_mean;  // (rho.x + rho.y + rho.z) / 3: photon Russian roulette
+    float rr_max;   // max(rho): eye Russian roulette
+    float ior;
+    float pad[2];
+};
+
+struct LightDev {
+    v3f corner, eu, ev, normal;
+    v3f flux;      // photon flux per emitted photon, set per render
+    float cdf_hi;  // cumulative selection probability (last = 1)
+};
+
+struct SceneDev {
+    const Node4* nodes;
+    const float4* tris;      // [3 * n] leaf order: v0 (w = original id bits), v1, v2
+    const int32_t* tri_mat;  // [n] original order
+    const Mat* mats;
+    const LightDev* lights;
+    int32_t n_lights;
+    int32_t root;            // root ref (inner index 0, a leaf ref, or EMPTY_REF)
+    int64_t n_tris;
+};
+
+struct Hit {
+    float t;
+    int32_t tri;  // leaf-order triangle index, -1 = miss
+    float u, v;
+};
+
+// Woop, Benthin, Wald (2013) watertight ray-triangle test in FP32.  No FP
+// contraction here: the edge functions of two triangles sharing an edge must
+// be exact negations of each other.
+struct RayPre {
+    v3f o;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ inline RayPre ray_pre(v3f o, v3f d) {
+    RayPre p;
+    p.o = o;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (get(d, kz) < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    p.kx = kx;
+    p.ky = ky;
+    p.kz = kz;
+    float dz = get(d, kz);
+    p.sx = get(d, kx) / dz;
+    p.sy = get(d, ky) / dz;
+    p.sz = 1.0f / dz;
+    return p;
+}
+
+__device__ inline bool tri_watertight(const RayPre& p, float4 v0, float4 v1, float4 v2, float tmin,
+                                      float tmax, float* t_out, float* u_out, float* v_out) {
+#pragma clang fp contract(off)
+    v3f A = xyz(v0) - p.o, B = xyz(v1) - p.o, C = xyz(v2) - p.o;
+    float Akz = get(A, p.kz), Bkz = get(B, p.kz), Ckz = get(C, p.kz);
+    float Ax = get(A, p.kx) - p.sx * Akz, Ay = get(A, p.ky) - p.sy * Akz;
+    float Bx = get(B, p.kx) - p.sx * Bkz, By = get(B, p.ky) - p.sy * Bkz;
+    float Cx = get(C, p.kx) - p.sx * Ckz, Cy = get(C, p.ky) - p.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = p.sz * Akz, Bz = p.sz * Bkz, Cz = p.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float sg = det < 0.0f ? -1.0f : 1.0f;
+    float Ts = T * sg, dets = det * sg;
+    if (Ts <= tmin * dets || Ts > tmax * dets) return false;
+    float rcp = 1.0f / det;
+    *t_out = T * rcp;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}

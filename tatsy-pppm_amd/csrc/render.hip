@@ -1,0 +1,427 @@
+// render.hip -- the probabilistic PPM pass on gfx950: photon pass, eye pass,
+// fixed-radius gather with Kahan accumulation, and the small kernels between.
+//
+// Restates sources/ppm_probabilistic.cc:166-387 (tracePhotons, traceRays,
+// radiance) in the P-KZ form: area-light emission, photons stored at
+// Lambertian hits with Russian roulette by mean albedo, mirror / dielectric
+// continuation, eye Russian roulette from bounce 3, one hit point per pixel
+// per pass, and a fixed-radius cone-filtered density estimate.
+#include "internal.h"
+#include "traverse.h"
+
+namespace {
+
+__device__ inline uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Append one record per active lane with one atomic per wave; returns the slot.
+__device__ inline unsigned long long wave_append(unsigned long long* counter) {
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (unsigned long long)__popcll(m & lanemask_lt());
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// sampler.cc:17-32 (cosine-weighted direction about w)
+__device__ inline v3f on_hemisphere(v3f w, float r1, float r2) {
+    v3f u = fabsf(w.x) > 1.0e-12f ? normalize(cross(mk(0.0f, 1.0f, 0.0f), w))
+                                  : normalize(cross(mk(1.0f, 0.0f, 0.0f), w));
+    v3f v = cross(w, u);
+    float s, c;
+    __sincosf(2.0f * PPPM_PI * r1, &s, &c);
+    float z2s = sqrtf(r2);
+    return normalize(u * (c * z2s) + v * (s * z2s) + w * sqrtf(1.0f - r2));
+}
+
+__device__ inline v3f reflect(v3f v, v3f n) { return v - n * (2.0f * dot(n, v)); }
+
+// brdf.cc:117-142 with reflectance.h:7-34: returns the new direction, *pdf
+__device__ inline v3f refraction_sample(v3f in, v3f n, float ior, float r1, float* pdf) {
+    bool into = true;
+    v3f on = n;
+    if (dot(in, n) >= 0.0f) {
+        into = false;
+        on = -n;
+    }
+    v3f rd = reflect(in, n);
+    float nnt = into ? 1.0f / ior : ior;
+    float ddn = dot(in, on);
+    float cos2t = 1.0f - nnt * nnt * (1.0f - ddn * ddn);
+    if (cos2t < 0.0f) {
+        *pdf = 1.0f;
+        return rd;
+    }
+    v3f td = normalize(in * nnt - n * ((into ? 1.0f : -1.0f) * (ddn * nnt + sqrtf(cos2t))));
+    float a = ior - 1.0f, b = ior + 1.0f;
+    float R0 = (a * a) / (b * b);
+    float c = 1.0f - (into ? -ddn : dot(td, -on));
+    float c2 = c * c;
+    float fre = R0 + (1.0f - R0) * (c2 * c2 * c);
+    float ftr = 1.0f - fre;
+    const float rp = 0.5f;  // 0.25 + 0.5 * REFLECT_PROBABILITY (brdf.cc:133)
+    if (r1 < rp) {
+        *pdf = rp / fre;
+        return rd;
+    }
+    *pdf = (1.0f - rp) / ftr;
+    return td;
+}
+
+__device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
+    return dot(dir, ng) >= 0.0f ? p + ng * eps : p - ng * eps;
+}
+
+#define TRACE_TMAX 1.0e30f
+
+__global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
+    __shared__ int32_t s_stack[STACK_LDS * 256];
+    int32_t* stk = s_stack + threadIdx.x;
+    unsigned long long segs = 0;
+    bool ovf = false;
+    const SceneDev& sc = a.sc;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n_photons;
+         i += (int64_t)gridDim.x * 256) {
+        Rng rng;
+        rng.init(a.seed, KIND_PHOTON, a.pass, (uint32_t)i, a.scene_id);
+        // emission: light by power, uniform point, cosine direction
+        float ul = rng.at(0);
+        int l = 0;
+        while (l < sc.n_lights - 1 && sc.lights[l].cdf_hi <= ul) l++;
+        const LightDev L = sc.lights[l];
+        v3f pos = L.corner + L.eu * rng.at(1) + L.ev * rng.at(2);
+        v3f dir = on_hemisphere(L.normal, rng.at(3), rng.at(4));
+        v3f flux = L.flux;
+        v3f o = pos + L.normal * a.ray_eps;
+        for (int bounce = 0;; bounce++) {
+            if (bounce >= a.max_bounces || vmax3(flux) <= 0.0f) break;
+            Hit h = trace_closest<256>(sc, o, dir, 0.0f, TRACE_TMAX, stk, &ovf);
+            segs++;
+            if (h.tri < 0) break;
+            const int dd = PH_DIM_BOUNCE + 3 * bounce;
+            const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
+            const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
+            const v3f hp = o + dir * h.t;
+            const v3f ng = tri_normal(sc, h.tri);
+            const v3f on = dot(ng, dir) < 0.0f ? ng : -ng;
+            if (m.type & PPPM_BSDF_EMITTER) break;
+            if (m.type & PPPM_BSDF_LAMBERTIAN) {
+                unsigned long long slot = wave_append(&a.counters->photons_stored);
+                if (slot < (unsigned long long)a.cap) {
+                    a.ph_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                    a.ph_flux[slot] = make_float4(flux.x, flux.y, flux.z,
+                                                  __uint_as_float(((uint32_t)i << 6) | (uint32_t)bounce));
+                }
+                if (r0 < m.rr_mean) {
+                    dir = on_hemisphere(on, r1, r2);
+                    o = spawn(hp, ng, dir, a.ray_eps);
+                    flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / m.rr_mean);
+                } else {
+                    break;
+                }
+            } else {
+                float pdf = 1.0f;
+                v3f nd;
+                if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(dir, on);
+                else nd = refraction_sample(dir, ng, m.ior, r0, &pdf);
+                o = spawn(hp, ng, nd, a.ray_eps);
+                dir = nd;
+                flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / pdf);
+            }
+        }
+    }
+    segs = wave_sum(segs);
+    uint64_t any_ovf = __ballot(ovf);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&a.counters->photon_segments, segs);
+        if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
+    }
+}
+
+__device__ inline void kahan_add(float* sum, float* comp, int64_t k, float v) {
+    float s = sum[k], c = comp[k];
+    float y = v - c;
+    float t = s + y;
+    comp[k] = (t - s) - y;
+    sum[k] = t;
+}
+
+__global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
+    __shared__ int32_t s_stack[STACK_LDS * 256];
+    int32_t* stk = s_stack + threadIdx.x;
+    unsigned long long segs = 0;
+    bool ovf = false;
+    const SceneDev& sc = a.sc;
+    const int W = a.cam.width, H = a.cam.height;
+    const int64_t np = (int64_t)W * H;
+    const v3f C = mk(a.cam_basis[0], a.cam_basis[1], a.cam_basis[2]);
+    const v3f D = mk(a.cam_basis[3], a.cam_basis[4], a.cam_basis[5]);
+    const v3f U = mk(a.cam_basis[6], a.cam_basis[7], a.cam_basis[8]);
+    const v3f V = mk(a.cam_basis[9], a.cam_basis[10], a.cam_basis[11]);
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < np;
+         p += (int64_t)gridDim.x * 256) {
+        const int x = (int)(p % W), row = (int)(p / W);
+        Rng rng;
+        rng.init(a.seed, KIND_EYE, a.pass, (uint32_t)p, a.scene_id);
+        const float px = (float)x + rng.at(0);
+        const float py = (float)(H - 1 - row) + rng.at(1);
+        v3f d = normalize(D + U * (2.0f * px / (float)W - 1.0f) + V * (2.0f * py / (float)H - 1.0f));
+        v3f o = C;
+        v3f thr = mk(1.0f, 1.0f, 1.0f);
+        v3f direct = mk(0.0f, 0.0f, 0.0f);
+        for (int bounce = 0;; bounce++) {
+            if (bounce >= a.max_bounces) break;
+            Hit h = trace_closest<256>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf);
+            segs++;
+            if (h.tri < 0) break;
+            const int dd = EYE_DIM_BOUNCE + 3 * bounce;
+            const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
+            (void)r2;
+            const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
+            const v3f hp = o + d * h.t;
+            const v3f ng = tri_normal(sc, h.tri);
+            if (m.type & PPPM_BSDF_EMITTER) {
+                if (dot(d, ng) < 0.0f) direct = thr * mk(m.le[0], m.le[1], m.le[2]);
+                break;
+            }
+            float roulette = m.rr_max;
+            if (bounce >= 3) {
+                if (r0 > roulette) break;
+            } else {
+                roulette = 1.0f;
+            }
+            const v3f on = dot(ng, d) < 0.0f ? ng : -ng;
+            const v3f rho = mk(m.rho[0], m.rho[1], m.rho[2]);
+            if (m.type & PPPM_BSDF_LAMBERTIAN) {
+                const v3f w = thr * rho * (PPPM_INV_PI / roulette);
+                unsigned long long slot = wave_append(&a.counters->hitpoints);
+                a.hp_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                a.hp_w[slot] = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
+                break;
+            }
+            float pdf = 1.0f;
+            v3f nd;
+            if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(d, on);
+            else nd = refraction_sample(d, ng, m.ior, r1, &pdf);
+            thr = thr * rho * (1.0f / (pdf * roulette));
+            o = spawn(hp, ng, nd, a.ray_eps);
+            d = nd;
+        }
+        if (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f) {
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 0, direct.x);
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 1, direct.y);
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 2, direct.z);
+        }
+    }
+    segs = wave_sum(segs);
+    uint64_t any_ovf = __ballot(ovf);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&a.counters->eye_segments, segs);
+        if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
+    }
+}
+
+__device__ inline int32_t cell_of(double x, double mn, double scale) {
+#pragma clang fp contract(off)
+    double b = (x - mn) * scale;
+    int32_t c = (int32_t)b;
+    return c < 0 ? -c : c;
+}
+
+__device__ inline uint32_t hash3(int32_t ix, int32_t iy, int32_t iz, uint32_t size) {
+    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    return h % size;
+}
+
+#define CONE_K 1.1f
+
+// One hit point per lane; walks the de-duplicated buckets of the cells the
+// ball of radius r overlaps (cell edge 2r, so at most 2 per axis).
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    unsigned long long nb = 0, nc = 0, ni = 0;
+    if (i < n) {
+        const float4 hp = a.hp_pos[i];
+        const float4 hw = a.hp_w[i];
+        const v3f x = xyz(hp);
+        const v3f nx = oct_decode(__float_as_uint(hp.w));
+        const double r = a.radius;
+        int32_t lo[3], hi[3];
+        const double xs[3] = {hp.x, hp.y, hp.z};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = cell_of(xs[d] - r, a.g.bmin[d], a.g.scale);
+            hi[d] = cell_of(xs[d] + r, a.g.bmin[d], a.g.scale);
+        }
+        uint32_t seen[8];
+        int ns = 0;
+        const float r2 = (float)(r * r);
+        const float inv_kr = (float)(1.0 / (1.1 * r));
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
+            for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
+                for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
+                    const uint32_t h = hash3(ix, iy, iz, a.g.size);
+                    bool dup = false;
+                    for (int k = 0; k < ns; k++) dup |= (seen[k] == h);
+                    if (dup || ns >= 8) continue;
+                    seen[ns++] = h;
+                    nb++;
+                    const uint32_t b0 = a.off[h], b1 = a.off[h + 1];
+                    nc += b1 - b0;
+                    for (uint32_t j = b0; j < b1; j++) {
+                        const float4 pp = a.ph_pos[j];
+                        const float dx = x.x - pp.x, dy = x.y - pp.y, dz = x.z - pp.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 > r2) continue;
+                        // sign of n_p . n_x needs no normalisation of the decoded n_p
+                        const uint32_t e = __float_as_uint(pp.w);
+                        float qx = (float)(int16_t)(e & 0xFFFF), qy = (float)(int16_t)(e >> 16);
+                        qx *= (1.0f / 32767.0f);
+                        qy *= (1.0f / 32767.0f);
+                        float qz = 1.0f - fabsf(qx) - fabsf(qy);
+                        if (qz < 0.0f) {
+                            const float ox = (1.0f - fabsf(qy)) * sgnf(qx);
+                            const float oy = (1.0f - fabsf(qx)) * sgnf(qy);
+                            qx = ox;
+                            qy = oy;
+                        }
+                        if (qx * nx.x + qy * nx.y + qz * nx.z <= 0.0f) continue;
+                        ni++;
+                        const float4 f = a.ph_flux[j];
+                        const float wt = 1.0f - sqrtf(d2) * inv_kr;
+                        ax += wt * f.x;
+                        ay += wt * f.y;
+                        az += wt * f.z;
+                    }
+                }
+        const float inv_norm = (float)(1.0 / (3.14159265358979323846 * r * r * (1.0 - 2.0 / (3.0 * 1.1))));
+        const float Lx = hw.x * ax * inv_norm, Ly = hw.y * ay * inv_norm, Lz = hw.z * az * inv_norm;
+        if (a.img_sum) {
+            const int64_t pix = (int64_t)__float_as_uint(hw.w);
+            kahan_add(a.img_sum, a.img_comp, 3 * pix + 0, Lx);
+            kahan_add(a.img_sum, a.img_comp, 3 * pix + 1, Ly);
+            kahan_add(a.img_sum, a.img_comp, 3 * pix + 2, Lz);
+        } else {
+            a.out_rgb[3 * i + 0] = Lx;
+            a.out_rgb[3 * i + 1] = Ly;
+            a.out_rgb[3 * i + 2] = Lz;
+        }
+    }
+    nb = wave_sum(nb);
+    nc = wave_sum(nc);
+    ni = wave_sum(ni);
+    if ((threadIdx.x & 63) == 0 && a.counters) {
+        atomicAdd(&a.counters->gather_buckets, nb);
+        atomicAdd(&a.counters->gather_candidates, nc);
+        atomicAdd(&a.counters->gather_inradius, ni);
+    }
+}
+
+__global__ void k_origin_keys(const float4* __restrict__ ph_flux, int64_t n, uint32_t* keys,
+                              uint32_t* vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = __float_as_uint(ph_flux[i].w);
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void k_permute2(const float4* __restrict__ a_in, const float4* __restrict__ b_in,
+                           const uint32_t* __restrict__ perm, const uint32_t* __restrict__ perm2,
+                           int64_t n, float4* __restrict__ a_out, float4* __restrict__ b_out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t j = perm2 ? perm2[k] : (uint32_t)k;
+    uint32_t src = perm ? perm[j] : j;
+    a_out[k] = a_in[src];
+    b_out[k] = b_in[src];
+}
+
+__global__ void k_compose(const uint32_t* perm1, const uint32_t* perm2, int64_t n, uint32_t* out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = perm1[perm2[k]];
+}
+
+__global__ void k_zero(uint32_t* p, int64_t words) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// [n][3] float arrays -> float4 records (w = oct normal bits when normals != 0)
+__global__ void k_pack(const float* pos3, const float* b3, const float* n3, int64_t n, float4* pos4,
+                       float4* b4, int normals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t e = 0;
+    if (normals) e = oct_encode(normalize(mk(n3[3 * i], n3[3 * i + 1], n3[3 * i + 2])));
+    pos4[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], __uint_as_float(e));
+    b4[i] = make_float4(b3[3 * i], b3[3 * i + 1], b3[3 * i + 2], __uint_as_float((uint32_t)i));
+}
+
+}  // namespace
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit) {
+    if (a.n_photons <= 0) return;
+    unsigned g = grid1(a.n_photons, 256);
+    if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
+    hipLaunchKernelGGL(k_photon, dim3(g), dim3(256), 0, s, a);
+}
+
+void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
+    int64_t np = (int64_t)a.cam.width * a.cam.height;
+    if (np <= 0) return;
+    unsigned g = grid1(np, 256);
+    if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
+    hipLaunchKernelGGL(k_eye, dim3(g), dim3(256), 0, s, a);
+}
+
+void launch_gather(hipStream_t s, const GatherArgs& a) {
+    // with a device-side count the grid covers the worst case (one per pixel)
+    if (a.n_hit <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
+}
+
+void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
+                        uint32_t* vals) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_origin_keys, dim3(grid1(n, 256)), dim3(256), 0, s, ph_flux, n, keys, vals);
+}
+
+void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,
+                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_permute2, dim3(grid1(n, 256)), dim3(256), 0, s, a_in, b_in, perm, perm2, n,
+                       a_out, b_out);
+}
+
+void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,
+                    uint32_t* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_compose, dim3(grid1(n, 256)), dim3(256), 0, s, perm1, perm2, n, out);
+}
+
+void launch_zero_u32(hipStream_t s, void* p, int64_t words) {
+    if (words <= 0) return;
+    int64_t g = grid1(words, 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)g), dim3(256), 0, s, (uint32_t*)p, words);
+}
+
+void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const float* n3,
+                        int64_t n, float4* pos4, float4* b4, int normals) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack, dim3(grid1(n, 256)), dim3(256), 0, s, pos3, a3, n3, n, pos4, b4,
+                       normals);
+}

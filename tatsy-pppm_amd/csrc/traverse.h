@@ -1,0 +1,121 @@
+// traverse.h -- closest-hit traversal of the 4-wide BVH (device code).
+//
+// One ray per lane.  The traversal stack keeps its top STACK_LDS entries in
+// LDS (laid out [slot][lane] so the 64 lanes of a wave hit 64 different banks)
+// and spills deeper entries to a per-lane private array.  Nodes are fetched as
+// seven 16-byte loads from one 128-byte line; children are visited near-first.
+#pragma once
+
+#include "pppm_device.h"
+
+#define STACK_LDS 16
+#define STACK_OVF 112
+#define STACK_MAX (STACK_LDS + STACK_OVF)
+
+// Conservative slab test (Ize 2013): the exit distance is scaled up by
+// 1 + 2^-22 so FP32 rounding cannot cull a box the triangle test would hit.
+#define SLAB_FAR_SCALE 1.00000024f
+
+template <int BLOCK>
+__device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin, float tmax,
+                                    int32_t* __restrict__ stk, bool* overflow) {
+    Hit h;
+    h.t = tmax;
+    h.tri = -1;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    if (sc.root == EMPTY_REF) return h;
+    const RayPre pre = ray_pre(o, d);
+    // inverse direction as the reference computes it (qbvh_accel.cc:219-221)
+    const float ix = d.x == 0.0f ? 1.0e20f : 1.0f / d.x;
+    const float iy = d.y == 0.0f ? 1.0e20f : 1.0f / d.y;
+    const float iz = d.z == 0.0f ? 1.0e20f : 1.0f / d.z;
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    int32_t ovf[STACK_OVF];
+    int sp = 0;
+    int32_t cur = sc.root;
+    while (true) {
+        if (cur >= 0) {
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const int4 ch = reinterpret_cast<const int4*>(np)[6];
+            float tn[4], tf[4];
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                float x0 = lxa[c] * ix - ox, x1 = hxa[c] * ix - ox;
+                float y0 = lya[c] * iy - oy, y1 = hya[c] * iy - oy;
+                float z0 = lza[c] * iz - oz, z1 = hza[c] * iz - oz;
+                float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+                float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h.t));
+                tn[c] = (n <= f * SLAB_FAR_SCALE) ? n : __builtin_inff();
+                tf[c] = f;
+            }
+            int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+            float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
+            // sorting network on (key, ref), ascending
+#define CSWAP(ka, ra, kb, rb)                     \
+    if (kb < ka) {                                \
+        float tk = ka; ka = kb; kb = tk;          \
+        int32_t tr = ra; ra = rb; rb = tr;        \
+    }
+            CSWAP(k0, r0, k1, r1);
+            CSWAP(k2, r2, k3, r3);
+            CSWAP(k0, r0, k2, r2);
+            CSWAP(k1, r1, k3, r3);
+            CSWAP(k1, r1, k2, r2);
+#undef CSWAP
+            const float INF = __builtin_inff();
+            int32_t push[3] = {r3, r2, r1};
+            bool pv[3] = {k3 != INF, k2 != INF, k1 != INF};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                if (pv[q]) {
+                    if (sp < STACK_LDS) {
+                        stk[sp * BLOCK] = push[q];
+                    } else if (sp < STACK_MAX) {
+                        ovf[sp - STACK_LDS] = push[q];
+                    } else {
+                        *overflow = true;
+                        sp--;
+                    }
+                    sp++;
+                }
+            }
+            if (k0 != INF) {
+                cur = r0;
+                continue;
+            }
+        } else if (cur != EMPTY_REF) {
+            const uint32_t v = (uint32_t)(~cur);
+            const int32_t first = (int32_t)(v >> 3);
+            const int32_t cnt = (int32_t)(v & 7u) + 1;
+            for (int k = 0; k < cnt; k++) {
+                const float4* tp = sc.tris + 3 * (int64_t)(first + k);
+                float t, u, w;
+                if (tri_watertight(pre, tp[0], tp[1], tp[2], tmin, h.t, &t, &u, &w)) {
+                    h.t = t;
+                    h.tri = first + k;
+                    h.u = u;
+                    h.v = w;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        cur = sp < STACK_LDS ? stk[sp * BLOCK] : ovf[sp - STACK_LDS];
+    }
+    return h;
+}
+
+// Geometric normal of a leaf-order triangle (triangle.cc:45-47).
+__device__ inline v3f tri_normal(const SceneDev& sc, int32_t t) {
+    const float4* tp = sc.tris + 3 * (int64_t)t;
+    v3f a = xyz(tp[0]), b = xyz(tp[1]), c = xyz(tp[2]);
+    return normalize(cross(b - a, c - a));
+}
+__device__ inline int32_t tri_orig_id(const SceneDev& sc, int32_t t) {
+    return __float_as_int(sc.tris[3 * (int64_t)t].w);
+}

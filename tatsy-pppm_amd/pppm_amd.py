@@ -1,0 +1,329 @@
+"""ctypes binding of libpppm_hip.so (the C ABI in include/pppm.h).
+
+Thin plumbing for Python callers (tests, bench.py): numpy arrays in and out,
+no compute here.  The library is loaded from this directory; if it is absent
+the import fails loudly -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpppm_hip.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
+LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
+SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
+
+
+class Bsdf(C.Structure):
+    _fields_ = [("type", C.c_int32), ("reflectance", C.c_float * 3), ("emission", C.c_float * 3),
+                ("ior", C.c_float), ("sigma_a", C.c_float * 3), ("sigmap_s", C.c_float * 3),
+                ("eta", C.c_float), ("pad", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("corner", C.c_float * 3), ("edge_u", C.c_float * 3), ("edge_v", C.c_float * 3),
+                ("emission", C.c_float * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("direction", C.c_double * 3), ("up", C.c_double * 3),
+                ("fov_y_deg", C.c_double), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("leaves", C.c_int64), ("depth", C.c_int32),
+                ("max_leaf", C.c_int32), ("build_ms", C.c_double)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("photons", C.c_int64), ("passes", C.c_int32), ("max_bounces", C.c_int32),
+                ("alpha", C.c_double), ("radius1", C.c_double), ("ray_eps", C.c_double),
+                ("seed", C.c_uint64), ("scene_id", C.c_uint32), ("hash_size", C.c_int32),
+                ("capacity_factor", C.c_double)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("pass_index", C.c_int32), ("hash_size", C.c_int32), ("radius", C.c_double),
+                ("photons_emitted", C.c_int64), ("photons_stored", C.c_int64),
+                ("photon_segments", C.c_int64), ("eye_segments", C.c_int64),
+                ("hitpoints", C.c_int64), ("gather_buckets", C.c_int64),
+                ("gather_candidates", C.c_int64), ("gather_inradius", C.c_int64),
+                ("photon_ms", C.c_double), ("order_ms", C.c_double), ("grid_ms", C.c_double),
+                ("eye_ms", C.c_double), ("gather_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n_tris", C.c_int64), ("tri_xyz", C.POINTER(C.c_float)),
+                ("bsdf_id", C.POINTER(C.c_int32)), ("n_bsdf", C.c_int32),
+                ("bsdfs", C.POINTER(Bsdf)), ("n_lights", C.c_int32),
+                ("lights", C.POINTER(Light)), ("camera", Camera),
+                ("bbox_min", C.c_double * 3), ("bbox_max", C.c_double * 3)]
+
+
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("tri", "<i4"), ("u", "<f4"), ("v", "<f4")])
+
+_P = C.c_void_p
+_DECLS = {
+    "pppm_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32]),
+    "pppm_destroy": (C.c_int, [_P]),
+    "pppm_last_error": (C.c_char_p, [_P]),
+    "pppm_device_info": (C.c_int, [_P, C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "pppm_scene_upload": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(Bsdf), C.c_int32, C.POINTER(Light), C.c_int32]),
+    "pppm_bvh_build": (C.c_int, [_P, C.POINTER(BvhInfo)]),
+    "pppm_intersect": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "pppm_hashgrid_build": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
+    "pppm_hashgrid_build_boxes": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P,
+                                            C.c_int64, C.POINTER(C.c_int64), _P, _P]),
+    "pppm_radius_schedule": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, _P]),
+    "pppm_camera_set": (C.c_int, [_P, C.POINTER(Camera)]),
+    "pppm_render_begin": (C.c_int, [_P, C.POINTER(RenderParams)]),
+    "pppm_render_pass": (C.c_int, [_P, C.POINTER(PassStats)]),
+    "pppm_image_read": (C.c_int, [_P, _P, C.c_int64]),
+    "pppm_last_photons": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "pppm_last_grid": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, C.POINTER(C.c_double)]),
+    "pppm_last_hitpoints": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
+                              C.c_int32, _P, _P, C.POINTER(PassStats)]),
+    "pppm_scenegen": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SceneDesc)]),
+    "pppm_scenegen_free": (None, [C.POINTER(SceneDesc)]),
+}
+EXPORTED = tuple(_DECLS)
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() (no fallback exists)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _DECLS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class PPPMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pppm error {code}: {msg}")
+        self.code = code
+
+
+class Scene:
+    """A procedural scene as numpy arrays (copied out of the library)."""
+
+    def __init__(self, tri, bsdf_id, bsdfs, lights, camera, bbox_min, bbox_max):
+        self.tri = tri
+        self.bsdf_id = bsdf_id
+        self.bsdfs = bsdfs
+        self.lights = lights
+        self.camera = camera
+        self.bbox_min = bbox_min
+        self.bbox_max = bbox_max
+
+    @property
+    def n_tris(self):
+        return len(self.bsdf_id)
+
+    def diagonal(self):
+        return float(np.linalg.norm(np.asarray(self.bbox_max) - np.asarray(self.bbox_min)))
+
+    def bsdf_array(self):
+        return (Bsdf * len(self.bsdfs))(*self.bsdfs)
+
+    def light_array(self):
+        return (Light * len(self.lights))(*self.lights)
+
+
+def scenegen(kind, seed=1, frequency=0, width=256, height=256):
+    d = SceneDesc()
+    rc = lib().pppm_scenegen(kind, seed, frequency, width, height, C.byref(d))
+    if rc != OK:
+        raise PPPMError(rc, "scenegen failed")
+    try:
+        n = d.n_tris
+        tri = np.ctypeslib.as_array(d.tri_xyz, shape=(n, 9)).copy() if n else np.zeros((0, 9), np.float32)
+        ids = np.ctypeslib.as_array(d.bsdf_id, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+        bsdfs = [Bsdf.from_buffer_copy(d.bsdfs[i]) for i in range(d.n_bsdf)]
+        lights = [Light.from_buffer_copy(d.lights[i]) for i in range(d.n_lights)]
+        cam = Camera.from_buffer_copy(d.camera)
+        s = Scene(tri, ids, bsdfs, lights, cam, tuple(d.bbox_min), tuple(d.bbox_max))
+    finally:
+        lib().pppm_scenegen_free(C.byref(d))
+    return s
+
+
+def radius_schedule(r1, alpha, n, literal=False):
+    out = np.zeros(n, np.float64)
+    rc = lib().pppm_radius_schedule(r1, alpha, n, 1 if literal else 0, _ptr(out))
+    if rc != OK:
+        raise PPPMError(rc, "bad radius schedule arguments")
+    return out
+
+
+class Context:
+    """One device context (pppm_create / pppm_destroy)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.pppm_create(C.byref(h), device)
+        if rc != OK:
+            raise PPPMError(rc, self.L.pppm_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.pppm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise PPPMError(rc, self.L.pppm_last_error(self.h).decode())
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int32()
+        hbm = C.c_int64()
+        self._chk(self.L.pppm_device_info(self.h, name, 256, C.byref(cu), C.byref(hbm)))
+        return {"name": name.value.decode(), "cu": cu.value, "hbm_bytes": hbm.value}
+
+    def upload(self, scene):
+        tri = np.ascontiguousarray(scene.tri, np.float32)
+        ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
+        self._chk(self.L.pppm_scene_upload(self.h, _ptr(tri), _ptr(ids), len(ids), scene.bsdf_array(),
+                                           len(scene.bsdfs), scene.light_array(), len(scene.lights)))
+
+    def build_bvh(self):
+        info = BvhInfo()
+        self._chk(self.L.pppm_bvh_build(self.h, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_}
+
+    def intersect(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        self._chk(self.L.pppm_intersect(self.h, _ptr(rays), len(rays), _ptr(hits)))
+        return hits
+
+    def hashgrid(self, pos, hash_size, hash_scale, bbox_min):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = len(pos)
+        sidx = np.zeros(max(n, 1), np.int32)
+        cs = np.zeros(hash_size, np.int32)
+        ce = np.zeros(hash_size, np.int32)
+        bm = np.ascontiguousarray(bbox_min, np.float64)
+        self._chk(self.L.pppm_hashgrid_build(self.h, _ptr(pos), n, hash_size, hash_scale, _ptr(bm),
+                                             _ptr(sidx), _ptr(cs), _ptr(ce)))
+        return sidx[:n], cs, ce
+
+    def hashgrid_boxes(self, bmin, bmax, hash_size, hash_scale, bbox_min, cap):
+        bmin = np.ascontiguousarray(bmin, np.float64).reshape(-1, 3)
+        bmax = np.ascontiguousarray(bmax, np.float64).reshape(-1, 3)
+        items = np.zeros(max(cap, 1), np.int32)
+        cs = np.zeros(hash_size, np.int32)
+        ce = np.zeros(hash_size, np.int32)
+        npairs = C.c_int64()
+        bm = np.ascontiguousarray(bbox_min, np.float64)
+        self._chk(self.L.pppm_hashgrid_build_boxes(self.h, _ptr(bmin), _ptr(bmax), len(bmin), hash_size,
+                                                   hash_scale, _ptr(bm), _ptr(items), cap,
+                                                   C.byref(npairs), _ptr(cs), _ptr(ce)))
+        return items[:npairs.value], cs, ce
+
+    def set_camera(self, cam):
+        self._chk(self.L.pppm_camera_set(self.h, C.byref(cam)))
+        self.camera = cam
+
+    def begin(self, photons, passes, radius1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
+              hash_size=0, max_bounces=64, capacity_factor=8.0):
+        p = RenderParams(photons=photons, passes=passes, max_bounces=max_bounces, alpha=alpha,
+                         radius1=radius1, ray_eps=ray_eps, seed=seed, scene_id=scene_id,
+                         hash_size=hash_size, capacity_factor=capacity_factor)
+        self._chk(self.L.pppm_render_begin(self.h, C.byref(p)))
+        self.params = p
+
+    def render_pass(self):
+        st = PassStats()
+        self._chk(self.L.pppm_render_pass(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def image(self):
+        w, h = self.camera.width, self.camera.height
+        out = np.zeros((h, w, 3), np.float64)
+        self._chk(self.L.pppm_image_read(self.h, _ptr(out), out.size))
+        return out
+
+    def last_photons(self, cap=None):
+        n = C.c_int64()
+        self._chk(self.L.pppm_last_photons(self.h, None, None, None, None, 0, C.byref(n)))
+        m = n.value if cap is None else min(cap, n.value)
+        pos = np.zeros((max(m, 1), 3), np.float32)
+        flux = np.zeros((max(m, 1), 3), np.float32)
+        nrm = np.zeros((max(m, 1), 3), np.float32)
+        key = np.zeros(max(m, 1), np.uint32)
+        self._chk(self.L.pppm_last_photons(self.h, _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), m,
+                                           C.byref(n)))
+        return pos[:m], flux[:m], nrm[:m], key[:m]
+
+    def last_grid(self, n, hash_size):
+        sidx = np.zeros(max(n, 1), np.int32)
+        cs = np.zeros(hash_size, np.int32)
+        ce = np.zeros(hash_size, np.int32)
+        bm = np.zeros(3, np.float64)
+        sc = C.c_double()
+        self._chk(self.L.pppm_last_grid(self.h, _ptr(sidx), _ptr(cs), _ptr(ce), hash_size, _ptr(bm),
+                                        C.byref(sc)))
+        return sidx[:n], cs, ce, bm, sc.value
+
+    def last_hitpoints(self):
+        n = C.c_int64()
+        self._chk(self.L.pppm_last_hitpoints(self.h, None, None, None, None, 0, C.byref(n)))
+        m = n.value
+        pos = np.zeros((max(m, 1), 3), np.float32)
+        nrm = np.zeros((max(m, 1), 3), np.float32)
+        w = np.zeros((max(m, 1), 3), np.float32)
+        pix = np.zeros(max(m, 1), np.int32)
+        self._chk(self.L.pppm_last_hitpoints(self.h, _ptr(pos), _ptr(nrm), _ptr(w), _ptr(pix), m,
+                                             C.byref(n)))
+        return pos[:m], nrm[:m], w[:m], pix[:m]
+
+    def gather(self, ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w, radius, hash_size, bbox_min):
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+             for x in (ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w)]
+        out = np.zeros((max(len(a[3]), 1), 3), np.float32)
+        st = PassStats()
+        bm = np.ascontiguousarray(bbox_min, np.float64)
+        self._chk(self.L.pppm_gather(self.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), len(a[0]), _ptr(a[3]),
+                                     _ptr(a[4]), _ptr(a[5]), len(a[3]), radius, hash_size, _ptr(bm),
+                                     _ptr(out), C.byref(st)))
+        return out[:len(a[3])], st.as_dict()

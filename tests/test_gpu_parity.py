@@ -1,0 +1,323 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same
+seeded inputs, plus size-independent properties at full benchmark sizes.
+
+Bars: bit-exact for the hash grid (integer/index work); stated tolerances for
+floating point (FP32 GPU against the FP64 oracle), written next to each check.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pppm_amd as P
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-4  # continuation-ray offset used throughout (scene units)
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def random_rays(rng, n, lo, hi):
+    rays = np.zeros(n, P.RAY_DTYPE)
+    rays["origin"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["dir"] = d
+    rays["tmin"] = 0.0
+    rays["tmax"] = 1e30
+    return rays
+
+
+# ---------------------------------------------------------------------------
+# BVH + traversal
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind,freq", [(P.SCENE_BOX, 8), (P.SCENE_BOX_CAUSTIC, 24), (P.SCENE_TWO_ROOMS, 40)])
+def test_intersect_matches_brute_force(gpu_ctx, kind, freq):
+    say(f"intersect kind={kind} f={freq}")
+    s = P.scenegen(kind, 1, freq, 32, 32)
+    gpu_ctx.upload(s)
+    info = gpu_ctx.build_bvh()
+    say(f"bvh {info} tris={s.n_tris}")
+    assert info["nodes"] >= 1 and 3 * info["depth"] + 1 <= 128
+    rng = np.random.default_rng(kind * 100 + freq)
+    lo, hi = np.array(s.bbox_min) + 0.05, np.array(s.bbox_max) - 0.05
+    rays = random_rays(rng, 20000, lo, hi)
+    hits = gpu_ctx.intersect(rays)
+    os_ = O.OracleScene(s)
+    t_ref, tri_ref = os_.intersect(rays.view(np.float32).reshape(-1, 8), brute=True)
+    same_tri = hits["tri"] == tri_ref
+    # FP32 watertight vs FP64 Moller-Trumbore: ties at shared edges may pick
+    # either neighbour; hit/miss must agree and distances must match
+    assert np.mean(same_tri) > 0.999, np.mean(same_tri)
+    both = (hits["tri"] >= 0) & (tri_ref >= 0)
+    assert np.array_equal(hits["tri"] >= 0, tri_ref >= 0)
+    np.testing.assert_allclose(hits["t"][both], t_ref[both], rtol=2e-5, atol=2e-6)
+
+
+def test_watertight_sphere_never_leaks(gpu_ctx):
+    say("watertight icosphere")
+    s = P.scenegen(P.SCENE_BOX, 3, 64, 8, 8)
+    keep = s.bsdf_id == s.bsdf_id.max()  # the second sphere (centre 3.5, 1.0, 3.0)
+    sub = P.Scene(s.tri[keep], np.zeros(keep.sum(), np.int32), [s.bsdfs[int(s.bsdf_id.max())]], s.lights,
+                  s.camera, s.bbox_min, s.bbox_max)
+    gpu_ctx.upload(sub)
+    gpu_ctx.build_bvh()
+    rng = np.random.default_rng(9)
+    n = 200000
+    rays = np.zeros(n, P.RAY_DTYPE)
+    rays["origin"] = np.array([3.5, 1.0, 3.0]) + rng.uniform(-0.3, 0.3, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # include axis-parallel and edge-grazing directions
+    d[:6] = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]])
+    rays["dir"] = d
+    rays["tmax"] = 1e30
+    hits = gpu_ctx.intersect(rays)
+    assert np.all(hits["tri"] >= 0), int(np.sum(hits["tri"] < 0))
+
+
+def test_intersect_empty_and_tiny_scenes(gpu_ctx):
+    say("tiny scenes")
+    base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
+    for n in (1, 2, 3, 4, 5, 9):
+        sub = P.Scene(base.tri[:n], np.zeros(n, np.int32), [base.bsdfs[0]], base.lights, base.camera,
+                      base.bbox_min, base.bbox_max)
+        gpu_ctx.upload(sub)
+        gpu_ctx.build_bvh()
+        rng = np.random.default_rng(n)
+        rays = random_rays(rng, 4000, np.array([0.1] * 3), np.array([4.9] * 3))
+        hits = gpu_ctx.intersect(rays)
+        t_ref, tri_ref = O.OracleScene(sub).intersect(rays.view(np.float32).reshape(-1, 8), brute=True)
+        assert np.array_equal(hits["tri"] >= 0, tri_ref >= 0)
+
+
+# ---------------------------------------------------------------------------
+# Hash grid: bit-exact
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["uniform", "boundaries", "wrap", "negative", "duplicates", "single"])
+def test_hashgrid_bit_exact_golden(gpu_ctx, golden, name):
+    say(f"hashgrid golden {name}")
+    pos = golden[f"hg_{name}_pos"]
+    prm = golden[f"hg_{name}_params"]
+    S, scale, bmin = int(prm[0]), float(prm[1]), prm[2:5]
+    g = gpu_ctx.hashgrid(pos, S, scale, bmin)
+    o = O.hashgrid_points(pos, S, scale, bmin)
+    for a, b in zip(g, o):
+        assert np.array_equal(a, b)
+    lists, off = golden[f"hg_{name}_lists"], golden[f"hg_{name}_off"]
+    sidx, cs, ce = g
+    for i, p in enumerate(pos):
+        c = [abs(int((float(p[d]) - bmin[d]) * scale)) for d in range(3)]
+        h = O.orc().orc_hash(c[0], c[1], c[2], S)
+        assert np.array_equal(sidx[cs[h]:ce[h]], lists[off[i]:off[i + 1]])
+
+
+@pytest.mark.parametrize("n,S", [(0, 5), (1, 1), (1000, 1), (100000, 99991), (2_000_000, 1_999_993),
+                                 (3_000_000, 16_777_259)])
+def test_hashgrid_bit_exact_random(gpu_ctx, n, S):
+    say(f"hashgrid random n={n} S={S}")
+    rng = np.random.default_rng(n + S)
+    pos = rng.uniform(0.0, 5.0, (n, 3)).astype(np.float32)
+    scale = 1.0 / (2 * 0.0866 * 0.3)
+    bmin = np.array([-0.0866, -0.0866, -0.0866])
+    g = gpu_ctx.hashgrid(pos, S, scale, bmin)
+    o = O.hashgrid_points(pos, S, scale, bmin)
+    for a, b in zip(g, o):
+        assert np.array_equal(a, b)
+
+
+def test_hashgrid_boxes_bit_exact(gpu_ctx, golden):
+    say("hashgrid boxes")
+    bmin_i, bmax_i = golden["hgbox_bmin"], golden["hgbox_bmax"]
+    prm = golden["hgbox_params"]
+    S, scale, gmin = int(prm[0]), float(prm[1]), prm[2:5]
+    g = gpu_ctx.hashgrid_boxes(bmin_i, bmax_i, S, scale, gmin, 10 ** 6)
+    o = O.hashgrid_boxes(bmin_i, bmax_i, S, scale, gmin, 10 ** 6)
+    for a, b in zip(g, o):
+        assert np.array_equal(a, b)
+    with pytest.raises(P.PPPMError) as e:
+        gpu_ctx.hashgrid_boxes(bmin_i, bmax_i, S, scale, gmin, 10)
+    assert e.value.code == P.ERR_CAPACITY
+
+
+# ---------------------------------------------------------------------------
+# Pass pieces against the oracle (closed box, 1,280-triangle spheres)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c1_small():
+    s = P.scenegen(P.SCENE_BOX, 1, 0, 96, 96)
+    return s, O.OracleScene(s)
+
+
+def _render_one_pass(ctx, s, photons, r1, seed=0x7A75, passes=1, **kw):
+    ctx.upload(s)
+    ctx.build_bvh()
+    ctx.set_camera(s.camera)
+    ctx.begin(photons, passes, r1, ray_eps=EPS, seed=seed, **kw)
+    return [ctx.render_pass() for _ in range(passes)]
+
+
+def test_photon_pass_matches_oracle(gpu_ctx, c1_small):
+    s, os_ = c1_small
+    say("photon pass vs oracle")
+    N = 30000
+    st = _render_one_pass(gpu_ctx, s, N, 0.0866)[0]
+    gp, gf, gn, gk = gpu_ctx.last_photons()
+    assert st["photons_stored"] == len(gp)
+    op, of, on = os_.trace_photons(N, 1, EPS, 0x7A75)
+    ok = os_.last_keys
+    # keys are sorted (deterministic order) on both sides
+    assert np.all(np.diff(gk.astype(np.int64)) > 0)
+    common, gi, oi = np.intersect1d(gk, ok, return_indices=True)
+    # paths can diverge only where FP32 and FP64 disagree (grazing hits, RR ties)
+    assert len(common) > 0.995 * max(len(gk), len(ok)), (len(common), len(gk), len(ok))
+    dp = np.abs(gp[gi] - op[oi]).max(axis=1)
+    assert np.mean(dp < 1e-3) > 0.995
+    rel = np.abs(gf[gi] - of[oi]) / np.abs(of[oi])
+    assert np.mean(rel.max(axis=1) < 1e-4) > 0.995
+    # oriented normals agree in sign
+    assert np.mean(np.sum(gn[gi] * on[oi], axis=1) > 0.999) > 0.995
+    # total stored flux within 0.5 %
+    np.testing.assert_allclose(gf.sum(axis=0), of.sum(axis=0), rtol=5e-3)
+
+
+def test_hash_grid_of_a_pass_is_bit_exact(gpu_ctx, c1_small):
+    s, _ = c1_small
+    say("pass hash grid vs oracle")
+    st = _render_one_pass(gpu_ctx, s, 40000, 0.0866)[0]
+    gp, _, _, _ = gpu_ctx.last_photons()
+    S = st["hash_size"]
+    sidx, cs, ce, bmin, scale = gpu_ctx.last_grid(len(gp), S)
+    assert scale == 1.0 / (2.0 * 0.0866)
+    o = O.hashgrid_points(gp, S, scale, bmin)
+    assert np.array_equal(sidx, o[0]) and np.array_equal(cs, o[1]) and np.array_equal(ce, o[2])
+    if O.ref_available():
+        n = 3000  # the compiled reference HashGrid<int> on the first records
+        lists = np.zeros(n * 64, np.int32)
+        off = np.zeros(n + 1, np.int64)
+        pos = np.ascontiguousarray(gp[:n])
+        k = O.ref().ref_hashgrid_point_buckets(O._ptr(pos), n, S, scale, O._ptr(bmin), O._ptr(bmin + 1),
+                                               O._ptr(lists), len(lists), O._ptr(off))
+        assert k >= 0
+        sub = gpu_ctx.hashgrid(pos, S, scale, bmin)
+        for i in range(n):
+            c = [abs(int((float(pos[i, d]) - bmin[d]) * scale)) for d in range(3)]
+            h = O.orc().orc_hash(c[0], c[1], c[2], S)
+            assert np.array_equal(sub[0][sub[1][h]:sub[2][h]], lists[off[i]:off[i + 1]])
+
+
+def test_eye_pass_matches_oracle(gpu_ctx, c1_small):
+    s, os_ = c1_small
+    say("eye pass vs oracle")
+    _render_one_pass(gpu_ctx, s, 1000, 0.0866)
+    hp, hn, hw, pix = gpu_ctx.last_hitpoints()
+    has, op, on, ow, dr = os_.eye_pass(s.camera, 1, EPS, 0x7A75)
+    g_has = np.zeros_like(has)
+    g_has[pix] = 1
+    assert np.mean(g_has == has) > 0.999
+    sel = has[pix] == 1
+    p = pix[sel]
+    assert np.mean(np.abs(hp[sel] - op[p]).max(axis=1) < 1e-3) > 0.999
+    np.testing.assert_allclose(hw[sel], ow[p], rtol=1e-5, atol=1e-7)
+    assert np.mean(np.sum(hn[sel] * on[p], axis=1) > 0.9999) > 0.999
+
+
+def test_gather_matches_oracle(gpu_ctx, c1_small):
+    s, os_ = c1_small
+    say("gather vs oracle")
+    pos, flux, nrm = os_.trace_photons(60000, 1, EPS, 0x7A75)
+    has, hp, hn, hw, dr = os_.eye_pass(s.camera, 1, EPS, 0x7A75)
+    sel = has == 1
+    r = 0.0866
+    bmin = np.array(s.bbox_min) - r
+    S = len(pos)
+    g, st = gpu_ctx.gather(pos, flux, nrm, hp[sel], hn[sel], hw[sel], r, S, bmin)
+    o, cnt = O.gather(pos, flux, nrm, hp[sel], hn[sel], hw[sel], r, S, bmin)
+    assert st["gather_buckets"] == cnt[0] and st["gather_candidates"] == cnt[1]
+    # FP32 sums vs FP64; photons on the radius boundary and octahedral normals
+    # (16-bit) may flip a few terms of a thousand
+    scale = np.abs(o).max(axis=1) + 1e-30
+    err = np.abs(g - o).max(axis=1) / scale
+    assert np.mean(err < 1e-4) > 0.99, np.quantile(err, [0.5, 0.99, 1.0])
+    assert np.all(err < 2e-2)
+    assert abs(st["gather_inradius"] - cnt[2]) <= 1e-3 * cnt[2]
+
+
+# ---------------------------------------------------------------------------
+# Whole render against the oracle
+# ---------------------------------------------------------------------------
+def rel_rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.mean(b))
+
+
+def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small):
+    s, os_ = c1_small
+    say("render vs oracle (4 passes)")
+    N, passes, r1 = 50000, 4, 0.01 * s.diagonal()
+    _render_one_pass(gpu_ctx, s, N, r1, passes=passes)
+    g = gpu_ctx.image()
+    o, _ = os_.render(s.camera, N, passes, r1, ray_eps=EPS)
+    e = rel_rmse(g, o)
+    say(f"relative RMSE GPU vs oracle at equal photons: {e:.3e}")
+    # same random numbers on both sides: the images differ only by FP32 vs FP64
+    # arithmetic, so the difference is far below the Monte Carlo noise (~0.3)
+    assert e < 1e-2
+    assert np.all(np.isfinite(g)) and np.all(g >= 0)
+
+
+def test_render_is_bitwise_deterministic(gpu_ctx, c1_small):
+    s, _ = c1_small
+    say("determinism")
+    _render_one_pass(gpu_ctx, s, 30000, 0.0866, passes=2)
+    a = gpu_ctx.image()
+    _render_one_pass(gpu_ctx, s, 30000, 0.0866, passes=2)
+    b = gpu_ctx.image()
+    assert np.array_equal(a, b)
+
+
+def test_capacity_error_is_reported(gpu_ctx, c1_small):
+    s, _ = c1_small
+    say("capacity error")
+    with pytest.raises(P.PPPMError) as e:
+        _render_one_pass(gpu_ctx, s, 20000, 0.0866, capacity_factor=0.5)
+    assert e.value.code == P.ERR_CAPACITY
+    # the context stays usable
+    _render_one_pass(gpu_ctx, s, 20000, 0.0866)
+
+
+# ---------------------------------------------------------------------------
+# Full size (1024^2, 4M photons, ~0.7M triangles): size-independent properties
+# ---------------------------------------------------------------------------
+def test_full_size_pass_properties(gpu_ctx):
+    say("full size caustic box pass")
+    s = P.scenegen(P.SCENE_BOX_CAUSTIC, 1, 0, 1024, 1024)
+    r1 = 0.01 * s.diagonal()
+    st = _render_one_pass(gpu_ctx, s, 4_000_000, r1)[0]
+    say(f"stats {st}")
+    n = st["photons_stored"]
+    assert 1.2 * 4e6 < n < 8 * 4e6
+    assert st["hitpoints"] > 0.5 * 1024 * 1024
+    img = gpu_ctx.image()
+    assert np.all(np.isfinite(img)) and np.all(img >= 0)
+    # grid invariants on the real pass: ranges partition 0..n-1 in bucket order,
+    # and every record sits in the bucket its FP64 cell hashes to
+    gp, _, _, gk = gpu_ctx.last_photons()
+    assert np.all(np.diff(gk.astype(np.int64)) > 0)
+    S = st["hash_size"]
+    sidx, cs, ce, bmin, scale = gpu_ctx.last_grid(n, S)
+    assert cs[0] == 0 and ce[-1] == n and np.all(cs[1:] == ce[:-1])
+    assert np.array_equal(np.sort(sidx), np.arange(n))
+    c = np.abs(np.trunc((gp.astype(np.float64) - bmin) * scale)).astype(np.int64)
+    h = ((c[:, 0] * 73856093) ^ (c[:, 1] * 19349663) ^ (c[:, 2] * 83492791)) & 0xFFFFFFFF
+    h = h % S
+    bucket_of_pos = np.repeat(np.arange(S), ce - cs)
+    assert np.array_equal(h[sidx], bucket_of_pos)
+    # within each bucket the insertion order is kept (stable)
+    same = bucket_of_pos[1:] == bucket_of_pos[:-1]
+    assert np.all(sidx[1:][same] > sidx[:-1][same])
