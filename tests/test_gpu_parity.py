@@ -263,11 +263,12 @@ def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small):
     _render_one_pass(gpu_ctx, s, N, r1, passes=passes)
     g = gpu_ctx.image()
     o, _ = os_.render(s.camera, N, passes, r1, ray_eps=EPS)
-    e = rel_rmse(g, o)
-    say(f"relative RMSE GPU vs oracle at equal photons: {e:.3e}")
-    # same random numbers on both sides: the images differ only by FP32 vs FP64
-    # arithmetic, so the difference is far below the Monte Carlo noise (~0.3)
-    assert e < 1e-2
+    o2, _ = os_.render(s.camera, N, passes, r1, ray_eps=EPS, seed=0x7A76)
+    e, noise = rel_rmse(g, o), rel_rmse(o2, o)
+    say(f"relative RMSE GPU vs oracle at equal photons: {e:.3e}, seed-to-seed noise {noise:.3e}")
+    # same random numbers on both sides: the images differ only where FP32 and
+    # FP64 arithmetic take different paths, far inside the Monte Carlo noise
+    assert e < 0.05 * noise and e < 1e-2
     assert np.all(np.isfinite(g)) and np.all(g >= 0)
 
 
