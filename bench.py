@@ -91,6 +91,10 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--gather-mode", choices=["tiled", "lanes"], default="tiled")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
+                         "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
@@ -110,7 +114,8 @@ def main():
     bvh = ctx.build_bvh()
     ctx.set_camera(scene.camera)
     total = args.warmup + args.steps
-    ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed)
+    mode = P.GATHER_TILED if args.gather_mode == "tiled" else P.GATHER_LANES
+    ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed, gather_mode=mode)
     for _ in range(args.warmup):
         ctx.render_pass()
     stats = []
@@ -165,11 +170,17 @@ def main():
         "photon_kernel_photons_per_s": args.photons / (photon_kernel_ms * 1e-3),
         "gather_gbps": achieved,
         "stage_ms": {k: sum(s[k] for s in stats) / args.steps
-                     for k in ("photon_ms", "order_ms", "grid_ms", "eye_ms", "gather_ms", "total_ms")},
+                     for k in ("photon_ms", "order_ms", "grid_ms", "eye_ms", "hitsort_ms", "gather_ms",
+                               "total_ms")},
+        "gather_mode": args.gather_mode,
         "photons_stored_per_pass": sum(s["photons_stored"] for s in stats) / args.steps,
         "hitpoints_per_pass": sum(s["hitpoints"] for s in stats) / args.steps,
         "time_to_rmse": None,
     }
+    if args.copy_calibration > 0:
+        copy_ms = ctx.stream_copy(args.copy_calibration)
+        out["copy_calibration"] = {"bytes": args.copy_calibration, "ms": copy_ms,
+                                   "GBps": 2 * args.copy_calibration / (copy_ms * 1e-3) / 1e9}
     ctx.close()
     if not args.no_cpu:
         cb, _ = cpu_baseline(scene, args.photons, r1, ray_eps, seed)

@@ -106,7 +106,12 @@ typedef struct {
     uint32_t scene_id;     /* RNG stream id of this scene */
     int32_t hash_size;     /* 0: the number of stored photons of each pass */
     double capacity_factor;/* photon buffer = photons * factor records (0: 8) */
+    int32_t gather_mode;   /* PPPM_GATHER_TILED (default) or PPPM_GATHER_LANES */
+    int32_t reserved;
 } pppm_render_params;
+
+#define PPPM_GATHER_TILED 0 /* LDS-tiled: Morton-sorted hit points share bucket tiles */
+#define PPPM_GATHER_LANES 1 /* one lane per hit point over its own buckets */
 
 /* Timings and counters of one pass. */
 typedef struct {
@@ -122,6 +127,7 @@ typedef struct {
     int64_t gather_candidates; /* sum of photon records in them */
     int64_t gather_inradius;
     double photon_ms, order_ms, grid_ms, eye_ms, gather_ms, total_ms;
+    double hitsort_ms;         /* Morton sort of the hit points (tiled gather) */
 } pppm_pass_stats;
 
 /* ---- context ---- */
@@ -188,7 +194,13 @@ int pppm_last_hitpoints(void* ctx, float* pos, float* normal, float* weight, int
 int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const float* ph_nrm,
                 int64_t n_ph, const float* hit_pos, const float* hit_nrm, const float* hit_w,
                 int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
-                float* out_rgb, pppm_pass_stats* stats);
+                int32_t gather_mode, float* out_rgb, pppm_pass_stats* stats);
+
+/* ---- measurement ---- */
+/* Plain device-to-device copy of `bytes` (16-byte loads and stores, no
+ * reference counterpart): the known-traffic kernel that scales profiler
+ * FETCH_SIZE / WRITE_SIZE counters, and a measured copy bandwidth. */
+int pppm_stream_copy(void* ctx, int64_t bytes, double* ms);
 
 /* ---- procedural scenes (no reference counterpart: the reference loads PLY
  * meshes, sources/trimesh.cc:125-217) ---- */

@@ -25,8 +25,9 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["radix.hip", "hash_grid.hip", "bvh.hip", "render.hip", "capi.hip"]
-CXX_SOURCES = ["scenegen.cc"]
+HIP_SOURCES = ["radix.hip", "hash_grid.hip", "bvh.hip", "render.hip", "gather.hip", "capi.hip"]
+CXX_SOURCES = ["scenegen.cc", "pppm_amd.cc"]
+FACADE_TEST = os.path.join(ROOT, "tests", "cpp", "facade_main")
 
 
 def _run(cmd):
@@ -60,6 +61,9 @@ def build_lib(jobs=None, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the C++ interface driven like the reference's application (a test binary)
+    _run(["g++"] + CXX_FLAGS + [FACADE_TEST + ".cc", "-o", FACADE_TEST, "-L" + HERE, "-lpppm_hip",
+                                "-Wl,-rpath,$ORIGIN/../../tatsy-pppm_amd"])
     return LIB
 
 

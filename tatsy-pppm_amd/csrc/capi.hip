@@ -73,6 +73,8 @@ struct Ctx {
     double grid_bmin[3] = {0, 0, 0};
     DBuf ph_pos, ph_flux, ph_pos_s, ph_flux_s, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp,
         counters, scratch;
+    // hit points in Morton order of their half cells (tiled gather)
+    DBuf hp_pos_s, hp_w_s, hk[4], hhist;
     // last pass
     bool last_valid = false;
     int64_t last_n = 0;
@@ -167,6 +169,29 @@ hipError_t ensure_sort_buffers(Ctx* c, int64_t n, uint32_t S) {
     hipError_t e = c->hist.ensure(sizeof(uint32_t) * (size_t)radix_hist_words(n > 0 ? n : 1));
     if (e != hipSuccess) return e;
     return c->off.ensure(sizeof(uint32_t) * ((size_t)S + 1));
+}
+
+hipError_t ensure_hit_sort(Ctx* c, int64_t n) {
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    hipError_t e;
+    for (auto* b : {&c->hp_pos_s, &c->hp_w_s})
+        if ((e = b->ensure(sizeof(float4) * m)) != hipSuccess) return e;
+    for (auto& b : c->hk)
+        if ((e = b.ensure(sizeof(uint32_t) * m)) != hipSuccess) return e;
+    return c->hhist.ensure(sizeof(uint32_t) * (size_t)radix_hist_words((int64_t)m));
+}
+
+// The hit points among the cap slots of (hp, hw), compacted into hp_pos_s /
+// hp_w_s in Morton order of their half cells (edge r); a stable sort, so the
+// order is a function of the slot order alone.
+void sort_hitpoints(Ctx* c, const float4* hp, const float4* hw, int64_t cap, const GridParams& g,
+                    double radius) {
+    uint32_t* k[4];
+    for (int i = 0; i < 4; i++) k[i] = c->hk[i].as<uint32_t>();
+    launch_hit_keys(c->stream, hp, hw, cap, g, radius, k[0], k[1]);
+    bool alt = radix_sort_pairs(c->stream, k[0], k[1], k[2], k[3], c->hhist.as<uint32_t>(), cap, 32);
+    launch_permute2(c->stream, hp, hw, nullptr, alt ? k[3] : k[1], cap, c->hp_pos_s.as<float4>(),
+                    c->hp_w_s.as<float4>());
 }
 
 void copy_ranges(const std::vector<uint32_t>& off, int32_t S, int32_t* cs, int32_t* ce) {
@@ -475,7 +500,8 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     if (!ctx->has_bvh) return ctx->fail(PPPM_ERR_STATE, "no BVH built");
     if (!ctx->has_cam) return ctx->fail(PPPM_ERR_STATE, "no camera set");
     if (!p || p->photons <= 0 || p->passes <= 0 || !(p->alpha > 0.0 && p->alpha < 1.0) ||
-        !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0)
+        !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0 ||
+        (p->gather_mode != PPPM_GATHER_TILED && p->gather_mode != PPPM_GATHER_LANES))
         return ctx->fail(PPPM_ERR_INVALID, "bad render parameters");
     if (p->photons > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "at most 2^26 photons per pass");
     (void)hipSetDevice(ctx->device);
@@ -531,6 +557,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     TRY_HIP(ctx, ensure_sort_buffers(ctx, cap, Smax));
     TRY_HIP(ctx, ctx->hp_pos.ensure(sizeof(float4) * (size_t)npix));
     TRY_HIP(ctx, ctx->hp_w.ensure(sizeof(float4) * (size_t)npix));
+    TRY_HIP(ctx, ensure_hit_sort(ctx, npix));
     TRY_HIP(ctx, ctx->img_sum.ensure(sizeof(float) * 3 * (size_t)npix));
     TRY_HIP(ctx, ctx->img_comp.ensure(sizeof(float) * 3 * (size_t)npix));
     TRY_HIP(ctx, ctx->counters.ensure(sizeof(Counters)));
@@ -650,11 +677,14 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     // gather
     GatherArgs ga;
+    ga.mode = ctx->rp.gather_mode;
     ga.ph_pos = ctx->ph_pos_s.as<float4>();
     ga.ph_flux = ctx->ph_flux_s.as<float4>();
     ga.off = ctx->off.as<uint32_t>();
-    ga.hp_pos = ctx->hp_pos.as<float4>();
-    ga.hp_w = ctx->hp_w.as<float4>();
+    sort_hitpoints(ctx, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, radius);
+    ga.hp_pos = ctx->hp_pos_s.as<float4>();
+    ga.hp_w = ctx->hp_w_s.as<float4>();
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     ga.hp_count_src = dc;
     ga.n_hit = npix;
     ga.g = g;
@@ -665,7 +695,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ga.counters = dc;
     launch_gather(s, ga);
     TRY_HIP(ctx, hipGetLastError());
-    TRY_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[6], s));
     TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
     TRY_HIP(ctx, hipStreamSynchronize(s));
     if (hc.stack_overflow) return ctx->fail(PPPM_ERR_HIP, "traversal stack overflow");
@@ -677,10 +707,10 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ctx->last_perm1 = perm1;
     ctx->last_sorted = sorted;
     if (st) {
-        float t[5] = {0, 0, 0, 0, 0};
-        for (int k = 0; k < 5; k++) (void)hipEventElapsedTime(&t[k], ctx->ev[k], ctx->ev[k + 1]);
+        float t[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 6; k++) (void)hipEventElapsedTime(&t[k], ctx->ev[k], ctx->ev[k + 1]);
         float tot = 0.0f;
-        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]);
+        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[6]);
         st->hash_size = (int32_t)S;
         st->photon_segments = (int64_t)hc.photon_segments;
         st->eye_segments = (int64_t)hc.eye_segments;
@@ -692,7 +722,8 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         st->order_ms = t[1];
         st->grid_ms = t[2];
         st->eye_ms = t[3];
-        st->gather_ms = t[4];
+        st->hitsort_ms = t[4];
+        st->gather_ms = t[5];
         st->total_ms = tot;
     }
     return PPPM_OK;
@@ -773,8 +804,9 @@ int pppm_last_hitpoints(void* ctx_raw, float* pos, float* normal, float* weight,
     const int64_t m = nh < cap ? nh : cap;
     if (m <= 0) return PPPM_OK;
     std::vector<float4> a((size_t)m), b((size_t)m);
-    TRY_HIP(ctx, hipMemcpy(a.data(), ctx->hp_pos.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
-    TRY_HIP(ctx, hipMemcpy(b.data(), ctx->hp_w.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    // the compacted, Morton-ordered hit points the gather consumed
+    TRY_HIP(ctx, hipMemcpy(a.data(), ctx->hp_pos_s.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    TRY_HIP(ctx, hipMemcpy(b.data(), ctx->hp_w_s.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m; i++) {
         if (pos) { pos[3 * i] = a[i].x; pos[3 * i + 1] = a[i].y; pos[3 * i + 2] = a[i].z; }
         if (normal) {
@@ -791,8 +823,10 @@ int pppm_last_hitpoints(void* ctx_raw, float* pos, float* normal, float* weight,
 int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const float* ph_nrm,
                 int64_t n_ph, const float* hit_pos, const float* hit_nrm, const float* hit_w,
                 int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
-                float* out_rgb, pppm_pass_stats* st) {
+                int32_t gather_mode, float* out_rgb, pppm_pass_stats* st) {
     CHECK_CTX(ctx);
+    if (gather_mode != PPPM_GATHER_TILED && gather_mode != PPPM_GATHER_LANES)
+        return ctx->fail(PPPM_ERR_INVALID, "unknown gather mode");
     if (n_ph < 0 || n_hit < 0 || hash_size <= 0 || !(radius > 0.0) || !bbox_min ||
         (n_ph > 0 && (!ph_pos || !ph_flux || !ph_nrm)) ||
         (n_hit > 0 && (!hit_pos || !hit_nrm || !hit_w || !out_rgb)) || n_ph > 0x7FFFFFFFLL)
@@ -836,13 +870,21 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     g.size = (uint32_t)hash_size;
     uint32_t* sorted = build_grid(ctx, p4, nullptr, nullptr, n_ph, g, -1);
     launch_permute2(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s);
-    TRY_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     GatherArgs ga;
+    ga.mode = gather_mode;
     ga.ph_pos = p4s;
     ga.ph_flux = f4s;
     ga.off = ctx->off.as<uint32_t>();
-    ga.hp_pos = h4;
-    ga.hp_w = w4;
+    if (gather_mode == PPPM_GATHER_TILED && n_hit > 0) {
+        TRY_HIP(ctx, ensure_hit_sort(ctx, n_hit));
+        sort_hitpoints(ctx, h4, w4, n_hit, g, radius);
+        ga.hp_pos = ctx->hp_pos_s.as<float4>();
+        ga.hp_w = ctx->hp_w_s.as<float4>();
+    } else {
+        ga.hp_pos = h4;
+        ga.hp_w = w4;
+    }
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     ga.hp_count_src = nullptr;
     ga.n_hit = n_hit;
     ga.g = g;
@@ -872,6 +914,33 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
         st->gather_candidates = (int64_t)hc.gather_candidates;
         st->gather_inradius = (int64_t)hc.gather_inradius;
     }
+    return PPPM_OK;
+}
+
+int pppm_stream_copy(void* ctx_raw, int64_t bytes, double* ms) {
+    CHECK_CTX(ctx);
+    if (bytes < 16) return ctx->fail(PPPM_ERR_INVALID, "copy at least 16 bytes");
+    (void)hipSetDevice(ctx->device);
+    const int64_t n = bytes / 16;
+    void* a = nullptr;
+    void* b = nullptr;
+    TRY_HIP(ctx, hipMalloc(&a, (size_t)n * 16));
+    hipError_t e = hipMalloc(&b, (size_t)n * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return ctx->fail(PPPM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    launch_zero_u32(ctx->stream, a, 4 * n);
+    (void)hipEventRecord(ctx->ev[0], ctx->stream);
+    launch_stream_copy(ctx->stream, reinterpret_cast<const float4*>(a), reinterpret_cast<float4*>(b), n);
+    (void)hipEventRecord(ctx->ev[1], ctx->stream);
+    e = hipEventSynchronize(ctx->ev[1]);
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
+    if (ms) *ms = t;
     return PPPM_OK;
 }
 

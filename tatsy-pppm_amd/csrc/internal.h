@@ -94,6 +94,9 @@ struct PhotonPassArgs {
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 
+// pixel id marking a pixel slot without a hit point
+#define NO_HITPOINT 0xFFFFFFFFu
+
 struct EyePassArgs {
     SceneDev sc;
     pppm_camera cam;
@@ -111,7 +114,11 @@ struct EyePassArgs {
 };
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 
+#define GATHER_TILED 0
+#define GATHER_LANES 1
+
 struct GatherArgs {
+    int32_t mode;           // GATHER_TILED or GATHER_LANES
     const float4* ph_pos;   // cell order
     const float4* ph_flux;
     const uint32_t* off;    // [S + 1]
@@ -123,10 +130,14 @@ struct GatherArgs {
     double radius;
     float* img_sum;   // Kahan-accumulated image (null: write out_rgb)
     float* img_comp;
-    float* out_rgb;   // [n_hit][3] raw estimates (test hook)
+    float* out_rgb;   // [pixel id][3] raw estimates (test hook)
     Counters* counters;
 };
 void launch_gather(hipStream_t s, const GatherArgs& a);
+// Morton keys of the half cells of the hit points; slots whose pixel id is
+// NO_HITPOINT get key 0xFFFFFFFF and sort behind every hit point
+void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
+                     double radius, uint32_t* keys, uint32_t* vals);
 
 // origin keys of the stored photons, and the permutation applied to the records
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
@@ -136,5 +147,6 @@ void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, cons
 void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,
                     uint32_t* out);
 void launch_zero_u32(hipStream_t s, void* p, int64_t words);
+void launch_stream_copy(hipStream_t s, const float4* src, float4* dst, int64_t n);
 void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const float* n3,
                         int64_t n, float4* pos4, float4* b4, int normals);

@@ -232,3 +232,41 @@ __device__ inline bool tri_watertight(const RayPre& p, float4 v0, float4 v1, flo
     *v_out = W * rcp;
     return true;
 }
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Kahan-compensated FP32 accumulation of one value into sum[k] / comp[k]
+__device__ inline void kahan_add(float* sum, float* comp, int64_t k, float v) {
+    float s = sum[k], c = comp[k];
+    float y = v - c;
+    float t = s + y;
+    comp[k] = (t - s) - y;
+    sum[k] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Reference hash grid cell and bucket (sources/hash_grid_detail.h:35-43, 60-62):
+// |trunc((x - min) * scale)| in FP64, wrapping Teschner hash modulo the size
+// ---------------------------------------------------------------------------
+__device__ inline int32_t grid_cell(double x, double mn, double scale) {
+#pragma clang fp contract(off)
+    double b = (x - mn) * scale;
+    int32_t c = (int32_t)b;
+    return c < 0 ? -c : c;
+}
+
+__device__ inline uint32_t grid_hash(int32_t ix, int32_t iy, int32_t iz, uint32_t size) {
+    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    return h % size;
+}

@@ -11,10 +11,6 @@
 
 namespace {
 
-__device__ inline uint64_t lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
 
 // Append one record per active lane with one atomic per wave; returns the slot.
 __device__ inline unsigned long long wave_append(unsigned long long* counter) {
@@ -27,10 +23,6 @@ __device__ inline unsigned long long wave_append(unsigned long long* counter) {
     return base + (unsigned long long)__popcll(m & lanemask_lt());
 }
 
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
 
 // sampler.cc:17-32 (cosine-weighted direction about w)
 __device__ inline v3f on_hemisphere(v3f w, float r1, float r2) {
@@ -147,18 +139,11 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
     }
 }
 
-__device__ inline void kahan_add(float* sum, float* comp, int64_t k, float v) {
-    float s = sum[k], c = comp[k];
-    float y = v - c;
-    float t = s + y;
-    comp[k] = (t - s) - y;
-    sum[k] = t;
-}
 
 __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
-    unsigned long long segs = 0;
+    unsigned long long segs = 0, nhit = 0;
     bool ovf = false;
     const SceneDev& sc = a.sc;
     const int W = a.cam.width, H = a.cam.height;
@@ -178,6 +163,8 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         v3f o = C;
         v3f thr = mk(1.0f, 1.0f, 1.0f);
         v3f direct = mk(0.0f, 0.0f, 0.0f);
+        float4 hit_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
         for (int bounce = 0;; bounce++) {
             if (bounce >= a.max_bounces) break;
             Hit h = trace_closest<256>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf);
@@ -203,9 +190,9 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
             const v3f rho = mk(m.rho[0], m.rho[1], m.rho[2]);
             if (m.type & PPPM_BSDF_LAMBERTIAN) {
                 const v3f w = thr * rho * (PPPM_INV_PI / roulette);
-                unsigned long long slot = wave_append(&a.counters->hitpoints);
-                a.hp_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
-                a.hp_w[slot] = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
+                hit_pos = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                hit_w = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
+                nhit++;
                 break;
             }
             float pdf = 1.0f;
@@ -216,6 +203,10 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
             o = spawn(hp, ng, nd, a.ray_eps);
             d = nd;
         }
+        // the hit point of pixel p lives in slot p (pixel id all ones: none), so
+        // the hit point order entering the Morton sort is the pixel order
+        a.hp_pos[p] = hit_pos;
+        a.hp_w[p] = hit_w;
         if (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f) {
             kahan_add(a.img_sum, a.img_comp, 3 * p + 0, direct.x);
             kahan_add(a.img_sum, a.img_comp, 3 * p + 1, direct.y);
@@ -223,107 +214,12 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         }
     }
     segs = wave_sum(segs);
+    nhit = wave_sum(nhit);
     uint64_t any_ovf = __ballot(ovf);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&a.counters->eye_segments, segs);
+        atomicAdd(&a.counters->hitpoints, nhit);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
-    }
-}
-
-__device__ inline int32_t cell_of(double x, double mn, double scale) {
-#pragma clang fp contract(off)
-    double b = (x - mn) * scale;
-    int32_t c = (int32_t)b;
-    return c < 0 ? -c : c;
-}
-
-__device__ inline uint32_t hash3(int32_t ix, int32_t iy, int32_t iz, uint32_t size) {
-    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
-    return h % size;
-}
-
-#define CONE_K 1.1f
-
-// One hit point per lane; walks the de-duplicated buckets of the cells the
-// ball of radius r overlaps (cell edge 2r, so at most 2 per axis).
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    unsigned long long nb = 0, nc = 0, ni = 0;
-    if (i < n) {
-        const float4 hp = a.hp_pos[i];
-        const float4 hw = a.hp_w[i];
-        const v3f x = xyz(hp);
-        const v3f nx = oct_decode(__float_as_uint(hp.w));
-        const double r = a.radius;
-        int32_t lo[3], hi[3];
-        const double xs[3] = {hp.x, hp.y, hp.z};
-        for (int d = 0; d < 3; d++) {
-            lo[d] = cell_of(xs[d] - r, a.g.bmin[d], a.g.scale);
-            hi[d] = cell_of(xs[d] + r, a.g.bmin[d], a.g.scale);
-        }
-        uint32_t seen[8];
-        int ns = 0;
-        const float r2 = (float)(r * r);
-        const float inv_kr = (float)(1.0 / (1.1 * r));
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
-            for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
-                for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
-                    const uint32_t h = hash3(ix, iy, iz, a.g.size);
-                    bool dup = false;
-                    for (int k = 0; k < ns; k++) dup |= (seen[k] == h);
-                    if (dup || ns >= 8) continue;
-                    seen[ns++] = h;
-                    nb++;
-                    const uint32_t b0 = a.off[h], b1 = a.off[h + 1];
-                    nc += b1 - b0;
-                    for (uint32_t j = b0; j < b1; j++) {
-                        const float4 pp = a.ph_pos[j];
-                        const float dx = x.x - pp.x, dy = x.y - pp.y, dz = x.z - pp.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        if (d2 > r2) continue;
-                        // sign of n_p . n_x needs no normalisation of the decoded n_p
-                        const uint32_t e = __float_as_uint(pp.w);
-                        float qx = (float)(int16_t)(e & 0xFFFF), qy = (float)(int16_t)(e >> 16);
-                        qx *= (1.0f / 32767.0f);
-                        qy *= (1.0f / 32767.0f);
-                        float qz = 1.0f - fabsf(qx) - fabsf(qy);
-                        if (qz < 0.0f) {
-                            const float ox = (1.0f - fabsf(qy)) * sgnf(qx);
-                            const float oy = (1.0f - fabsf(qx)) * sgnf(qy);
-                            qx = ox;
-                            qy = oy;
-                        }
-                        if (qx * nx.x + qy * nx.y + qz * nx.z <= 0.0f) continue;
-                        ni++;
-                        const float4 f = a.ph_flux[j];
-                        const float wt = 1.0f - sqrtf(d2) * inv_kr;
-                        ax += wt * f.x;
-                        ay += wt * f.y;
-                        az += wt * f.z;
-                    }
-                }
-        const float inv_norm = (float)(1.0 / (3.14159265358979323846 * r * r * (1.0 - 2.0 / (3.0 * 1.1))));
-        const float Lx = hw.x * ax * inv_norm, Ly = hw.y * ay * inv_norm, Lz = hw.z * az * inv_norm;
-        if (a.img_sum) {
-            const int64_t pix = (int64_t)__float_as_uint(hw.w);
-            kahan_add(a.img_sum, a.img_comp, 3 * pix + 0, Lx);
-            kahan_add(a.img_sum, a.img_comp, 3 * pix + 1, Ly);
-            kahan_add(a.img_sum, a.img_comp, 3 * pix + 2, Lz);
-        } else {
-            a.out_rgb[3 * i + 0] = Lx;
-            a.out_rgb[3 * i + 1] = Ly;
-            a.out_rgb[3 * i + 2] = Lz;
-        }
-    }
-    nb = wave_sum(nb);
-    nc = wave_sum(nc);
-    ni = wave_sum(ni);
-    if ((threadIdx.x & 63) == 0 && a.counters) {
-        atomicAdd(&a.counters->gather_buckets, nb);
-        atomicAdd(&a.counters->gather_candidates, nc);
-        atomicAdd(&a.counters->gather_inradius, ni);
     }
 }
 
@@ -387,12 +283,6 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
     hipLaunchKernelGGL(k_eye, dim3(g), dim3(256), 0, s, a);
 }
 
-void launch_gather(hipStream_t s, const GatherArgs& a) {
-    // with a device-side count the grid covers the worst case (one per pixel)
-    if (a.n_hit <= 0) return;
-    hipLaunchKernelGGL(k_gather, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
-}
-
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
                         uint32_t* vals) {
     if (n <= 0) return;
@@ -424,4 +314,19 @@ void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack, dim3(grid1(n, 256)), dim3(256), 0, s, pos3, a3, n3, n, pos4, b4,
                        normals);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+}  // namespace
+
+void launch_stream_copy(hipStream_t s, const float4* src, float4* dst, int64_t n) {
+    if (n <= 0) return;
+    int64_t g = grid1(n, 256);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(256), 0, s, src, dst, n);
 }

@@ -1,0 +1,409 @@
+// pppm_amd.h -- C++ interface of the MI355X photon mapper with the class
+// names, signatures and call pattern of tatsy/tatsy-pppm (paths relative to
+// the reference root), implemented over the C ABI of include/pppm.h.
+//
+//   Scene scene;                                    // sources/scene.h:26-67
+//   scene.add(Triangle(a, b, c), LambertianBRDF::factory(rho));
+//   scene.addAreaLight(corner, edgeU, edgeV, Le);   // new: the configs' emitter
+//   scene.setAccelerator();                         // GPU 4-wide BVH build
+//   PerspectiveCamera camera(center, dir, up, fovDeg, w, h, 1.0);
+//   ProgressivePhotonMappingProb ppm;               // sources/ppm_probabilistic.h:23-42
+//   ppm.render(scene, camera, RenderParameters(photons, passes, 0, 0.0));
+//   const Image& img = ppm.result();
+//
+// Everything lives in namespace pppm_amd; `using namespace pppm_amd;` makes a
+// reference call site compile unchanged.  Errors throw pppm_amd::Error (the
+// reference aborts through Assertion, sources/common.h:24-35).
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pppm.h"
+
+namespace pppm_amd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---- sources/vector3d.h:21-98 (FP64 vector) --------------------------------
+class Vector3D {
+    double _x, _y, _z;
+
+public:
+    Vector3D() : _x(0.0), _y(0.0), _z(0.0) {}
+    Vector3D(double x, double y, double z) : _x(x), _y(y), _z(z) {}
+    double x() const { return _x; }
+    double y() const { return _y; }
+    double z() const { return _z; }
+    double operator[](int d) const { return d == 0 ? _x : (d == 1 ? _y : _z); }
+    static double dot(const Vector3D& u, const Vector3D& v);
+    static Vector3D cross(const Vector3D& u, const Vector3D& v);
+    static Vector3D minimum(const Vector3D& u, const Vector3D& v);
+    static Vector3D maximum(const Vector3D& u, const Vector3D& v);
+    double norm() const;
+    double squaredNorm() const { return dot(*this, *this); }
+    Vector3D normalized() const;
+    Vector3D operator-() const { return Vector3D(-_x, -_y, -_z); }
+    Vector3D& operator+=(const Vector3D& v);
+    Vector3D& operator-=(const Vector3D& v);
+    Vector3D& operator*=(double s);
+};
+Vector3D operator+(const Vector3D& u, const Vector3D& v);
+Vector3D operator-(const Vector3D& u, const Vector3D& v);
+Vector3D operator*(const Vector3D& u, const Vector3D& v);
+Vector3D operator*(const Vector3D& v, double s);
+Vector3D operator*(double s, const Vector3D& v);
+Vector3D operator/(const Vector3D& v, double s);
+
+// ---- sources/ray.h:16-79 ---------------------------------------------------
+class Ray {
+    Vector3D _origin, _direction;
+
+public:
+    Ray() {}
+    Ray(const Vector3D& origin, const Vector3D& direction) : _origin(origin), _direction(direction) {}
+    Vector3D origin() const { return _origin; }
+    Vector3D direction() const { return _direction; }
+};
+
+class Hitpoint {
+    double _distance = 1.0e20;  // INFTY, sources/ray.cc:48-52
+    Vector3D _normal, _position;
+
+public:
+    double distance() const { return _distance; }
+    Vector3D normal() const { return _normal; }
+    Vector3D position() const { return _position; }
+    void setDistance(double d) { _distance = d; }
+    void setNormal(const Vector3D& n) { _normal = n; }
+    void setPosition(const Vector3D& p) { _position = p; }
+};
+
+class Intersection {
+    Hitpoint _hitPoint;
+    int _objectId = -1;
+
+public:
+    int objectID() const { return _objectId; }
+    const Hitpoint& hitpoint() const { return _hitPoint; }
+    double hittingDistance() const { return _hitPoint.distance(); }
+    void setHitpoint(const Hitpoint& h) { _hitPoint = h; }
+    void setObjectId(int id) { _objectId = id; }
+};
+
+class Triangle;
+
+// ---- sources/bbox.h:21-56 --------------------------------------------------
+class BBox {
+    Vector3D _posMin, _posMax;
+
+public:
+    BBox() : _posMin(1e20, 1e20, 1e20), _posMax(-1e20, -1e20, -1e20) {}
+    BBox(const Vector3D& mn, const Vector3D& mx) : _posMin(mn), _posMax(mx) {}
+    const Vector3D& posMin() const { return _posMin; }
+    const Vector3D& posMax() const { return _posMax; }
+    void merge(const Vector3D& v);
+    void merge(const BBox& b);
+    void merge(const Triangle& t);
+};
+
+// ---- sources/triangle.h:16-45 and geometry_interface.h ----------------------
+class IGeometry {
+public:
+    virtual ~IGeometry() {}
+    virtual std::vector<Triangle> triangulate() const = 0;
+};
+
+class Triangle : public IGeometry {
+    Vector3D _p0, _p1, _p2;
+
+public:
+    Triangle() {}
+    Triangle(const Vector3D& p0, const Vector3D& p1, const Vector3D& p2) : _p0(p0), _p1(p1), _p2(p2) {}
+    Vector3D p(int id) const { return id == 0 ? _p0 : (id == 1 ? _p1 : _p2); }
+    Vector3D p0() const { return _p0; }
+    Vector3D p1() const { return _p1; }
+    Vector3D p2() const { return _p2; }
+    Vector3D gravity() const { return (_p0 + _p1 + _p2) / 3.0; }
+    Vector3D normal() const { return Vector3D::cross(_p1 - _p0, _p2 - _p0).normalized(); }
+    double area() const { return 0.5 * Vector3D::cross(_p1 - _p0, _p2 - _p1).norm(); }
+    std::vector<Triangle> triangulate() const override { return std::vector<Triangle>(1, *this); }
+};
+
+// ---- sources/bsdf.h:19-59, sources/brdf.h:37-101 ---------------------------
+enum BsdfType {
+    BSDF_TYPE_NONE = 0x00,
+    BSDF_TYPE_LAMBERTIAN_BRDF = PPPM_BSDF_LAMBERTIAN,
+    BSDF_TYPE_SPECULAR_BRDF = PPPM_BSDF_SPECULAR,
+    BSDF_TYPE_PHONG_BRDF = PPPM_BSDF_PHONG,
+    BSDF_TYPE_REFRACTION = PPPM_BSDF_REFRACTION,
+    BSDF_TYPE_BSSRDF = PPPM_BSDF_BSSRDF,
+    BSDF_TYPE_EMITTER = PPPM_BSDF_EMITTER
+};
+
+// Value type holding the material parameters the GPU table needs.
+class BSDF {
+    pppm_bsdf _b;
+
+public:
+    BSDF();
+    explicit BSDF(const pppm_bsdf& b) : _b(b) {}
+    Vector3D reflectance() const;
+    BsdfType type() const { return (BsdfType)_b.type; }
+    const pppm_bsdf& raw() const { return _b; }
+};
+
+struct LambertianBRDF {
+    static BSDF factory(const Vector3D& reflectance);
+};
+struct SpecularBRDF {
+    static BSDF factory(const Vector3D& reflectance);
+};
+struct RefractionBSDF {
+    static BSDF factory(const Vector3D& reflectance);
+};
+
+// ---- sources/perspective_camera.h:24-31 ------------------------------------
+// fov is the vertical field of view in DEGREES (the unit the reference's own
+// caller passes, sources/main/appmain.cc:123; its implementation reads
+// radians, sources/perspective_camera.cc:18).
+class PerspectiveCamera {
+    pppm_camera _c;
+    double _sensitivity;
+
+public:
+    PerspectiveCamera(const Vector3D& center, const Vector3D& direction, const Vector3D& up,
+                      double fov, int imageWidth, int imageHeight, double sensitivity);
+    const pppm_camera& raw() const { return _c; }
+    int width() const { return _c.width; }
+    int height() const { return _c.height; }
+    double sensitivity() const { return _sensitivity; }
+};
+typedef PerspectiveCamera Camera;
+
+// ---- sources/render_parameters.h:4-44 --------------------------------------
+class RenderParameters {
+    int _photons, _spp, _gatherPhotons;
+    double _gatherRadius;
+
+public:
+    explicit RenderParameters(int photons = 1000000, int spp = 16, int gatherPhotons = 16,
+                              double gatherRadius = 16.0)
+        : _photons(photons), _spp(spp), _gatherPhotons(gatherPhotons), _gatherRadius(gatherRadius) {}
+    int photons() const { return _photons; }
+    int spp() const { return _spp; }
+    int gatherPhotons() const { return _gatherPhotons; }
+    double gatherRadius() const { return _gatherRadius; }
+};
+
+// Knobs the reference hard-codes (ALPHA, ppm_probabilistic.cc:9; the initial
+// radius, :55) and the extensions of the GPU path.
+struct PPMSettings {
+    double alpha = 0.7;
+    double radiusFraction = 0.01;  // r_1 = fraction * scene bbox diagonal
+    double rayEps = 1.0e-4;
+    uint64_t seed = 0x7A75;
+    uint32_t sceneId = 0;
+    int bounceLimit = 64;
+    int hashSize = 0;
+    int gatherMode = PPPM_GATHER_TILED;
+};
+
+// ---- sources/image.h (FP64 RGB buffer) --------------------------------------
+class Image {
+    int _w = 0, _h = 0;
+    std::vector<double> _rgb;
+
+public:
+    Image() {}
+    Image(int w, int h) : _w(w), _h(h), _rgb(3 * (size_t)w * h, 0.0) {}
+    int width() const { return _w; }
+    int height() const { return _h; }
+    Vector3D operator()(int x, int y) const;
+    std::vector<double>& data() { return _rgb; }
+    const std::vector<double>& data() const { return _rgb; }
+};
+
+// A device context shared by the objects that use the GPU.
+class Device {
+    void* _ctx = nullptr;
+
+public:
+    explicit Device(int device = 0);
+    ~Device();
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    void* handle() const { return _ctx; }
+    void check(int rc) const;
+    static std::shared_ptr<Device> shared(int device = 0);
+};
+
+// ---- sources/qbvh_accel.h:13-62 --------------------------------------------
+class QBVHAccel {
+    std::shared_ptr<Device> _dev;
+    std::vector<Triangle> _tris;
+    bool _built = false;
+
+public:
+    QBVHAccel();
+    explicit QBVHAccel(std::shared_ptr<Device> dev);
+    // GPU build over the triangle soup (replaces the serial object-median build)
+    void construct(const std::vector<Triangle>& triangles);
+    // triangle id or -1; hitpoint->distance() is the ray's tMax on input
+    int intersect(const Ray& ray, Hitpoint* hitpoint) const;
+    // batch form: the production path
+    void intersect(const std::vector<Ray>& rays, std::vector<Intersection>* out) const;
+};
+
+// ---- sources/scene.h:26-67 -------------------------------------------------
+class Scene {
+    std::shared_ptr<Device> _dev;
+    std::vector<Triangle> _triangles;
+    std::vector<int> _bsdfIds;
+    std::vector<BSDF> _bsdfs;
+    std::vector<pppm_light> _lights;
+    bool _accel = false;
+
+public:
+    Scene();
+    explicit Scene(std::shared_ptr<Device> dev);
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+
+    template <class Ty>
+    void add(const Ty& primitive, const BSDF& bsdf) {
+        static_assert(std::is_base_of<IGeometry, Ty>::value, "Ty must be an IGeometry");
+        std::vector<Triangle> t = primitive.triangulate();
+        _triangles.insert(_triangles.end(), t.begin(), t.end());
+        _bsdfIds.insert(_bsdfIds.end(), t.size(), (int)_bsdfs.size());
+        _bsdfs.push_back(bsdf);
+    }
+    // parallelogram emitter (two triangles + a photon source)
+    void addAreaLight(const Vector3D& corner, const Vector3D& edgeU, const Vector3D& edgeV,
+                      const Vector3D& emission);
+    // load a procedural benchmark scene (include/pppm.h PPPM_SCENE_*)
+    PerspectiveCamera loadProcedural(int kind, uint64_t seed, int frequency, int width, int height);
+
+    const Triangle& getTriangle(int id) const { return _triangles.at(id); }
+    const BSDF& getBsdf(int triId) const { return _bsdfs.at(_bsdfIds.at(triId)); }
+    size_t numTriangles() const { return _triangles.size(); }
+    void clear();
+    void setAccelerator();
+    bool intersect(const Ray& ray, Intersection& isect) const;
+    BBox bounds() const;
+    std::shared_ptr<Device> device() const { return _dev; }
+    void upload() const;
+};
+
+// ---- sources/hash_grid.h:8-39 ----------------------------------------------
+// Same API.  add() records (item, box); the buckets are built on the GPU by a
+// stable radix sort (construct(), or lazily on the first operator[]), with
+// the bucket contents of the reference: every item of every cell its box
+// covers, in insertion order.
+template <class Ty>
+class HashGrid {
+    int _hashSize = -1;
+    BBox _bbox;
+    double _hashScale = 0.0;
+    std::vector<Ty> _items;
+    std::vector<double> _bmin, _bmax;
+    std::vector<int32_t> _sorted, _start, _end;
+    mutable std::vector<std::vector<Ty>> _data;
+    bool _dirty = true;
+    std::shared_ptr<Device> _dev;
+
+public:
+    HashGrid() : _dev(Device::shared()) {}
+    void init(int hashSize, double hashScale, const BBox& bbox) {
+        _hashSize = hashSize;
+        _hashScale = hashScale;
+        _bbox = bbox;
+        clear();
+    }
+    void add(const Ty& p, const Vector3D& boxMin, const Vector3D& boxMax) {
+        _items.push_back(p);
+        _bmin.insert(_bmin.end(), {boxMin.x(), boxMin.y(), boxMin.z()});
+        _bmax.insert(_bmax.end(), {boxMax.x(), boxMax.y(), boxMax.z()});
+        _dirty = true;
+    }
+    void clear() {
+        _items.clear();
+        _bmin.clear();
+        _bmax.clear();
+        _data.clear();
+        _dirty = true;
+    }
+    // builds the buckets on the GPU (the reference's construct is an empty stub)
+    void construct(std::vector<Ty>& points = dummy(), const int imageW = -1, const int imageH = -1) {
+        (void)points; (void)imageW; (void)imageH;
+        if (_hashSize <= 0) throw Error(PPPM_ERR_STATE, "hash size is not initialized");
+        const double bm[3] = {_bbox.posMin().x(), _bbox.posMin().y(), _bbox.posMin().z()};
+        int64_t n = (int64_t)_items.size(), npairs = 0;
+        _start.assign(_hashSize, 0);
+        _end.assign(_hashSize, 0);
+        int64_t cap = 8 * n + 16;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            _sorted.assign((size_t)cap, 0);
+            int rc = pppm_hashgrid_build_boxes(_dev->handle(), _bmin.data(), _bmax.data(), n, _hashSize,
+                                               _hashScale, bm, _sorted.data(), cap, &npairs,
+                                               _start.data(), _end.data());
+            if (rc == PPPM_ERR_CAPACITY) { cap = npairs; continue; }
+            _dev->check(rc);
+            break;
+        }
+        _sorted.resize((size_t)npairs);
+        _data.assign(_hashSize, std::vector<Ty>());
+        for (int h = 0; h < _hashSize; h++)
+            for (int32_t j = _start[h]; j < _end[h]; j++) _data[h].push_back(_items[_sorted[j]]);
+        _dirty = false;
+    }
+    std::vector<Ty>& operator[](const Vector3D& v) {
+        if (_dirty) construct();
+        const double b[3] = {(v.x() - _bbox.posMin().x()) * _hashScale, (v.y() - _bbox.posMin().y()) * _hashScale,
+                             (v.z() - _bbox.posMin().z()) * _hashScale};
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            int i = static_cast<int>(b[d]);
+            c[d] = i < 0 ? -i : i;
+        }
+        uint32_t h = ((uint32_t)c[0] * 73856093u) ^ ((uint32_t)c[1] * 19349663u) ^ ((uint32_t)c[2] * 83492791u);
+        return _data[h % (uint32_t)_hashSize];
+    }
+
+private:
+    static std::vector<Ty>& dummy() {
+        static std::vector<Ty> d;
+        return d;
+    }
+};
+
+// ---- sources/ppm_probabilistic.h:23-42 -------------------------------------
+class ProgressivePhotonMappingProb {
+    Image _result;
+    PPMSettings _settings;
+    std::vector<pppm_pass_stats> _stats;
+    std::function<void(int, const Image&)> _onPass;
+
+public:
+    static const double ALPHA;
+    ProgressivePhotonMappingProb() {}
+    explicit ProgressivePhotonMappingProb(const PPMSettings& s) : _settings(s) {}
+    // Runs params.spp() passes of params.photons() photons each on the GPU.
+    void render(const Scene& scene, const Camera& camera, const RenderParameters& params);
+    const Image& result() const { return _result; }
+    const std::vector<pppm_pass_stats>& passStats() const { return _stats; }
+    // called after every pass with the running average (the reference saves a
+    // PNG here, sources/ppm_probabilistic.cc:97-106)
+    void setPassCallback(std::function<void(int, const Image&)> f) { _onPass = std::move(f); }
+    PPMSettings& settings() { return _settings; }
+};
+
+}  // namespace pppm_amd

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpppm_hip.so")
 OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
+GATHER_TILED, GATHER_LANES = 0, 1
 
 
 class Bsdf(C.Structure):
@@ -42,7 +43,7 @@ class RenderParams(C.Structure):
     _fields_ = [("photons", C.c_int64), ("passes", C.c_int32), ("max_bounces", C.c_int32),
                 ("alpha", C.c_double), ("radius1", C.c_double), ("ray_eps", C.c_double),
                 ("seed", C.c_uint64), ("scene_id", C.c_uint32), ("hash_size", C.c_int32),
-                ("capacity_factor", C.c_double)]
+                ("capacity_factor", C.c_double), ("gather_mode", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -52,7 +53,8 @@ class PassStats(C.Structure):
                 ("hitpoints", C.c_int64), ("gather_buckets", C.c_int64),
                 ("gather_candidates", C.c_int64), ("gather_inradius", C.c_int64),
                 ("photon_ms", C.c_double), ("order_ms", C.c_double), ("grid_ms", C.c_double),
-                ("eye_ms", C.c_double), ("gather_ms", C.c_double), ("total_ms", C.c_double)]
+                ("eye_ms", C.c_double), ("gather_ms", C.c_double), ("total_ms", C.c_double),
+                ("hitsort_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -90,7 +92,8 @@ _DECLS = {
     "pppm_last_grid": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, C.POINTER(C.c_double)]),
     "pppm_last_hitpoints": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
-                              C.c_int32, _P, _P, C.POINTER(PassStats)]),
+                              C.c_int32, _P, C.c_int32, _P, C.POINTER(PassStats)]),
+    "pppm_stream_copy": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
     "pppm_scenegen": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SceneDesc)]),
     "pppm_scenegen_free": (None, [C.POINTER(SceneDesc)]),
 }
@@ -219,6 +222,11 @@ class Context:
         self._chk(self.L.pppm_device_info(self.h, name, 256, C.byref(cu), C.byref(hbm)))
         return {"name": name.value.decode(), "cu": cu.value, "hbm_bytes": hbm.value}
 
+    def stream_copy(self, nbytes):
+        ms = C.c_double()
+        self._chk(self.L.pppm_stream_copy(self.h, nbytes, C.byref(ms)))
+        return ms.value
+
     def upload(self, scene):
         tri = np.ascontiguousarray(scene.tri, np.float32)
         ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
@@ -265,10 +273,10 @@ class Context:
         self.camera = cam
 
     def begin(self, photons, passes, radius1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
-              hash_size=0, max_bounces=64, capacity_factor=8.0):
+              hash_size=0, max_bounces=64, capacity_factor=8.0, gather_mode=GATHER_TILED):
         p = RenderParams(photons=photons, passes=passes, max_bounces=max_bounces, alpha=alpha,
                          radius1=radius1, ray_eps=ray_eps, seed=seed, scene_id=scene_id,
-                         hash_size=hash_size, capacity_factor=capacity_factor)
+                         hash_size=hash_size, capacity_factor=capacity_factor, gather_mode=gather_mode)
         self._chk(self.L.pppm_render_begin(self.h, C.byref(p)))
         self.params = p
 
@@ -317,7 +325,8 @@ class Context:
                                              C.byref(n)))
         return pos[:m], nrm[:m], w[:m], pix[:m]
 
-    def gather(self, ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w, radius, hash_size, bbox_min):
+    def gather(self, ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w, radius, hash_size, bbox_min,
+               mode=0):
         a = [np.ascontiguousarray(x, np.float32).reshape(-1, 3)
              for x in (ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w)]
         out = np.zeros((max(len(a[3]), 1), 3), np.float32)
@@ -325,5 +334,5 @@ class Context:
         bm = np.ascontiguousarray(bbox_min, np.float64)
         self._chk(self.L.pppm_gather(self.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), len(a[0]), _ptr(a[3]),
                                      _ptr(a[4]), _ptr(a[5]), len(a[3]), radius, hash_size, _ptr(bm),
-                                     _ptr(out), C.byref(st)))
+                                     mode, _ptr(out), C.byref(st)))
         return out[:len(a[3])], st.as_dict()

@@ -228,16 +228,17 @@ def test_eye_pass_matches_oracle(gpu_ctx, c1_small):
     assert np.mean(np.sum(hn[sel] * on[p], axis=1) > 0.9999) > 0.999
 
 
-def test_gather_matches_oracle(gpu_ctx, c1_small):
+@pytest.mark.parametrize("mode", [P.GATHER_TILED, P.GATHER_LANES])
+def test_gather_matches_oracle(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
-    say("gather vs oracle")
+    say(f"gather vs oracle mode={mode}")
     pos, flux, nrm = os_.trace_photons(60000, 1, EPS, 0x7A75)
     has, hp, hn, hw, dr = os_.eye_pass(s.camera, 1, EPS, 0x7A75)
     sel = has == 1
     r = 0.0866
     bmin = np.array(s.bbox_min) - r
     S = len(pos)
-    g, st = gpu_ctx.gather(pos, flux, nrm, hp[sel], hn[sel], hw[sel], r, S, bmin)
+    g, st = gpu_ctx.gather(pos, flux, nrm, hp[sel], hn[sel], hw[sel], r, S, bmin, mode=mode)
     o, cnt = O.gather(pos, flux, nrm, hp[sel], hn[sel], hw[sel], r, S, bmin)
     assert st["gather_buckets"] == cnt[0] and st["gather_candidates"] == cnt[1]
     # FP32 sums vs FP64; photons on the radius boundary and octahedral normals
@@ -256,11 +257,12 @@ def rel_rmse(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.mean(b))
 
 
-def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small):
+@pytest.mark.parametrize("mode", [P.GATHER_TILED, P.GATHER_LANES])
+def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
-    say("render vs oracle (4 passes)")
+    say(f"render vs oracle (4 passes) gather mode={mode}")
     N, passes, r1 = 50000, 4, 0.01 * s.diagonal()
-    _render_one_pass(gpu_ctx, s, N, r1, passes=passes)
+    _render_one_pass(gpu_ctx, s, N, r1, passes=passes, gather_mode=mode)
     g = gpu_ctx.image()
     o, _ = os_.render(s.camera, N, passes, r1, ray_eps=EPS)
     o2, _ = os_.render(s.camera, N, passes, r1, ray_eps=EPS, seed=0x7A76)
@@ -299,8 +301,16 @@ def test_full_size_pass_properties(gpu_ctx):
     say("full size caustic box pass")
     s = P.scenegen(P.SCENE_BOX_CAUSTIC, 1, 0, 1024, 1024)
     r1 = 0.01 * s.diagonal()
+    st_l = _render_one_pass(gpu_ctx, s, 4_000_000, r1, gather_mode=P.GATHER_LANES)[0]
+    img_lanes = gpu_ctx.image()
     st = _render_one_pass(gpu_ctx, s, 4_000_000, r1)[0]
     say(f"stats {st}")
+    # the two gather kernels sum the same photon set in different orders
+    img_tiled = gpu_ctx.image()
+    assert st_l["gather_inradius"] == st["gather_inradius"]
+    assert st_l["gather_candidates"] == st["gather_candidates"]
+    den = np.maximum(np.abs(img_lanes), 1e-6 * img_lanes.max())
+    assert np.quantile(np.abs(img_tiled - img_lanes) / den, 0.999) < 1e-4
     n = st["photons_stored"]
     assert 1.2 * 4e6 < n < 8 * 4e6
     assert st["hitpoints"] > 0.5 * 1024 * 1024
