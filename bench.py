@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--gather-mode", choices=["tiled", "lanes"], default="tiled")
+    ap.add_argument("--gather-mode", choices=["auto", "lanes", "tiled", "wave"], default="auto")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -114,7 +114,8 @@ def main():
     bvh = ctx.build_bvh()
     ctx.set_camera(scene.camera)
     total = args.warmup + args.steps
-    mode = P.GATHER_TILED if args.gather_mode == "tiled" else P.GATHER_LANES
+    mode = {"auto": P.GATHER_AUTO, "lanes": P.GATHER_LANES, "tiled": P.GATHER_TILED,
+            "wave": P.GATHER_WAVE}[args.gather_mode]
     ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed, gather_mode=mode)
     for _ in range(args.warmup):
         ctx.render_pass()

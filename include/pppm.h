@@ -110,8 +110,10 @@ typedef struct {
     int32_t reserved;
 } pppm_render_params;
 
-#define PPPM_GATHER_TILED 0 /* LDS-tiled: Morton-sorted hit points share bucket tiles */
-#define PPPM_GATHER_LANES 1 /* one lane per hit point over its own buckets */
+#define PPPM_GATHER_AUTO 0  /* the kernel measured fastest on gfx950 (currently LANES) */
+#define PPPM_GATHER_LANES 1 /* one lane per Morton-sorted hit point over its own buckets */
+#define PPPM_GATHER_TILED 2 /* LDS-tiled: a wave's union of buckets staged in 64-photon tiles */
+#define PPPM_GATHER_WAVE 3  /* wave-uniform bucket walk, photons through scalar loads */
 
 /* Timings and counters of one pass. */
 typedef struct {

@@ -809,8 +809,12 @@ static void gather_one(const float* ph_pos, const float* ph_flux, const float* p
     int32_t lo[3], hi[3];
     double xs[3] = {x.x, x.y, x.z};
     for (int d = 0; d < 3; d++) {
+        /* [x - r, x + r] has the width of one cell (scale = 1 / 2r): it meets at
+         * most two cells per axis; a third only appears through FP64 rounding at
+         * an exact cell boundary and is dropped (as on the GPU) */
         lo[d] = cell_coord(xs[d] - r, bmin[d], scale);
         hi[d] = cell_coord(xs[d] + r, bmin[d], scale);
+        if (hi[d] > lo[d] + 1) hi[d] = lo[d] + 1;
     }
     uint32_t seen[8];
     int ns = 0;

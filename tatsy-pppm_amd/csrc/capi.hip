@@ -71,7 +71,9 @@ struct Ctx {
     std::vector<double> radii;
     int64_t cap = 0;
     double grid_bmin[3] = {0, 0, 0};
-    DBuf ph_pos, ph_flux, ph_pos_s, ph_flux_s, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp,
+    // ph_pos / ph_flux: photons as the photon pass appended them; ph_a/b/c: the
+    // gather layout in bucket order
+    DBuf ph_pos, ph_flux, ph_a, ph_b, ph_c, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp,
         counters, scratch;
     // hit points in Morton order of their half cells (tiled gather)
     DBuf hp_pos_s, hp_w_s, hk[4], hhist;
@@ -244,7 +246,7 @@ int pppm_destroy(void* ctx_raw) {
     (void)hipStreamSynchronize(c->stream);
     c->tri9.release(); c->tri_mat.release(); c->mats.release(); c->lights.release();
     (void)hipFree(c->nodes); (void)hipFree(c->tris4);
-    c->ph_pos.release(); c->ph_flux.release(); c->ph_pos_s.release(); c->ph_flux_s.release();
+    c->ph_pos.release(); c->ph_flux.release(); c->ph_a.release(); c->ph_b.release(); c->ph_c.release();
     for (auto& b : c->u) b.release();
     c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
     c->img_sum.release(); c->img_comp.release(); c->counters.release(); c->scratch.release();
@@ -501,11 +503,12 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     if (!ctx->has_cam) return ctx->fail(PPPM_ERR_STATE, "no camera set");
     if (!p || p->photons <= 0 || p->passes <= 0 || !(p->alpha > 0.0 && p->alpha < 1.0) ||
         !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0 ||
-        (p->gather_mode != PPPM_GATHER_TILED && p->gather_mode != PPPM_GATHER_LANES))
+        (p->gather_mode < PPPM_GATHER_AUTO || p->gather_mode > PPPM_GATHER_WAVE))
         return ctx->fail(PPPM_ERR_INVALID, "bad render parameters");
     if (p->photons > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "at most 2^26 photons per pass");
     (void)hipSetDevice(ctx->device);
     ctx->rp = *p;
+    if (ctx->rp.gather_mode == PPPM_GATHER_AUTO) ctx->rp.gather_mode = GATHER_AUTO_CHOICE;
     ctx->radii.assign((size_t)p->passes, 0.0);
     pppm_radius_schedule(p->radius1, p->alpha, p->passes, 0, ctx->radii.data());
     double factor = p->capacity_factor > 0.0 ? p->capacity_factor : 8.0;
@@ -551,8 +554,9 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     const int64_t cap = ctx->cap;
     TRY_HIP(ctx, ctx->ph_pos.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_flux.ensure(sizeof(float4) * (size_t)cap));
-    TRY_HIP(ctx, ctx->ph_pos_s.ensure(sizeof(float4) * (size_t)cap));
-    TRY_HIP(ctx, ctx->ph_flux_s.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_a.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_b.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_c.ensure(sizeof(float4) * (size_t)cap));
     uint32_t Smax = p->hash_size > 0 ? (uint32_t)p->hash_size : (uint32_t)cap;
     TRY_HIP(ctx, ensure_sort_buffers(ctx, cap, Smax));
     TRY_HIP(ctx, ctx->hp_pos.ensure(sizeof(float4) * (size_t)npix));
@@ -634,8 +638,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     g.size = S;
     TRY_HIP(ctx, ctx->off.ensure(sizeof(uint32_t) * ((size_t)S + 1)));
     uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
-    launch_permute2(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
-                    ctx->ph_pos_s.as<float4>(), ctx->ph_flux_s.as<float4>());
+    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
+                         ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
+                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     // eye pass
@@ -678,8 +683,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     // gather
     GatherArgs ga;
     ga.mode = ctx->rp.gather_mode;
-    ga.ph_pos = ctx->ph_pos_s.as<float4>();
-    ga.ph_flux = ctx->ph_flux_s.as<float4>();
+    ga.ph_a = ctx->ph_a.as<float4>();
+    ga.ph_b = ctx->ph_b.as<float4>();
+    ga.ph_c = ctx->ph_c.as<float4>();
     ga.off = ctx->off.as<uint32_t>();
     sort_hitpoints(ctx, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, radius);
     ga.hp_pos = ctx->hp_pos_s.as<float4>();
@@ -825,8 +831,9 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
                 int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
                 int32_t gather_mode, float* out_rgb, pppm_pass_stats* st) {
     CHECK_CTX(ctx);
-    if (gather_mode != PPPM_GATHER_TILED && gather_mode != PPPM_GATHER_LANES)
+    if (gather_mode < PPPM_GATHER_AUTO || gather_mode > PPPM_GATHER_WAVE)
         return ctx->fail(PPPM_ERR_INVALID, "unknown gather mode");
+    if (gather_mode == PPPM_GATHER_AUTO) gather_mode = GATHER_AUTO_CHOICE;
     if (n_ph < 0 || n_hit < 0 || hash_size <= 0 || !(radius > 0.0) || !bbox_min ||
         (n_ph > 0 && (!ph_pos || !ph_flux || !ph_nrm)) ||
         (n_hit > 0 && (!hit_pos || !hit_nrm || !hit_w || !out_rgb)) || n_ph > 0x7FFFFFFFLL)
@@ -835,7 +842,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     hipStream_t s = ctx->stream;
     const int64_t np = n_ph > 0 ? n_ph : 1, nh = n_hit > 0 ? n_hit : 1;
     // staging: photons [3 x 3 floats] + hit points [3 x 3 floats] + outputs
-    size_t bytes = sizeof(float) * 9 * (size_t)(np + nh) + sizeof(float4) * 4 * (size_t)np +
+    size_t bytes = sizeof(float) * 9 * (size_t)(np + nh) + sizeof(float4) * 5 * (size_t)np +
                    sizeof(float4) * 2 * (size_t)nh + sizeof(float) * 3 * (size_t)nh;
     TRY_HIP(ctx, ctx->scratch.ensure(bytes));
     float* f = ctx->scratch.as<float>();
@@ -845,7 +852,8 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     float4* f4 = p4 + np;
     float4* p4s = f4 + np;
     float4* f4s = p4s + np;
-    float4* h4 = f4s + np;
+    float4* c4s = f4s + np;
+    float4* h4 = c4s + np;
     float4* w4 = h4 + nh;
     float* dout = reinterpret_cast<float*>(w4 + nh);
     if (n_ph > 0) {
@@ -869,13 +877,14 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     g.scale = 1.0 / (2.0 * radius);
     g.size = (uint32_t)hash_size;
     uint32_t* sorted = build_grid(ctx, p4, nullptr, nullptr, n_ph, g, -1);
-    launch_permute2(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s);
+    launch_gather_layout(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s, gather_mode == GATHER_LANES ? nullptr : c4s);
     GatherArgs ga;
     ga.mode = gather_mode;
-    ga.ph_pos = p4s;
-    ga.ph_flux = f4s;
+    ga.ph_a = p4s;
+    ga.ph_b = f4s;
+    ga.ph_c = c4s;
     ga.off = ctx->off.as<uint32_t>();
-    if (gather_mode == PPPM_GATHER_TILED && n_hit > 0) {
+    if (n_hit > 0) {  // every kernel reads the hit points in Morton order, as in a pass
         TRY_HIP(ctx, ensure_hit_sort(ctx, n_hit));
         sort_hitpoints(ctx, h4, w4, n_hit, g, radius);
         ga.hp_pos = ctx->hp_pos_s.as<float4>();

@@ -6,38 +6,48 @@
 // |x - p|^2 <= r^2 and n_p . n_x > 0 contributes (1 - |x - p| / (1.1 r)) Phi_p;
 // L = W * sum / (pi r^2 (1 - 2 / 3.3)).
 //
-// Two kernels compute the same sum:
-//   k_gather_tiled (default): hit points sorted along a Morton curve of half
-//     cells; each 64-lane workgroup forms the union of its lanes' buckets with
-//     ballots, streams the union's photons through LDS in coalesced chunks of
-//     64, and every lane tests its hit point against every staged photon.
+// Photons are read in bucket (cell) order from a gather layout of three
+// 16-byte words: (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g) (Phi_b, -, -, -), the
+// normal decoded once when the layout is written.  Hit points arrive sorted
+// along a Morton curve of half cells (edge r), so neighbouring lanes share
+// their eight buckets.  Three kernels compute the same sum:
+//   k_gather_wave (default): each wave walks the union of its 64 lanes'
+//     buckets one bucket at a time with a wave-uniform photon index, so photon
+//     records arrive through scalar loads and are tested by all 64 lanes.
+//   k_gather_tiled: the union's photons are staged through LDS in coalesced
+//     64-photon chunks and tested by all 64 lanes.
 //   k_gather_lanes: one lane per hit point walking its own buckets.
 #include "internal.h"
 
 namespace {
 
-#define CONE_K 1.1f
-
-struct HitCtx {
-    v3f x, nx, w;
-    int64_t pix;
-    int32_t lo[3], hi[3];
-};
-
-__device__ inline void hit_cells(const GatherArgs& a, float4 hp, int32_t lo[3], int32_t hi[3]) {
+// Cells of edge 2r met by [x - r, x + r] per axis: at most two (a third can
+// only come from FP64 rounding at an exact cell boundary, and is dropped).
+__device__ __forceinline__ void hit_cells(const GatherArgs& a, float4 hp, int32_t lo[3], int32_t hi[3]) {
     const double r = a.radius;
     const double xs[3] = {hp.x, hp.y, hp.z};
     for (int d = 0; d < 3; d++) {
         lo[d] = grid_cell(xs[d] - r, a.g.bmin[d], a.g.scale);
-        hi[d] = grid_cell(xs[d] + r, a.g.bmin[d], a.g.scale);
+        hi[d] = min(grid_cell(xs[d] + r, a.g.bmin[d], a.g.scale), lo[d] + 1);
     }
 }
 
-// sign of n_p . n_x needs no normalisation of the decoded octahedral n_p
-__device__ inline v3f oct_direction(uint32_t e) {
+// photon records through the constant address space: loads with a
+// wave-uniform index become scalar loads
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4v* const_f4;
+__device__ __forceinline__ const_f4 as_const(const float4* p) { return (const_f4)p; }
+__device__ __forceinline__ float4 ld(const_f4 p, uint32_t j) {
+    const f4v v = p[j];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// direction of an octahedral normal code, unnormalised (only its sign
+// against n_x is needed)
+__device__ __forceinline__ v3f oct_direction(uint32_t e) {
     float qx = (float)(int16_t)(e & 0xFFFF) * (1.0f / 32767.0f);
     float qy = (float)(int16_t)(e >> 16) * (1.0f / 32767.0f);
-    float qz = 1.0f - fabsf(qx) - fabsf(qy);
+    const float qz = 1.0f - fabsf(qx) - fabsf(qy);
     if (qz < 0.0f) {
         const float ox = (1.0f - fabsf(qy)) * sgnf(qx);
         const float oy = (1.0f - fabsf(qx)) * sgnf(qy);
@@ -47,12 +57,28 @@ __device__ inline v3f oct_direction(uint32_t e) {
     return mk(qx, qy, qz);
 }
 
-__device__ inline float cone_norm(double r) {
+// Smallest bucket id among the slots of mb set in m (all ones if none), and
+// the slot set with that bucket cleared: value operations only, so the eight
+// slots stay in registers.
+__device__ __forceinline__ uint32_t min_bucket(const uint32_t mb[8], uint32_t m) {
+    uint32_t v = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((m >> k) & 1u) v = min(v, mb[k]);
+    return v;
+}
+__device__ __forceinline__ uint32_t clear_bucket(const uint32_t mb[8], uint32_t m, uint32_t b) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (mb[k] == b) m &= ~(1u << k);
+    return m;
+}
+
+__device__ __forceinline__ float cone_norm(double r) {
     return (float)(1.0 / (3.14159265358979323846 * r * r * (1.0 - 2.0 / (3.0 * 1.1))));
 }
 
-__device__ inline void write_result(const GatherArgs& a, int64_t i, int64_t pix, v3f w, float ax, float ay,
-                                    float az) {
+__device__ __forceinline__ void write_result(const GatherArgs& a, int64_t pix, v3f w, float ax, float ay, float az) {
     const float inv_norm = cone_norm(a.radius);
     const float Lx = w.x * ax * inv_norm, Ly = w.y * ay * inv_norm, Lz = w.z * az * inv_norm;
     if (a.img_sum) {
@@ -60,67 +86,61 @@ __device__ inline void write_result(const GatherArgs& a, int64_t i, int64_t pix,
         kahan_add(a.img_sum, a.img_comp, 3 * pix + 1, Ly);
         kahan_add(a.img_sum, a.img_comp, 3 * pix + 2, Lz);
     } else {
-        (void)i;
         a.out_rgb[3 * pix + 0] = Lx;
         a.out_rgb[3 * pix + 1] = Ly;
         a.out_rgb[3 * pix + 2] = Lz;
     }
 }
 
-// the de-duplicated buckets of one hit point (at most 2 cells per axis: cell edge 2r)
-__device__ inline int own_buckets(const GatherArgs& a, float4 hp, uint32_t mb[8]) {
+// The buckets of the (up to 2 x 2 x 2) cells of one hit point in the
+// reference's z, y, x loop order; bit c of the returned mask marks slot c as
+// a cell of the range whose bucket no earlier slot holds (hash collisions and
+// abs() folding can map two cells to one bucket).
+__device__ __forceinline__ uint32_t own_buckets(const GatherArgs& a, float4 hp, uint32_t mb[8]) {
     int32_t lo[3], hi[3];
     hit_cells(a, hp, lo, hi);
-    int ns = 0;
-    for (int32_t iz = lo[2]; iz <= hi[2]; iz++)
-        for (int32_t iy = lo[1]; iy <= hi[1]; iy++)
-            for (int32_t ix = lo[0]; ix <= hi[0]; ix++) {
-                const uint32_t h = grid_hash(ix, iy, iz, a.g.size);
-                bool dup = false;
-                for (int k = 0; k < ns; k++) dup |= (mb[k] == h);
-                if (!dup && ns < 8) mb[ns++] = h;
-            }
-    return ns;
+    uint32_t vm = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const bool in = lo[0] + dx <= hi[0] && lo[1] + dy <= hi[1] && lo[2] + dz <= hi[2];
+        mb[c] = grid_hash(lo[0] + dx, lo[1] + dy, lo[2] + dz, a.g.size);
+        bool dup = false;
+#pragma unroll
+        for (int k = 0; k < c; k++) dup |= ((vm >> k) & 1u) && mb[k] == mb[c];
+        if (in && !dup) vm |= 1u << c;
+    }
+    return vm;
 }
 
-__global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
-    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    unsigned long long nb = 0, nc = 0, ni = 0;
-    if (i < n) {
-        const float4 hp = a.hp_pos[i];
-        const float4 hw = a.hp_w[i];
-        const v3f x = xyz(hp);
-        const v3f nx = oct_decode(__float_as_uint(hp.w));
-        uint32_t mb[8];
-        const int ns = own_buckets(a, hp, mb);
-        const float r2 = (float)(a.radius * a.radius);
-        const float inv_kr = (float)(1.0 / (1.1 * a.radius));
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for (int k = 0; k < ns; k++) {
-            const uint32_t h = mb[k];
-            const uint32_t b0 = a.off[h], b1 = a.off[h + 1];
-            nb++;
-            nc += b1 - b0;
-            for (uint32_t j = b0; j < b1; j++) {
-                const float4 pp = a.ph_pos[j];
-                const float dx = x.x - pp.x, dy = x.y - pp.y, dz = x.z - pp.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 > r2) continue;
-                if (dot(oct_direction(__float_as_uint(pp.w)), nx) <= 0.0f) continue;
-                ni++;
-                const float4 f = a.ph_flux[j];
-                const float wt = 1.0f - sqrtf(d2) * inv_kr;
-                ax += wt * f.x;
-                ay += wt * f.y;
-                az += wt * f.z;
-            }
-        }
-        write_result(a, i, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
+struct Acc {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    unsigned long long n = 0;
+};
+
+// |x - p|^2 with explicit fused operations: every kernel decides the radius
+// test of a photon on the same bits
+__device__ __forceinline__ float dist2(v3f x, float4 A) {
+    const float dx = x.x - A.x, dy = x.y - A.y, dz = x.z - A.z;
+    return __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, __fmul_rn(dz, dz)));
+}
+
+// one candidate inside the radius: normal agreement, cone weight
+__device__ __forceinline__ void accumulate(Acc& acc, v3f nx, float d2, float inv_kr, float4 A, float4 B, float4 C) {
+    if (A.w * nx.x + B.x * nx.y + B.y * nx.z > 0.0f) {
+        const float wt = 1.0f - sqrtf(d2) * inv_kr;
+        acc.x += wt * B.z;
+        acc.y += wt * B.w;
+        acc.z += wt * C.x;
+        acc.n++;
     }
-    nb = wave_sum(nb);
-    nc = wave_sum(nc);
-    ni = wave_sum(ni);
+}
+
+__device__ __forceinline__ void flush_counters(const GatherArgs& a, bool valid, unsigned long long nb,
+                                      unsigned long long nc, unsigned long long ni) {
+    nb = wave_sum(valid ? nb : 0);
+    nc = wave_sum(valid ? nc : 0);
+    ni = wave_sum(valid ? ni : 0);
     if ((threadIdx.x & 63) == 0 && a.counters) {
         atomicAdd(&a.counters->gather_buckets, nb);
         atomicAdd(&a.counters->gather_candidates, nc);
@@ -128,57 +148,156 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
     }
 }
 
-// One wave (64 lanes) per workgroup, 64 Morton-adjacent hit points.
+__global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
+    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    unsigned long long nb = 0, nc = 0;
+    Acc acc;
+    if (valid) {
+        const float4 hp = a.hp_pos[i];
+        const float4 hw = a.hp_w[i];
+        const v3f x = xyz(hp);
+        const v3f nx = oct_decode(__float_as_uint(hp.w));
+        uint32_t mb[8];
+        uint32_t todo = own_buckets(a, hp, mb);
+        const float r2 = (float)(a.radius * a.radius);
+        const float inv_kr = (float)(1.0 / (1.1 * a.radius));
+        // slots in the reference's z, y, x cell order: lanes of neighbouring
+        // half cells then walk their shared buckets in step
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            if (!((todo >> c) & 1u)) continue;
+            const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
+            nb++;
+            nc += b1 - b0;
+            // compact layout: (x, y, z, octahedral normal) (Phi_r, Phi_g, Phi_b, key)
+            for (uint32_t j = b0; j < b1; j++) {
+                const float4 A = a.ph_a[j];
+                const float d2 = dist2(x, A);
+                if (d2 > r2) continue;
+                const v3f np = oct_direction(__float_as_uint(A.w));
+                const float4 F = a.ph_b[j];
+                if (np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f) {
+                    const float wt = 1.0f - sqrtf(d2) * inv_kr;
+                    acc.x += wt * F.x;
+                    acc.y += wt * F.y;
+                    acc.z += wt * F.z;
+                    acc.n++;
+                }
+            }
+        }
+        write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), acc.x, acc.y, acc.z);
+    }
+    flush_counters(a, valid, nb, nc, acc.n);
+}
+
+// Per-lane setup shared by the wave-cooperative kernels.
+struct LaneHit {
+    bool valid;
+    float4 hp, hw;
+    v3f x, nx;
+    uint32_t mb[8];
+    uint32_t vm;  // valid, de-duplicated slots of mb
+    unsigned long long nb, nc;
+};
+
+__device__ __forceinline__ void lane_hit(const GatherArgs& a, int64_t n, int64_t i, LaneHit& h) {
+    h.valid = i < n;
+    h.hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    h.hw = h.hp;
+    if (h.valid) {
+        h.hp = a.hp_pos[i];
+        h.hw = a.hp_w[i];
+    }
+    h.x = xyz(h.hp);
+    h.nx = h.valid ? oct_decode(__float_as_uint(h.hp.w)) : mk(0.0f, 0.0f, 0.0f);
+    const uint32_t vm = own_buckets(a, h.hp, h.mb);  // every lane: no divergent array writes
+    h.vm = h.valid ? vm : 0u;
+    h.nb = (unsigned long long)__popc(h.vm);
+    h.nc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((h.vm >> k) & 1u) h.nc += a.off[h.mb[k] + 1] - a.off[h.mb[k]];
+}
+
+// Next bucket of the wave's union (wave-uniform), removing it from every
+// lane's pending set; false when the union is exhausted.
+// The union is walked in ascending bucket id: the wave minimum of the lanes'
+// smallest pending buckets.
+__device__ __forceinline__ bool next_union_bucket(const LaneHit& h, uint32_t& pending, uint32_t& bucket) {
+    if (__ballot(pending != 0u) == 0) return false;
+    uint32_t v = min_bucket(h.mb, pending);
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
+    const uint32_t b = __builtin_amdgcn_readfirstlane(v);
+    pending = clear_bucket(h.mb, pending, b);
+    bucket = b;
+    return true;
+}
+
+// Four independent waves per workgroup, 64 Morton-adjacent hit points each.
+__global__ __launch_bounds__(256) void k_gather_wave(GatherArgs a) {
+    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (((int64_t)blockIdx.x * 256 + (threadIdx.x & ~63)) >= n) return;  // idle wave (uniform)
+    LaneHit h;
+    lane_hit(a, n, i, h);
+    uint32_t pending = h.vm;
+    const float r2 = (float)(a.radius * a.radius);
+    const float inv_kr = (float)(1.0 / (1.1 * a.radius));
+    const const_f4 pa = as_const(a.ph_a), pb = as_const(a.ph_b), pc = as_const(a.ph_c);
+    const __attribute__((address_space(4))) uint32_t* off =
+        (const __attribute__((address_space(4))) uint32_t*)a.off;
+    Acc acc;
+    uint32_t b;
+    while (next_union_bucket(h, pending, b)) {
+        const uint32_t j0 = off[b], j1 = off[b + 1];
+        uint32_t j = j0;
+        // four positions in flight per wait; the bucket's last record repeats
+        // past its end and is masked out
+        for (; j < j1; j += 4) {
+            const uint32_t last = j1 - 1;
+            float4 A[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) A[q] = ld(pa, min(j + q, last));
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float d2 = dist2(h.x, A[q]);
+                if (j + q < j1 && d2 <= r2) accumulate(acc, h.nx, d2, inv_kr, A[q], ld(pb, j + q), ld(pc, j + q));
+            }
+        }
+    }
+    if (h.valid) write_result(a, (int64_t)__float_as_uint(h.hw.w), xyz(h.hw), acc.x, acc.y, acc.z);
+    flush_counters(a, h.valid, h.nb, h.nc, acc.n);
+}
+
+// One wave (64 lanes) per workgroup; the union's photons staged through LDS.
 __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
-    __shared__ float4 s_pos[64];
-    __shared__ float4 s_nrm[64];
-    __shared__ float4 s_flux[64];
+    __shared__ float4 s_a[64];
+    __shared__ float4 s_b[64];
+    __shared__ float4 s_c[64];
     __shared__ uint32_t s_start[64];
     __shared__ uint32_t s_pre[65];
     const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
     const int lane = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * 64 + lane;
     if ((int64_t)blockIdx.x * 64 >= n) return;  // whole workgroup idle (uniform)
-    const bool valid = i < n;
-    float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), hw = hp;
-    if (valid) {
-        hp = a.hp_pos[i];
-        hw = a.hp_w[i];
-    }
-    const v3f x = xyz(hp);
-    const v3f nx = valid ? oct_decode(__float_as_uint(hp.w)) : mk(0.0f, 0.0f, 0.0f);
-    uint32_t mb[8];
-    const int ns = valid ? own_buckets(a, hp, mb) : 0;
-    unsigned long long nb = (unsigned long long)ns, nc = 0, ni = 0;
-    for (int k = 0; k < 8; k++)
-        if (k < ns) nc += a.off[mb[k] + 1] - a.off[mb[k]];
-    uint32_t pending = (1u << ns) - 1u;
+    LaneHit h;
+    lane_hit(a, n, i, h);
+    uint32_t pending = h.vm;
     const float r2 = (float)(a.radius * a.radius);
     const float inv_kr = (float)(1.0 / (1.1 * a.radius));
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    Acc acc;
     if (lane == 0) s_pre[0] = 0;
     while (true) {
-        // ---- collect up to 64 buckets of the wave's union, one per lane ----
-        uint32_t my_bucket = 0;
+        // up to 64 buckets of the wave's union, one per lane
+        uint32_t my_bucket = 0, b;
         int nu = 0;
-        while (nu < 64) {
-            const uint64_t any = __ballot(pending != 0u);
-            if (any == 0) break;
-            const int leader = __ffsll((unsigned long long)any) - 1;
-            uint32_t first = 0;
-            {
-                const int k0 = pending ? __ffs(pending) - 1 : 0;
-                for (int k = 0; k < 8; k++)
-                    if (k == k0) first = mb[k];
-            }
-            const uint32_t b = __shfl(first, leader);
-            for (int k = 0; k < 8; k++)
-                if (k < ns && mb[k] == b) pending &= ~(1u << k);
+        while (nu < 64 && next_union_bucket(h, pending, b)) {
             if (lane == nu) my_bucket = b;
             nu++;
         }
         if (nu == 0) break;
-        // ---- bucket ranges and their prefix over the lanes ----
         uint32_t start = 0, size = 0;
         if (lane < nu) {
             start = a.off[my_bucket];
@@ -194,7 +313,6 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
         s_start[lane] = start;
         s_pre[lane + 1] = incl;
         __syncthreads();
-        // ---- stream the union's photons through LDS ----
         int u0 = 0;
         for (uint32_t base = 0; base < total; base += 64) {
             const uint32_t g = base + (uint32_t)lane;
@@ -202,48 +320,27 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
                 int u = u0;
                 while (g >= s_pre[u + 1]) u++;
                 const uint32_t j = s_start[u] + (g - s_pre[u]);
-                const float4 pp = a.ph_pos[j];
-                const float4 ff = a.ph_flux[j];
-                const v3f d = oct_direction(__float_as_uint(pp.w));
-                s_pos[lane] = make_float4(pp.x, pp.y, pp.z, 0.0f);
-                s_nrm[lane] = make_float4(d.x, d.y, d.z, 0.0f);
-                s_flux[lane] = ff;
+                s_a[lane] = a.ph_a[j];
+                s_b[lane] = a.ph_b[j];
+                s_c[lane] = a.ph_c[j];
             }
             __syncthreads();
             const int m = (total - base) < 64u ? (int)(total - base) : 64;
             for (int k = 0; k < m; k++) {
-                const float4 pp = s_pos[k];
-                const float dx = x.x - pp.x, dy = x.y - pp.y, dz = x.z - pp.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 <= r2) {
-                    const float4 pn = s_nrm[k];
-                    if (pn.x * nx.x + pn.y * nx.y + pn.z * nx.z > 0.0f) {
-                        const float4 f = s_flux[k];
-                        const float wt = 1.0f - sqrtf(d2) * inv_kr;
-                        ax += wt * f.x;
-                        ay += wt * f.y;
-                        az += wt * f.z;
-                        ni++;
-                    }
-                }
+                const float4 A = s_a[k];
+                const float d2 = dist2(h.x, A);
+                if (d2 <= r2) accumulate(acc, h.nx, d2, inv_kr, A, s_b[k], s_c[k]);
             }
             __syncthreads();
             const uint32_t next = base + 64;
             while (u0 < nu - 1 && next >= s_pre[u0 + 1]) u0++;
         }
     }
-    if (valid) write_result(a, i, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
-    nb = wave_sum(valid ? nb : 0);
-    nc = wave_sum(valid ? nc : 0);
-    ni = wave_sum(valid ? ni : 0);
-    if (lane == 0 && a.counters) {
-        atomicAdd(&a.counters->gather_buckets, nb);
-        atomicAdd(&a.counters->gather_candidates, nc);
-        atomicAdd(&a.counters->gather_inradius, ni);
-    }
+    if (h.valid) write_result(a, (int64_t)__float_as_uint(h.hw.w), xyz(h.hw), acc.x, acc.y, acc.z);
+    flush_counters(a, h.valid, h.nb, h.nc, acc.n);
 }
 
-__device__ inline uint32_t spread10(uint32_t v) {
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
     v = (v * 0x00000101u) & 0x0F00F00Fu;
     v = (v * 0x00000011u) & 0xC30C30C3u;
@@ -272,6 +369,28 @@ __global__ void k_hit_keys(const float4* __restrict__ hp_pos, const float4* __re
     vals[i] = (uint32_t)i;
 }
 
+// gather layout of the photons in bucket order: record k = photon perm[sorted[k]]
+__global__ void k_gather_layout(const float4* __restrict__ pos, const float4* __restrict__ flux,
+                                const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sorted,
+                                int64_t n, float4* __restrict__ A, float4* __restrict__ B,
+                                float4* __restrict__ Cc) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = sorted ? sorted[k] : (uint32_t)k;
+    const uint32_t src = perm ? perm[j] : j;
+    const float4 p = pos[src];
+    const float4 f = flux[src];
+    if (!Cc) {  // compact: the records as stored
+        A[k] = p;
+        B[k] = f;
+        return;
+    }
+    const v3f nrm = oct_decode(__float_as_uint(p.w));
+    A[k] = make_float4(p.x, p.y, p.z, nrm.x);
+    B[k] = make_float4(nrm.y, nrm.z, f.x, f.y);
+    Cc[k] = make_float4(f.z, f.w, 0.0f, 0.0f);
+}
+
 }  // namespace
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -281,8 +400,10 @@ void launch_gather(hipStream_t s, const GatherArgs& a) {
     if (a.n_hit <= 0) return;
     if (a.mode == GATHER_LANES)
         hipLaunchKernelGGL(k_gather_lanes, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
-    else
+    else if (a.mode == GATHER_TILED)
         hipLaunchKernelGGL(k_gather_tiled, dim3(grid1(a.n_hit, 64)), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_gather_wave, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
 }
 
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
@@ -290,4 +411,11 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
     if (cap <= 0) return;
     hipLaunchKernelGGL(k_hit_keys, dim3(grid1(cap, 256)), dim3(256), 0, s, hp_pos, hp_w, cap, g, radius, keys,
                        vals);
+}
+
+void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
+                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_layout, dim3(grid1(n, 256)), dim3(256), 0, s, pos, flux, perm, sorted, n, A, B,
+                       C);
 }

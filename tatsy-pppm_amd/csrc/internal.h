@@ -114,13 +114,19 @@ struct EyePassArgs {
 };
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 
-#define GATHER_TILED 0
-#define GATHER_LANES 1
+#define GATHER_TILED PPPM_GATHER_TILED
+#define GATHER_LANES PPPM_GATHER_LANES
+#define GATHER_WAVE PPPM_GATHER_WAVE
+// PPPM_GATHER_AUTO resolves to the kernel measured fastest on MI355X
+#define GATHER_AUTO_CHOICE GATHER_LANES
 
 struct GatherArgs {
-    int32_t mode;           // GATHER_TILED or GATHER_LANES
-    const float4* ph_pos;   // cell order
-    const float4* ph_flux;
+    int32_t mode;           // GATHER_WAVE, GATHER_TILED or GATHER_LANES
+    // photons in bucket order.  Lanes: (x, y, z, oct normal) (Phi, key); the
+    // others: (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g) (Phi_b, key, -, -)
+    const float4* ph_a;
+    const float4* ph_b;
+    const float4* ph_c;
     const uint32_t* off;    // [S + 1]
     const float4* hp_pos;
     const float4* hp_w;
@@ -138,6 +144,10 @@ void launch_gather(hipStream_t s, const GatherArgs& a);
 // NO_HITPOINT get key 0xFFFFFFFF and sort behind every hit point
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
                      double radius, uint32_t* keys, uint32_t* vals);
+// gather layout of n photons: record k = photon perm[sorted[k]] (either may be
+// null); C null selects the compact two-word layout of the lanes kernel
+void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
+                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C);
 
 // origin keys of the stored photons, and the permutation applied to the records
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,

@@ -214,7 +214,7 @@ struct PPMSettings {
     uint32_t sceneId = 0;
     int bounceLimit = 64;
     int hashSize = 0;
-    int gatherMode = PPPM_GATHER_TILED;
+    int gatherMode = PPPM_GATHER_AUTO;
 };
 
 // ---- sources/image.h (FP64 RGB buffer) --------------------------------------

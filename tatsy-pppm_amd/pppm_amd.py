@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpppm_hip.so")
 OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
-GATHER_TILED, GATHER_LANES = 0, 1
+GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE = 0, 1, 2, 3
 
 
 class Bsdf(C.Structure):
@@ -273,7 +273,7 @@ class Context:
         self.camera = cam
 
     def begin(self, photons, passes, radius1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
-              hash_size=0, max_bounces=64, capacity_factor=8.0, gather_mode=GATHER_TILED):
+              hash_size=0, max_bounces=64, capacity_factor=8.0, gather_mode=GATHER_AUTO):
         p = RenderParams(photons=photons, passes=passes, max_bounces=max_bounces, alpha=alpha,
                          radius1=radius1, ray_eps=ray_eps, seed=seed, scene_id=scene_id,
                          hash_size=hash_size, capacity_factor=capacity_factor, gather_mode=gather_mode)

@@ -228,7 +228,7 @@ def test_eye_pass_matches_oracle(gpu_ctx, c1_small):
     assert np.mean(np.sum(hn[sel] * on[p], axis=1) > 0.9999) > 0.999
 
 
-@pytest.mark.parametrize("mode", [P.GATHER_TILED, P.GATHER_LANES])
+@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES])
 def test_gather_matches_oracle(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
     say(f"gather vs oracle mode={mode}")
@@ -257,7 +257,7 @@ def rel_rmse(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.mean(b))
 
 
-@pytest.mark.parametrize("mode", [P.GATHER_TILED, P.GATHER_LANES])
+@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES])
 def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
     say(f"render vs oracle (4 passes) gather mode={mode}")
@@ -303,14 +303,15 @@ def test_full_size_pass_properties(gpu_ctx):
     r1 = 0.01 * s.diagonal()
     st_l = _render_one_pass(gpu_ctx, s, 4_000_000, r1, gather_mode=P.GATHER_LANES)[0]
     img_lanes = gpu_ctx.image()
-    st = _render_one_pass(gpu_ctx, s, 4_000_000, r1)[0]
-    say(f"stats {st}")
-    # the two gather kernels sum the same photon set in different orders
-    img_tiled = gpu_ctx.image()
-    assert st_l["gather_inradius"] == st["gather_inradius"]
-    assert st_l["gather_candidates"] == st["gather_candidates"]
     den = np.maximum(np.abs(img_lanes), 1e-6 * img_lanes.max())
-    assert np.quantile(np.abs(img_tiled - img_lanes) / den, 0.999) < 1e-4
+    for mode in (P.GATHER_TILED, P.GATHER_WAVE):
+        # the gather kernels sum the same photon set in different orders
+        st = _render_one_pass(gpu_ctx, s, 4_000_000, r1, gather_mode=mode)[0]
+        say(f"mode {mode} stats {st}")
+        img = gpu_ctx.image()
+        assert st_l["gather_inradius"] == st["gather_inradius"]
+        assert st_l["gather_candidates"] == st["gather_candidates"]
+        assert np.quantile(np.abs(img - img_lanes) / den, 0.999) < 1e-4
     n = st["photons_stored"]
     assert 1.2 * 4e6 < n < 8 * 4e6
     assert st["hitpoints"] > 0.5 * 1024 * 1024
