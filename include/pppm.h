@@ -130,6 +130,12 @@ typedef struct {
     int64_t gather_inradius;
     double photon_ms, order_ms, grid_ms, eye_ms, gather_ms, total_ms;
     double hitsort_ms;         /* Morton sort of the hit points (tiled gather) */
+    /* learned emission proposal (zero when it is off) */
+    double proposal_ms;        /* contribution records + training + tables */
+    double train_loss;         /* mean weighted cross entropy of the last retrain */
+    int64_t proposal_records;  /* records written this pass */
+    int32_t proposal_active;   /* photons of this pass used the learned mixture */
+    int32_t retrained;         /* the MLP was retrained after this pass */
 } pppm_pass_stats;
 
 /* ---- context ---- */
@@ -203,6 +209,54 @@ int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const floa
  * reference counterpart): the known-traffic kernel that scales profiler
  * FETCH_SIZE / WRITE_SIZE counters, and a measured copy bandwidth. */
 int pppm_stream_copy(void* ctx, int64_t bytes, double* ms);
+
+/* ---- learned emission proposal.  No reference counterpart: it replaces the
+ * cosine emission of sources/sampler.cc:17-32 (the light contract of
+ * sources/envmap.cc:62-86, used by sources/ppm_probabilistic.cc:166-187) by
+ * the defensive mixture  p(w | x) = lambda cos(theta) / pi
+ * + (1 - lambda) q_a(x)[k(w)] / (2 pi / 1024)  over a 32 x 32 equal-area grid
+ * of directions about the light normal (cell k = kz * 32 + kphi with
+ * 1 - cos(theta) in [kz / 32, (kz + 1) / 32) and phi in [kphi, kphi + 1) 2 pi / 32),
+ * where a(x) is the anchor cell of the light sample x on an anchors x anchors
+ * grid.  Photon flux carries the exact ratio (cos(theta) / pi) / p(w), so the
+ * estimator stays unbiased; lambda = 1 is plain emission bit for bit.
+ * q comes from an MLP (Fourier encoding of the light-sample position, 32 ->
+ * 64 -> 64 -> 64 -> 1024, ReLU, BF16 MFMA with FP32 accumulation, Adam in
+ * FP32) retrained every train_interval passes on contribution records of the
+ * last passes.  One light only. ---- */
+#define PPPM_PROPOSAL_CELLS 1024
+#define PPPM_MLP_PARAMS 76992  /* W1 [64][32] b1 W2 [64][64] b2 W3 [64][64] b3 W4 [1024][64] b4 */
+typedef struct {
+    double lambda;            /* weight of the cosine component, (0, 1] */
+    double learning_rate;     /* Adam step size */
+    int64_t ring_capacity;    /* contribution records kept (the most recent) */
+    int32_t anchors;          /* anchors per light edge (1..32) */
+    int32_t train_interval;   /* retrain after every this many passes */
+    int32_t train_steps;      /* Adam steps per retrain */
+    int32_t batch;            /* records per step, a multiple of 256 (<= 65536) */
+    uint64_t seed;            /* weight initialisation and mini-batch draws */
+} pppm_proposal_params;
+/* After pppm_render_begin: starts recording contributions and training; the
+ * first retrain happens after pass train_interval, and passes after it sample
+ * from the mixture.  pppm_render_begin switches the proposal off. */
+int pppm_proposal_enable(void* ctx, const pppm_proposal_params* params);
+/* Fixed tables instead of the MLP (tests, or a proposal trained elsewhere):
+ * prob [anchors^2][1024] cell probabilities per anchor (row-major by the
+ * anchor's (edge_u, edge_v) index), normalised on the device. */
+int pppm_proposal_set_table(void* ctx, const float* prob, int32_t anchors, double lambda);
+/* Current per-anchor inclusive CDFs [anchors^2][1024]. */
+int pppm_proposal_get_table(void* ctx, float* cdf, int64_t n_values, int32_t* anchors);
+/* The contribution records held (oldest first): [n][4] = u, v, cell, c. */
+int pppm_proposal_records(void* ctx, float* rec, int64_t cap, int64_t* n);
+/* MLP test hooks: FP32 master parameters; softmax probabilities [n][1024] at
+ * light-sample coordinates uv [n][2]; one step on given records [n][4]
+ * (n a multiple of 256): the reduced gradient (grad may be NULL), the weighted
+ * cross entropy, and an Adam step when learning_rate > 0 (step = Adam's t). */
+int pppm_mlp_set_params(void* ctx, const float* params, int64_t n);
+int pppm_mlp_get_params(void* ctx, float* params, int64_t n);
+int pppm_mlp_forward(void* ctx, const float* uv, int64_t n, float* prob);
+int pppm_mlp_train_step(void* ctx, const float* rec, int64_t n, double learning_rate, int32_t step,
+                        float* grad, double* loss);
 
 /* ---- procedural scenes (no reference counterpart: the reference loads PLY
  * meshes, sources/trimesh.cc:125-217) ---- */

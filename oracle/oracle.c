@@ -22,6 +22,9 @@
  * orc_eye_pass                  sources/ppm_probabilistic.cc:253-387       mirror-chain / direct-emission closed form
  * orc_gather                    sources/ppm_probabilistic.cc:343-377       brute force; uniform-density closed form
  * orc_render                    sources/ppm_probabilistic.cc:67-164        furnace-style energy check; convergence
+ * emit_direction /              no reference counterpart (the learned      uniform-table and one-hot-table closed
+ *   orc_trace_photons_prop        mixture replacing sampler.cc:17-32)        forms; lambda = 1 equals on_hemisphere;
+ *                                                                          E[factor] = 1 (unbiasedness)
  *
  * Deliberate differences from the literal reference (each is a quirk the
  * reference cannot run with): fixed-radius gather instead of the swapped kNN,
@@ -596,10 +599,61 @@ static inline v3 spawn(v3 p, v3 ng, v3 dir, double eps) {
 
 typedef struct { float p[3], f[3], n[3]; int32_t bounce; } deposit;
 
+/* Direction cell of the learned proposal (pppm_device.h prop_pick/prop_weight):
+ * 32 x 32 equal-area cells about the light normal, 1 - cos(theta) in bands of
+ * 1/32, phi in sectors of 2 pi / 32; mixture lambda * cosine + (1 - lambda) *
+ * the anchor's CDF row; returns the flux factor (cos / pi) / p(w). */
+static double emit_direction(const orc_proposal* pr, v3 nl, double pu, double pv, rng_t* r, v3* dir,
+                             uint32_t* cell_out) {
+    double r3 = rng_at(r, 3), r4 = rng_at(r, 4);
+    if (!pr || pr->side == 0) {
+        *dir = on_hemisphere(nl, r3, r4);
+        return 1.0;
+    }
+    double sel = rng_at(r, 5), j6 = rng_at(r, 6), j7 = rng_at(r, 7);
+    int side = pr->side;
+    int ia = (int)(pu * side), ja = (int)(pv * side);
+    if (ia > side - 1) ia = side - 1;
+    if (ja > side - 1) ja = side - 1;
+    const float* cdf = pr->cdf + (size_t)(ia * side + ja) * 1024;
+    double cos_t;
+    int kz, kp;
+    if (sel < pr->lambda) {
+        *dir = on_hemisphere(nl, r3, r4);
+        cos_t = sqrt(1.0 - r4);
+        kz = (int)((1.0 - cos_t) * 32.0);
+        kp = (int)(r3 * 32.0);
+        if (kz > 31) kz = 31;
+        if (kp > 31) kp = 31;
+    } else {
+        int lo = 0, hi = 1023;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (r3 < (double)cdf[mid]) hi = mid;
+            else lo = mid + 1;
+        }
+        kz = lo / 32;
+        kp = lo % 32;
+        cos_t = 1.0 - ((double)kz + j6) / 32.0;
+        double phi = 2.0 * PI * ((double)kp + j7) / 32.0;
+        double sin_t = sqrt(fmax(0.0, 1.0 - cos_t * cos_t));
+        v3 u;
+        if (fabs(nl.x) > EPS) u = normalized(cross(V(0.0, 1.0, 0.0), nl));
+        else u = normalized(cross(V(1.0, 0.0, 0.0), nl));
+        v3 v = cross(nl, u);
+        *dir = normalized(add(add(scl(u, cos(phi) * sin_t), scl(v, sin(phi) * sin_t)), scl(nl, cos_t)));
+    }
+    int cell = kz * 32 + kp;
+    if (cell_out) *cell_out = (uint32_t)cell;
+    float qf = cell ? cdf[cell] - cdf[cell - 1] : cdf[cell];
+    return cos_t / (pr->lambda * cos_t + (1.0 - pr->lambda) * 512.0 * (double)qf);
+}
+
 /* Emission: pick a light by power, uniform point on the parallelogram,
- * cosine direction about its normal (sampler.cc:17-32); flux = P_total / N. */
-static void emit_photon(const orc_scene* s, rng_t* r, int64_t N, v3* pos, v3* dir, v3* flux,
-                        v3* nl_out) {
+ * cosine direction about its normal (sampler.cc:17-32), or the learned
+ * mixture when pr is given; flux = P_total / N times the mixture factor. */
+static void emit_photon(const orc_scene* s, const orc_proposal* pr, rng_t* r, int64_t N, v3* pos, v3* dir,
+                        v3* flux, v3* nl_out) {
     double ul = rng_at(r, 0);
     int l = 0;
     double target = ul * s->light_total;
@@ -612,19 +666,19 @@ static void emit_photon(const orc_scene* s, rng_t* r, int64_t N, v3* pos, v3* di
     *pos = add(c, add(scl(eu, u1), scl(ev, u2)));
     v3 nl = normalized(cross(eu, ev));
     *nl_out = nl;
-    *dir = on_hemisphere(nl, rng_at(r, 3), rng_at(r, 4));
+    double wgt = emit_direction(pr, nl, u1, u2, r, dir, NULL);
     double pl = (s->light_cdf[l + 1] - s->light_cdf[l]) / s->light_total;
     double area = norm(cross(eu, ev));
-    double k = PI * area / (pl * (double)N);
+    double k = PI * area / (pl * (double)N) * wgt;
     *flux = V(L->emission[0] * k, L->emission[1] * k, L->emission[2] * k);
 }
 
-static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, int64_t idx,
-                            deposit* out, int cap_out) {
+static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, const orc_proposal* pr,
+                            int64_t idx, deposit* out, int cap_out) {
     rng_t r;
     rng_init(&r, P->seed, KIND_PHOTON, (uint32_t)P->pass_index, (uint32_t)idx, P->scene_id);
     v3 pos, dir, flux, nl;
-    emit_photon(s, &r, P->photons, &pos, &dir, &flux, &nl);
+    emit_photon(s, pr, &r, P->photons, &pos, &dir, &flux, &nl);
     v3 o = add(pos, scl(nl, P->ray_eps));
     int nd = 0;
     for (int bounce = 0;; bounce++) {
@@ -676,6 +730,11 @@ static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, int64_
 
 int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* P, float* pos, float* flux,
                           float* nrm, uint32_t* key, int64_t cap) {
+    return orc_trace_photons_prop(s, P, NULL, pos, flux, nrm, key, cap);
+}
+
+int64_t orc_trace_photons_prop(const orc_scene* s, const orc_pass_params* P, const orc_proposal* pr,
+                               float* pos, float* flux, float* nrm, uint32_t* key, int64_t cap) {
     int64_t N = P->photons;
     int32_t* counts = (int32_t*)calloc((size_t)(N + 1), sizeof(int32_t));
     deposit* tmp = (deposit*)malloc(sizeof(deposit) * (size_t)(N ? N : 1) * 4);
@@ -684,7 +743,7 @@ int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* P, float* p
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t i = 0; i < N; i++) {
         deposit buf[MAX_DEP];
-        int nd = trace_one_photon(s, P, i, buf, MAX_DEP);
+        int nd = trace_one_photon(s, P, pr, i, buf, MAX_DEP);
         counts[i + 1] = nd;
         for (int k = 0; k < nd && k < 4; k++) tmp[i * 4 + k] = buf[k];
     }
@@ -698,7 +757,7 @@ int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* P, float* p
         int nd = counts[i + 1];
         deposit buf[MAX_DEP];
         const deposit* src = &tmp[i * 4];
-        if (nd > 4) { trace_one_photon(s, P, i, buf, MAX_DEP); src = buf; }
+        if (nd > 4) { trace_one_photon(s, P, pr, i, buf, MAX_DEP); src = buf; }
         for (int k = 0; k < nd && k < MAX_DEP; k++) {
             int64_t j = off[i] + k;
             if (j >= cap) break;
@@ -996,4 +1055,29 @@ int32_t orc_tri_intersect(const double tri[9], const double o[3], const double d
     double tt = tri_intersect(&s, 0, V(o[0], o[1], o[2]), V(d[0], d[1], d[2]));
     *t = tt;
     return tt > 0.0 ? 1 : 0;
+}
+
+void orc_emission(const orc_proposal* pr, const orc_pass_params* P, int64_t idx, const double nl[3],
+                  double out_uv[2], double out_dir[3], double* weight, uint32_t* cell) {
+    rng_t r;
+    rng_init(&r, P->seed, KIND_PHOTON, (uint32_t)P->pass_index, (uint32_t)idx, P->scene_id);
+    double u1 = rng_at(&r, 1), u2 = rng_at(&r, 2);
+    v3 d;
+    uint32_t c = 0;
+    if (!pr || pr->side == 0) {
+        double r3 = rng_at(&r, 3), r4 = rng_at(&r, 4);
+        d = on_hemisphere(V(nl[0], nl[1], nl[2]), r3, r4);
+        double ct = sqrt(1.0 - r4);
+        int kz = (int)((1.0 - ct) * 32.0), kp = (int)(r3 * 32.0);
+        c = (uint32_t)((kz > 31 ? 31 : kz) * 32 + (kp > 31 ? 31 : kp));
+        *weight = 1.0;
+    } else {
+        *weight = emit_direction(pr, V(nl[0], nl[1], nl[2]), u1, u2, &r, &d, &c);
+    }
+    out_uv[0] = u1;
+    out_uv[1] = u2;
+    out_dir[0] = d.x;
+    out_dir[1] = d.y;
+    out_dir[2] = d.z;
+    *cell = c;
 }

@@ -118,6 +118,23 @@ typedef struct {
 int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* p,
                           float* pos, float* flux, float* normal, uint32_t* key, int64_t cap);
 
+/* Learned emission proposal (pppm_device.h, include/pppm.h): per-anchor
+ * inclusive CDF rows [side^2][1024] over 32 x 32 equal-area direction cells,
+ * and the weight lambda of the cosine component. */
+typedef struct {
+    const float* cdf;
+    double lambda;
+    int32_t side;
+    int32_t pad;
+} orc_proposal;
+/* Photon pass emitting from the mixture (pr NULL: plain cosine emission). */
+int64_t orc_trace_photons_prop(const orc_scene* s, const orc_pass_params* p, const orc_proposal* pr,
+                               float* pos, float* flux, float* normal, uint32_t* key, int64_t cap);
+/* Emission of photon idx of a pass about light normal nl: light-sample
+ * coordinates, direction, flux factor (cos / pi) / p(w) and direction cell. */
+void orc_emission(const orc_proposal* pr, const orc_pass_params* p, int64_t idx, const double nl[3],
+                  double out_uv[2], double out_dir[3], double* weight, uint32_t* cell);
+
 /* Eye pass: one path per pixel.  For each pixel writes either a hit point
  * (has_hit=1: pos, oriented normal, weight = throughput * rho / pi) or a
  * direct contribution (emission seen through specular chains). */

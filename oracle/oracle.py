@@ -26,6 +26,18 @@ class PassParams(C.Structure):
                 ("scene_id", C.c_uint32), ("pad", C.c_int32)]
 
 
+class Proposal(C.Structure):
+    _fields_ = [("cdf", C.c_void_p), ("lam", C.c_double), ("side", C.c_int32), ("pad", C.c_int32)]
+
+
+def proposal(cdf, side, lam):
+    """orc_proposal over a [side^2][1024] float32 CDF array (kept alive on the struct)."""
+    cdf = np.ascontiguousarray(cdf, np.float32).reshape(side * side, 1024)
+    pr = Proposal(cdf=cdf.ctypes.data, lam=lam, side=side)
+    pr._keep = cdf
+    return pr
+
+
 _ODECL = {
     "orc_philox4x32_10": (None, [_P, _P, _P]),
     "orc_uniforms": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _P]),
@@ -38,6 +50,9 @@ _ODECL = {
     "orc_hashgrid_boxes": (C.c_int64, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, C.c_int64, _P, _P]),
     "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_int32, C.c_int32, _P]),
     "orc_trace_photons": (C.c_int64, [_P, C.POINTER(PassParams), _P, _P, _P, _P, C.c_int64]),
+    "orc_trace_photons_prop": (C.c_int64, [_P, C.POINTER(PassParams), C.POINTER(Proposal), _P, _P, _P, _P,
+                                           C.c_int64]),
+    "orc_emission": (None, [C.POINTER(Proposal), C.POINTER(PassParams), C.c_int64, _P, _P, _P, _P, _P]),
     "orc_eye_pass": (None, [_P, _P, C.POINTER(PassParams), _P, _P, _P, _P, _P]),
     "orc_gather": (None, [_P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double, C.c_int32, _P, _P, _P]),
     "orc_render": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
@@ -143,14 +158,18 @@ class OracleScene:
         return PassParams(photons=photons, pass_index=pass_index, max_bounces=max_bounces,
                           radius=radius, ray_eps=ray_eps, seed=seed, scene_id=scene_id)
 
-    def trace_photons(self, photons, pass_index, ray_eps, seed, scene_id=0, cap=None):
+    def trace_photons(self, photons, pass_index, ray_eps, seed, scene_id=0, cap=None, prop=None):
         cap = cap or photons * 8 + 1024
         pos = np.zeros((cap, 3), np.float32)
         flux = np.zeros((cap, 3), np.float32)
         nrm = np.zeros((cap, 3), np.float32)
         key = np.zeros(cap, np.uint32)
         p = self._params(photons, pass_index, 0.0, ray_eps, seed, scene_id)
-        n = orc().orc_trace_photons(self.h, C.byref(p), _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), cap)
+        if prop is None:
+            n = orc().orc_trace_photons(self.h, C.byref(p), _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), cap)
+        else:
+            n = orc().orc_trace_photons_prop(self.h, C.byref(p), C.byref(prop), _ptr(pos), _ptr(flux), _ptr(nrm),
+                                             _ptr(key), cap)
         assert n <= cap
         self.last_keys = key[:n]
         return pos[:n], flux[:n], nrm[:n]
@@ -216,3 +235,17 @@ def gather(ph_pos, ph_flux, ph_nrm, hit_pos, hit_nrm, hit_w, radius, hash_size, 
     orc().orc_gather(_ptr(a[0]), _ptr(a[1]), _ptr(a[2]), len(a[0]), _ptr(a[3]), _ptr(a[4]), _ptr(a[5]),
                      len(a[3]), radius, hash_size, _ptr(bm), _ptr(out), _ptr(cnt))
     return out[:len(a[3])], cnt
+
+
+def emission(prop, photons, pass_index, seed, idx, normal, scene_id=0):
+    """Light-sample (u, v), direction, flux factor and cell of photon idx."""
+    p = PassParams(photons=photons, pass_index=pass_index, max_bounces=64, radius=0.0, ray_eps=0.0,
+                   seed=seed, scene_id=scene_id)
+    nl = np.ascontiguousarray(normal, np.float64)
+    uv = np.zeros(2)
+    d = np.zeros(3)
+    w = C.c_double()
+    c = C.c_uint32()
+    orc().orc_emission(C.byref(prop) if prop is not None else None, C.byref(p), idx, _ptr(nl), _ptr(uv),
+                       _ptr(d), C.byref(w), C.byref(c))
+    return uv, d, w.value, c.value

@@ -391,7 +391,24 @@ __global__ void k_gather_layout(const float4* __restrict__ pos, const float4* __
     Cc[k] = make_float4(f.z, f.w, 0.0f, 0.0f);
 }
 
+// flags[h] = 1 for every bucket that some hit point's gather visits
+__global__ __launch_bounds__(256) void k_mark_buckets(GatherArgs a, uint32_t* flags) {
+    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t mb[8];
+    const uint32_t vm = own_buckets(a, a.hp_pos[i], mb);
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+        if ((vm >> c) & 1u) flags[mb[c]] = 1u;
+}
+
 }  // namespace
+
+void launch_mark_buckets(hipStream_t s, const GatherArgs& a, uint32_t* flags) {
+    if (a.n_hit <= 0) return;
+    hipLaunchKernelGGL(k_mark_buckets, dim3((unsigned)((a.n_hit + 255) / 256)), dim3(256), 0, s, a, flags);
+}
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 

@@ -77,6 +77,7 @@ struct Counters {
     unsigned long long gather_candidates;
     unsigned long long gather_inradius;
     unsigned long long stack_overflow;
+    unsigned long long proposal_records;
 };
 
 struct PhotonPassArgs {
@@ -91,6 +92,7 @@ struct PhotonPassArgs {
     float4* ph_pos;   // xyz, oct normal bits
     float4* ph_flux;  // rgb, origin key bits (photon << 6 | bounce)
     Counters* counters;
+    PropDev prop;     // emission proposal (side 0: plain cosine emission)
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 
@@ -140,6 +142,9 @@ struct GatherArgs {
     Counters* counters;
 };
 void launch_gather(hipStream_t s, const GatherArgs& a);
+// flags[h] = 1 for each bucket h visited by the gather of a's hit points
+// (flags are not cleared here)
+void launch_mark_buckets(hipStream_t s, const GatherArgs& a, uint32_t* flags);
 // Morton keys of the half cells of the hit points; slots whose pixel id is
 // NO_HITPOINT get key 0xFFFFFFFF and sort behind every hit point
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
@@ -160,3 +165,90 @@ void launch_zero_u32(hipStream_t s, void* p, int64_t words);
 void launch_stream_copy(hipStream_t s, const float4* src, float4* dst, int64_t n);
 void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const float* n3,
                         int64_t n, float4* pos4, float4* b4, int normals);
+
+// ---------------------------------------------------------------------------
+// Learned emission proposal (proposal.hip)
+// ---------------------------------------------------------------------------
+#define MLP_IN 32
+#define MLP_HID 64
+#define MLP_OUT 1024
+#define MLP_SPLITS 16  // K chunks of the weight-gradient products
+// FP32 master parameters: W1 [64][32] b1, W2 [64][64] b2, W3 [64][64] b3, W4 [1024][64] b4
+#define OW1 0
+#define OB1 2048
+#define OW2 2112
+#define OB2 6208
+#define OW3 6272
+#define OB3 10368
+#define OW4 10432
+#define OB4 75968
+#define MLP_NPARAM 76992
+// BF16 mirror: W1, W2, W3, W4 and the transposes of W2, W3, W4
+#define MW1 0
+#define MW2 2048
+#define MW3 6144
+#define MW4 10240
+#define MW2T 75776
+#define MW3T 79872
+#define MW4T 83968
+#define MLP_NMIRROR 149504
+// feature-major BF16 activations of a training batch, rows of [F][B]
+#define ACT_X0 0
+#define ACT_H1 32
+#define ACT_H2 96
+#define ACT_H3 160
+#define ACT_D1 224
+#define ACT_D2 288
+#define ACT_D3 352
+#define ACT_DZ 416
+#define ACT_ROWS 1440
+
+struct MlpArgs {
+    const float* params;     // FP32 master (biases are read from here)
+    const uint16_t* wb;      // BF16 mirror
+    const float4* rec;       // training records (u, v, cell bits, weight)
+    const uint32_t* idx;     // batch sample -> record (null: identity)
+    const float* inv_wsum;   // 1 / sum of the batch weights
+    const float2* uv;        // inference inputs
+    int64_t n;               // samples (training: multiple of 256)
+    float* prob;             // inference output [n][1024]
+    uint16_t* act;           // training activations [ACT_ROWS][n]
+    float* loss;             // weighted cross entropy (accumulated; may be null)
+};
+struct AdamArgs {
+    float* params;
+    float* m;
+    float* v;
+    const float* part;       // [MLP_SPLITS][part_stride] split-K gradient partials
+    int64_t part_stride;
+    uint16_t* wb;
+    float* grad_out;         // reduced gradient (may be null)
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t)
+    int32_t apply;
+};
+struct ContribArgs {
+    const float4* ph_pos;    // photon records as appended
+    const float4* ph_flux;
+    const uint32_t* perm;    // origin order
+    int64_t n;
+    GridParams g;
+    const uint32_t* flags;   // buckets visited by hit points
+    float* seg_c;            // [n]
+    uint64_t* scan;          // [n]
+    PropDev prop;
+    uint64_t seed;
+    uint32_t pass;
+    uint32_t scene_id;
+    float4* ring;
+    int64_t ring_cap;
+    int64_t head;
+    Counters* counters;      // proposal_records = records of this pass
+};
+void launch_mlp(hipStream_t s, const MlpArgs& a, bool train);
+void launch_mlp_wgrad(hipStream_t s, const uint16_t* act, int64_t B, float* part);
+void launch_mlp_adam(hipStream_t s, const AdamArgs& a);
+void launch_mlp_mirror(hipStream_t s, const float* params, uint16_t* wb);
+void launch_mlp_prep(hipStream_t s, const float4* ring, int64_t n_valid, int64_t B, uint64_t seed, uint32_t step,
+                     int identity, uint32_t* idx, float* inv_wsum);
+void launch_prop_table(hipStream_t s, const float* prob, int32_t anchors, float* cdf);
+void launch_contrib(hipStream_t s, const ContribArgs& a, uint64_t* block_sums);

@@ -52,8 +52,8 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
     return c;
 }
 
-enum : uint32_t { KIND_PHOTON = 1u, KIND_EYE = 2u };
-// photon dims: 0 light pick, 1-2 position, 3-4 direction, 5-7 reserved,
+enum : uint32_t { KIND_PHOTON = 1u, KIND_EYE = 2u, KIND_TRAIN = 3u };
+// photon dims: 0 light pick, 1-2 position, 3-4 direction, 5-7 proposal,
 // bounce b: 8+3b .. 10+3b.  eye dims: 0-1 jitter, bounce b: 2+3b .. 4+3b.
 #define PH_DIM_BOUNCE 8
 #define EYE_DIM_BOUNCE 2
@@ -269,4 +269,72 @@ __device__ inline int32_t grid_cell(double x, double mn, double scale) {
 __device__ inline uint32_t grid_hash(int32_t ix, int32_t iy, int32_t iz, uint32_t size) {
     uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
     return h % size;
+}
+
+// ---------------------------------------------------------------------------
+// Learned emission proposal: the direction pdf about a light's normal w is the
+// defensive mixture  p(w) = lambda cos(theta) / pi + (1 - lambda) q_k / dOmega
+// over 32 x 32 equal-area cells (cos(theta) in 32 bands of 1/32, phi in 32
+// sectors; dOmega = 2 pi / 1024 for every cell), with q the CDF row of the
+// anchor (the light-position cell of the sample).  The cosine component is the
+// plain emission of sources/sampler.cc:17-32, so lambda = 1 reproduces it bit
+// for bit.  Photon dims: 3-4 direction, 5 component choice, 6-7 jitter inside
+// the chosen cell.
+// ---------------------------------------------------------------------------
+#define PROP_SIDE 32
+#define PROP_CELLS 1024
+#define PH_DIM_PROP_SEL 5
+#define PH_DIM_PROP_JIT 6
+
+struct PropDev {
+    const float* cdf;  // [side * side][1024] inclusive CDF per anchor, last entry 1
+    float lambda;      // weight of the cosine component
+    int32_t side;      // anchors per light edge; 0 = plain cosine emission
+};
+
+struct PropPick {
+    uint32_t cell;     // direction cell kz * 32 + kphi
+    float cos_t;       // cos(theta) of the direction
+    float phi_u;       // phi / (2 pi); < 0: the cosine component (use on_hemisphere)
+};
+
+// Component, cell and (cos theta, phi) of an emitted direction; pu, pv are the
+// light-sample coordinates along edge_u, edge_v.  r3, r4 are dims 3-4.
+__device__ inline PropPick prop_pick(const PropDev& p, float pu, float pv, float r3, float r4,
+                                     float sel, float j6, float j7) {
+    PropPick k;
+    if (p.side == 0 || sel < p.lambda) {
+        // cosine component: phi = 2 pi r3, sin^2(theta) = r4
+        k.cos_t = sqrtf(1.0f - r4);
+        const int kz = min((int)((1.0f - k.cos_t) * (float)PROP_SIDE), PROP_SIDE - 1);
+        const int kp = min((int)(r3 * (float)PROP_SIDE), PROP_SIDE - 1);
+        k.cell = (uint32_t)(kz * PROP_SIDE + kp);
+        k.phi_u = -1.0f;
+        return k;
+    }
+    const int ia = min((int)(pu * (float)p.side), p.side - 1);
+    const int ja = min((int)(pv * (float)p.side), p.side - 1);
+    const float* cdf = p.cdf + (size_t)(ia * p.side + ja) * PROP_CELLS;
+    int lo = 0, hi = PROP_CELLS - 1;  // first k with r3 < cdf[k]
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (r3 < cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    k.cell = (uint32_t)lo;
+    const int kz = lo / PROP_SIDE, kp = lo % PROP_SIDE;
+    k.cos_t = 1.0f - ((float)kz + j6) * (1.0f / PROP_SIDE);
+    k.phi_u = ((float)kp + j7) * (1.0f / PROP_SIDE);
+    return k;
+}
+
+// Flux factor (cos(theta) / pi) / p(w) of a picked direction; exactly 1 when
+// lambda = 1 (or the proposal is off)
+__device__ inline float prop_weight(const PropDev& p, float pu, float pv, const PropPick& k) {
+    if (p.side == 0) return 1.0f;
+    const int ia = min((int)(pu * (float)p.side), p.side - 1);
+    const int ja = min((int)(pv * (float)p.side), p.side - 1);
+    const float* cdf = p.cdf + (size_t)(ia * p.side + ja) * PROP_CELLS;
+    const float q = cdf[k.cell] - (k.cell ? cdf[k.cell - 1] : 0.0f);
+    return k.cos_t / (p.lambda * k.cos_t + (1.0f - p.lambda) * (0.5f * PROP_CELLS) * q);
 }

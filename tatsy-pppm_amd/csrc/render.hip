@@ -35,6 +35,17 @@ __device__ inline v3f on_hemisphere(v3f w, float r1, float r2) {
     return normalize(u * (c * z2s) + v * (s * z2s) + w * sqrtf(1.0f - r2));
 }
 
+// Direction (cos theta, phi = 2 pi phi_u) in on_hemisphere's frame about w
+__device__ inline v3f cell_direction(v3f w, float cos_t, float phi_u) {
+    v3f u = fabsf(w.x) > 1.0e-12f ? normalize(cross(mk(0.0f, 1.0f, 0.0f), w))
+                                  : normalize(cross(mk(1.0f, 0.0f, 0.0f), w));
+    v3f v = cross(w, u);
+    float s, c;
+    __sincosf(2.0f * PPPM_PI * phi_u, &s, &c);
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    return normalize(u * (c * sin_t) + v * (s * sin_t) + w * cos_t);
+}
+
 __device__ inline v3f reflect(v3f v, v3f n) { return v - n * (2.0f * dot(n, v)); }
 
 // brdf.cc:117-142 with reflectance.h:7-34: returns the new direction, *pdf
@@ -90,9 +101,20 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
         int l = 0;
         while (l < sc.n_lights - 1 && sc.lights[l].cdf_hi <= ul) l++;
         const LightDev L = sc.lights[l];
-        v3f pos = L.corner + L.eu * rng.at(1) + L.ev * rng.at(2);
-        v3f dir = on_hemisphere(L.normal, rng.at(3), rng.at(4));
+        const float pu = rng.at(1), pv = rng.at(2);
+        v3f pos = L.corner + L.eu * pu + L.ev * pv;
+        v3f dir;
         v3f flux = L.flux;
+        const float r3 = rng.at(3), r4 = rng.at(4);
+        if (a.prop.side == 0) {
+            dir = on_hemisphere(L.normal, r3, r4);
+        } else {
+            const float sel = rng.at(PH_DIM_PROP_SEL);
+            const float j6 = rng.at(PH_DIM_PROP_JIT), j7 = rng.at(PH_DIM_PROP_JIT + 1);
+            const PropPick k = prop_pick(a.prop, pu, pv, r3, r4, sel, j6, j7);
+            dir = k.phi_u < 0.0f ? on_hemisphere(L.normal, r3, r4) : cell_direction(L.normal, k.cos_t, k.phi_u);
+            flux = flux * prop_weight(a.prop, pu, pv, k);
+        }
         v3f o = pos + L.normal * a.ray_eps;
         for (int bounce = 0;; bounce++) {
             if (bounce >= a.max_bounces || vmax3(flux) <= 0.0f) break;

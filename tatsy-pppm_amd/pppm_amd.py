@@ -16,6 +16,8 @@ OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
 GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE = 0, 1, 2, 3
+PROPOSAL_CELLS = 1024
+MLP_PARAMS = 76992
 
 
 class Bsdf(C.Structure):
@@ -54,10 +56,17 @@ class PassStats(C.Structure):
                 ("gather_candidates", C.c_int64), ("gather_inradius", C.c_int64),
                 ("photon_ms", C.c_double), ("order_ms", C.c_double), ("grid_ms", C.c_double),
                 ("eye_ms", C.c_double), ("gather_ms", C.c_double), ("total_ms", C.c_double),
-                ("hitsort_ms", C.c_double)]
+                ("hitsort_ms", C.c_double), ("proposal_ms", C.c_double), ("train_loss", C.c_double),
+                ("proposal_records", C.c_int64), ("proposal_active", C.c_int32), ("retrained", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ProposalParams(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("learning_rate", C.c_double), ("ring_capacity", C.c_int64),
+                ("anchors", C.c_int32), ("train_interval", C.c_int32), ("train_steps", C.c_int32),
+                ("batch", C.c_int32), ("seed", C.c_uint64)]
 
 
 class SceneDesc(C.Structure):
@@ -94,6 +103,14 @@ _DECLS = {
     "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
                               C.c_int32, _P, C.c_int32, _P, C.POINTER(PassStats)]),
     "pppm_stream_copy": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
+    "pppm_proposal_enable": (C.c_int, [_P, C.POINTER(ProposalParams)]),
+    "pppm_proposal_set_table": (C.c_int, [_P, _P, C.c_int32, C.c_double]),
+    "pppm_proposal_get_table": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int32)]),
+    "pppm_proposal_records": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "pppm_mlp_set_params": (C.c_int, [_P, _P, C.c_int64]),
+    "pppm_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),
+    "pppm_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "pppm_mlp_train_step": (C.c_int, [_P, _P, C.c_int64, C.c_double, C.c_int32, _P, C.POINTER(C.c_double)]),
     "pppm_scenegen": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SceneDesc)]),
     "pppm_scenegen_free": (None, [C.POINTER(SceneDesc)]),
 }
@@ -336,3 +353,52 @@ class Context:
                                      _ptr(a[4]), _ptr(a[5]), len(a[3]), radius, hash_size, _ptr(bm),
                                      mode, _ptr(out), C.byref(st)))
         return out[:len(a[3])], st.as_dict()
+
+    # ---- learned emission proposal ----
+    def proposal_enable(self, lam=0.25, anchors=8, train_interval=8, train_steps=64, batch=16384,
+                        ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED):
+        p = ProposalParams(lambda_=lam, learning_rate=learning_rate, ring_capacity=ring_capacity,
+                           anchors=anchors, train_interval=train_interval, train_steps=train_steps,
+                           batch=batch, seed=seed)
+        self._chk(self.L.pppm_proposal_enable(self.h, C.byref(p)))
+
+    def proposal_set_table(self, prob, anchors, lam):
+        prob = np.ascontiguousarray(prob, np.float32).reshape(anchors * anchors, PROPOSAL_CELLS)
+        self._chk(self.L.pppm_proposal_set_table(self.h, _ptr(prob), anchors, lam))
+
+    def proposal_table(self):
+        a = C.c_int32()
+        self._chk(self.L.pppm_proposal_get_table(self.h, None, 0, C.byref(a)))
+        cdf = np.zeros((a.value * a.value, PROPOSAL_CELLS), np.float32)
+        self._chk(self.L.pppm_proposal_get_table(self.h, _ptr(cdf), cdf.size, C.byref(a)))
+        return cdf, a.value
+
+    def proposal_records(self):
+        n = C.c_int64()
+        self._chk(self.L.pppm_proposal_records(self.h, None, 0, C.byref(n)))
+        rec = np.zeros((max(n.value, 1), 4), np.float32)
+        self._chk(self.L.pppm_proposal_records(self.h, _ptr(rec), n.value, C.byref(n)))
+        return rec[:n.value]
+
+    def mlp_set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._chk(self.L.pppm_mlp_set_params(self.h, _ptr(w), w.size))
+
+    def mlp_params(self):
+        w = np.zeros(MLP_PARAMS, np.float32)
+        self._chk(self.L.pppm_mlp_get_params(self.h, _ptr(w), w.size))
+        return w
+
+    def mlp_forward(self, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((max(len(uv), 1), PROPOSAL_CELLS), np.float32)
+        self._chk(self.L.pppm_mlp_forward(self.h, _ptr(uv), len(uv), _ptr(out)))
+        return out[:len(uv)]
+
+    def mlp_train_step(self, rec, learning_rate=0.0, step=1):
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, 4)
+        grad = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        self._chk(self.L.pppm_mlp_train_step(self.h, _ptr(rec), len(rec), learning_rate, step, _ptr(grad),
+                                             C.byref(loss)))
+        return grad, loss.value

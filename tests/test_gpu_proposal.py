@@ -1,0 +1,253 @@
+"""GPU parity of the learned emission proposal through the C ABI.
+
+Bars: the CDF tables and contribution records are bit-exact against the CPU
+restatement on the same inputs (the record's cell may differ only where FP32
+and FP64 disagree at a cell boundary); MLP probabilities within 2 % per cell
+and gradients within 3 % (relative L2 per layer) of a float64 restatement that
+rounds weights and activations to BF16 as the device does (the device sums in
+FP32 on the matrix cores); photon records with a fixed table as for plain
+emission; lambda = 1 reproduces plain emission bit for bit.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pppm_amd as P
+import proposal_ref as R
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-4
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(R.NPARAM, np.float32)
+    o = 0
+    for name, shape in R.LAYOUT:
+        k = int(np.prod(shape))
+        fan = shape[1] if len(shape) == 2 else 64
+        w[o:o + k] = rng.normal(scale=scale * math.sqrt(2.0 / fan), size=k) if len(shape) == 2 else \
+            rng.normal(scale=0.1, size=k)
+        o += k
+    return w
+
+
+def _records(seed, n):
+    rng = np.random.default_rng(seed)
+    cells = np.where(rng.random(n) < 0.5, rng.integers(0, 64, n), rng.integers(0, 1024, n))
+    return np.column_stack([rng.random(n), rng.random(n), cells, rng.gamma(0.5, size=n) + 1e-3]).astype(np.float32)
+
+
+def test_mlp_forward_matches_reference(gpu_ctx):
+    say("mlp forward")
+    w = _random_params(1)
+    gpu_ctx.mlp_set_params(w)
+    assert np.array_equal(gpu_ctx.mlp_params(), w)
+    uv = np.random.default_rng(2).random((1000, 2)).astype(np.float32)
+    g = gpu_ctx.mlp_forward(uv).astype(np.float64)
+    r = R.forward(w, uv, emulate=True)
+    np.testing.assert_allclose(g.sum(axis=1), 1.0, atol=1e-4)
+    close = np.abs(g - r) <= 0.02 * r + 1e-7
+    assert close.mean() > 0.999, close.mean()
+    # a transposed operand or a wrong lane layout would scramble the cells
+    assert np.mean(np.argmax(g, axis=1) == np.argmax(r, axis=1)) > 0.98
+
+
+def test_mlp_forward_ragged_sizes(gpu_ctx):
+    say("mlp forward ragged")
+    w = _random_params(5)
+    gpu_ctx.mlp_set_params(w)
+    uv = np.random.default_rng(6).random((77, 2)).astype(np.float32)
+    full = gpu_ctx.mlp_forward(uv)
+    for n in (1, 15, 17, 64, 65):
+        assert np.array_equal(gpu_ctx.mlp_forward(uv[:n]), full[:n])
+
+
+def test_mlp_gradient_matches_reference(gpu_ctx):
+    say("mlp gradient")
+    w = _random_params(3)
+    gpu_ctx.mlp_set_params(w)
+    rec = _records(4, 1024)
+    grad, loss = gpu_ctx.mlp_train_step(rec, learning_rate=0.0)
+    rl, rg = R.loss_and_grad(w, rec, emulate=True)
+    assert abs(loss - rl) < 2e-3 * abs(rl), (loss, rl)
+    o = 0
+    for name, shape in R.LAYOUT:
+        k = int(np.prod(shape))
+        a, b = grad[o:o + k].astype(np.float64), rg[o:o + k]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert err < 3e-2, (name, err)
+        o += k
+    # no update with learning rate 0
+    assert np.array_equal(gpu_ctx.mlp_params(), w)
+
+
+def test_adam_step_matches_reference(gpu_ctx):
+    say("adam step")
+    w = _random_params(7)
+    gpu_ctx.mlp_set_params(w)
+    rec = _records(8, 512)
+    g1, _ = gpu_ctx.mlp_train_step(rec, learning_rate=1e-2, step=1)
+    w1 = gpu_ctx.mlp_params()
+    r1, m1, v1 = R.adam(w, np.zeros_like(w), np.zeros_like(w), g1, 1, 1e-2)
+    np.testing.assert_allclose(w1, r1, rtol=0, atol=2e-6)
+    g2, _ = gpu_ctx.mlp_train_step(rec, learning_rate=1e-2, step=2)
+    r2, _, _ = R.adam(r1, m1, v1, g2, 2, 1e-2)
+    np.testing.assert_allclose(gpu_ctx.mlp_params(), r2, rtol=0, atol=4e-6)
+
+
+def test_training_lowers_the_loss(gpu_ctx):
+    say("training reduces loss")
+    gpu_ctx.mlp_set_params(_random_params(9, scale=0.5) * np.r_[np.ones(10432), np.zeros(R.NPARAM - 10432)]
+                           .astype(np.float32))
+    rng = np.random.default_rng(10)
+    n = 4096
+    uv = rng.random((n, 2))
+    # contributions concentrated in a position-dependent cell
+    cell = (np.floor(uv[:, 0] * 4) * 32 + np.floor(uv[:, 1] * 4) + 100).astype(np.int64)
+    rec = np.column_stack([uv, cell, np.ones(n)]).astype(np.float32)
+    losses = [gpu_ctx.mlp_train_step(rec, learning_rate=3e-3, step=t)[1] for t in range(1, 121)]
+    assert losses[0] == pytest.approx(math.log(1024), rel=1e-3)
+    assert losses[-1] < 0.35 * losses[0], losses[::20]
+    p = gpu_ctx.mlp_forward(uv[:256].astype(np.float32))
+    assert np.mean(np.argmax(p, axis=1) == cell[:256]) > 0.9
+
+
+def _setup(ctx, s, photons, passes, r1, seed=0x7A75):
+    ctx.upload(s)
+    ctx.build_bvh()
+    ctx.set_camera(s.camera)
+    ctx.begin(photons, passes, r1, ray_eps=EPS, seed=seed)
+
+
+def test_table_build_is_bit_exact(gpu_ctx):
+    say("proposal table")
+    s = P.scenegen(P.SCENE_BOX, 1, 0, 32, 32)
+    _setup(gpu_ctx, s, 1000, 1, 0.1)
+    rng = np.random.default_rng(11)
+    prob = rng.gamma(0.4, size=(9, 1024)).astype(np.float32)
+    prob[4] = 0.0
+    gpu_ctx.proposal_set_table(prob, 3, 0.5)
+    cdf, side = gpu_ctx.proposal_table()
+    assert side == 3
+    assert np.array_equal(cdf, R.cdf_table(prob))
+
+
+@pytest.fixture(scope="module")
+def box():
+    s = P.scenegen(P.SCENE_BOX, 1, 0, 96, 96)
+    return s, O.OracleScene(s)
+
+
+def test_photon_pass_with_table_matches_oracle(gpu_ctx, box):
+    s, os_ = box
+    say("photon pass with a proposal table vs oracle")
+    N = 30000
+    _setup(gpu_ctx, s, N, 1, 0.0866)
+    rng = np.random.default_rng(12)
+    prob = rng.gamma(0.3, size=(16, 1024)).astype(np.float32)
+    gpu_ctx.proposal_set_table(prob, 4, 0.3)
+    st = gpu_ctx.render_pass()
+    assert st["proposal_active"] == 1
+    gp, gf, gn, gk = gpu_ctx.last_photons()
+    cdf, side = gpu_ctx.proposal_table()
+    op, of, on = os_.trace_photons(N, 1, EPS, 0x7A75, prop=O.proposal(cdf, side, 0.3))
+    ok = os_.last_keys
+    common, gi, oi = np.intersect1d(gk, ok, return_indices=True)
+    assert len(common) > 0.995 * max(len(gk), len(ok)), (len(common), len(gk), len(ok))
+    assert np.mean(np.abs(gp[gi] - op[oi]).max(axis=1) < 1e-3) > 0.995
+    rel = np.abs(gf[gi] - of[oi]) / np.abs(of[oi])
+    assert np.mean(rel.max(axis=1) < 1e-4) > 0.995
+    np.testing.assert_allclose(gf.sum(axis=0), of.sum(axis=0), rtol=5e-3)
+
+
+def test_lambda_one_equals_plain_emission_bit_for_bit(gpu_ctx, box):
+    s, _ = box
+    say("lambda = 1 vs plain")
+    _setup(gpu_ctx, s, 20000, 2, 0.0866)
+    for _ in range(2):
+        gpu_ctx.render_pass()
+    plain = gpu_ctx.image()
+    pk = gpu_ctx.last_photons()
+    _setup(gpu_ctx, s, 20000, 2, 0.0866)
+    gpu_ctx.proposal_set_table(np.random.default_rng(1).random((4, 1024)).astype(np.float32), 2, 1.0)
+    for _ in range(2):
+        assert gpu_ctx.render_pass()["proposal_active"] == 1
+    assert np.array_equal(gpu_ctx.image(), plain)
+    for a, b in zip(gpu_ctx.last_photons(), pk):
+        assert np.array_equal(a, b)
+
+
+def test_contribution_records_match_restatement(gpu_ctx, box):
+    s, _ = box
+    say("contribution records")
+    N = 40000
+    _setup(gpu_ctx, s, N, 1, 0.0866, seed=21)
+    gpu_ctx.proposal_enable(lam=0.5, anchors=4, train_interval=1000, train_steps=1, batch=256,
+                            ring_capacity=1 << 20)
+    st = gpu_ctx.render_pass()
+    rec = gpu_ctx.proposal_records()
+    assert st["proposal_records"] == len(rec) > 0
+    gp, gf, gn, gk = gpu_ctx.last_photons()
+    hp, _, _, _ = gpu_ctx.last_hitpoints()
+    S = st["hash_size"]
+    _, _, _, bmin, scale = gpu_ctx.last_grid(len(gp), S)
+    ph, c = R.contributions(gp, gf, gk, hp, st["radius"], bmin, scale, S)
+    assert len(ph) == len(rec)
+    bad = np.nonzero(rec[:, 3] != c)[0]
+    if len(bad):
+        k = int(bad[0])
+        sel = (gk >> 6) == ph[k]
+        vis = np.zeros(S, bool)
+        vis[R.visited_buckets(hp, st["radius"], bmin, scale, S)] = True
+        b = R.grid_hash(*(R.grid_cells(gp[sel, d].astype(np.float64), bmin[d], scale) for d in range(3)), S)
+        say(f"mismatches {len(bad)}; first photon {ph[k]}: gpu {rec[k, 3]!r} cpu {c[k]!r}; "
+            f"max rel {np.max(np.abs(rec[bad, 3] - c[bad]) / c[bad]):.3e}; deposits {gf[sel].tolist()} "
+            f"buckets {b.tolist()} visited {vis[b].tolist()}")
+    assert len(bad) == 0
+    same_cell = []
+    for k in range(0, len(ph), max(1, len(ph) // 300)):
+        uv, _, _, cell = O.emission(None, N, 1, 21, int(ph[k]), [0.0, -1.0, 0.0])
+        assert rec[k, 0] == np.float32(uv[0]) and rec[k, 1] == np.float32(uv[1])
+        same_cell.append(rec[k, 2] == cell)
+    # the cosine cell is computed in FP32 on the device: a boundary may round either way
+    assert np.mean(same_cell) > 0.99
+
+
+def test_learned_proposal_trains_and_renders(gpu_ctx):
+    say("learned proposal end to end (two rooms)")
+    s = P.scenegen(P.SCENE_TWO_ROOMS, 1, 12, 64, 64)
+    r1 = 0.01 * s.diagonal()
+
+    def run(enable):
+        _setup(gpu_ctx, s, 100000, 8, r1, seed=5)
+        if enable:
+            gpu_ctx.proposal_enable(lam=0.3, anchors=4, train_interval=2, train_steps=16, batch=4096,
+                                    ring_capacity=1 << 20, seed=3)
+        sts = [gpu_ctx.render_pass() for _ in range(8)]
+        return gpu_ctx.image(), sts
+
+    img, sts = run(True)
+    assert [t["retrained"] for t in sts] == [0, 1, 0, 1, 0, 1, 0, 1]
+    assert [t["proposal_active"] for t in sts] == [0, 0, 1, 1, 1, 1, 1, 1]
+    assert all(t["proposal_records"] > 0 for t in sts)
+    assert all(np.isfinite(t["train_loss"]) and 0 < t["train_loss"] < math.log(1024) + 1e-3
+               for t in sts if t["retrained"])
+    cdf, side = gpu_ctx.proposal_table()
+    assert side == 4 and np.all(np.diff(cdf, axis=1) >= 0) and np.all(cdf[:, -1] == 1.0)
+    assert np.all(np.isfinite(img)) and img.min() >= 0.0
+    # deterministic: the same seeds give the same image, training included
+    img2, _ = run(True)
+    assert np.array_equal(img, img2)
+    # unbiased change of emission density: same mean brightness as plain (noise-limited)
+    plain, _ = run(False)
+    assert abs(img.mean() / plain.mean() - 1.0) < 0.15, (img.mean(), plain.mean())
