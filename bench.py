@@ -81,6 +81,105 @@ def cpu_baseline(scene, photons, r1, ray_eps, seed):
     }, img
 
 
+def error_study(P, np, photons, width, passes, ref_passes, lam, probe_every=8):
+    """Time to equal error, plain against learned emission, on the occluded-light
+    scene (two 6^3 rooms joined by a 0.2 x 2 door slit, light in room A, camera
+    and a ~200k-triangle displaced sphere in room B).
+
+    Error = RMSE against the mean of two independent long plain runs, with the
+    references' own noise removed (their half-difference estimates it), relative
+    to the reference mean.  The threshold is plain's error after `passes`
+    passes; the learned run's time to reach it is interpolated between probes.
+    The noise part alone (no bias) comes from two seeds of each method at
+    `passes` passes: variance_ratio = plain / learned, and noise_only_speedup
+    scales it by the per-pass time ratio.
+    """
+    scene = P.scenegen(P.SCENE_TWO_ROOMS, 1, 100, width, width)
+    r1 = 0.01 * scene.diagonal()
+    ctx = P.Context(0)
+    ctx.upload(scene)
+    ctx.build_bvh()
+    ctx.set_camera(scene.camera)
+
+    def run(seed, n, learned, ref=None):
+        ctx.begin(photons, n, r1, seed=seed)
+        if learned:
+            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=64, batch=16384,
+                                ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
+        t, curve, extra = 0.0, [], 0.0
+        for i in range(1, n + 1):
+            t0 = time.perf_counter()
+            st = ctx.render_pass()
+            t += time.perf_counter() - t0
+            extra += st["proposal_ms"]
+            if ref is not None and (i % probe_every == 0 or i == 1):
+                d = ctx.image() - ref
+                curve.append((i, t, float(np.mean(d * d))))
+        return ctx.image(), t, curve, extra / n
+
+    ref1, t_ref, _, _ = run(0xBEE1, ref_passes, False)
+    ref2, _, _, _ = run(0xBEE2, ref_passes, False)
+    ref = 0.5 * (ref1 + ref2)
+    ref_noise = float(np.mean((ref1 - ref2) ** 2)) / 4.0
+    mean_ref = float(ref.mean())
+
+    def rel(mse):
+        return float(np.sqrt(max(mse - ref_noise, 0.0))) / mean_ref
+
+    img_pa, t_plain, c_plain, _ = run(0xA0A, passes, False, ref)
+    img_la, t_learned, c_learned, prop_ms = run(0xA0A, passes, True, ref)
+    # noise alone (no bias): half the mean squared difference of two seeds
+    img_pb, _, _, _ = run(0xB0B, passes, False)
+    img_lb, _, _, _ = run(0xB0B, passes, True)
+    noise_plain = float(np.mean((img_pa - img_pb) ** 2)) / 2.0
+    noise_learned = float(np.mean((img_la - img_lb) ** 2)) / 2.0
+    target = rel(c_plain[-1][2])
+    reach = None
+    prev = None
+    for i, t, m in c_learned:
+        e = rel(m)
+        if e <= target:
+            if prev is None:
+                reach = t
+            else:
+                (t0, e0) = prev
+                reach = t0 + (t - t0) * (e0 - target) / max(e0 - e, 1e-30)
+            break
+        prev = (t, e)
+    learned_at_plain_time = None
+    for (i, t, m) in c_learned:
+        if t >= t_plain:
+            learned_at_plain_time = rel(m)
+            break
+    ctx.close()
+    return {
+        "scene": "two 6^3 rooms, 0.2 x 2 door slit, light in room A, camera + displaced sphere in room B",
+        "triangles": int(scene.n_tris),
+        "resolution": [width, width],
+        "photons_per_pass": photons,
+        "error": "RMSE vs reference / mean(reference), reference noise removed",
+        "reference": f"mean of 2 independent plain GPU runs x {ref_passes} passes (noise estimate subtracted)",
+        "reference_rel_noise": float(np.sqrt(ref_noise)) / mean_ref,
+        "reference_mean_radiance": mean_ref,
+        "abs_rmse_plain_pass1": float(np.sqrt(c_plain[0][2])),
+        "threshold_rel": target,
+        "plain_s": t_plain,
+        "plain_passes": passes,
+        "learned_s": reach,
+        "speedup": (t_plain / reach) if reach else None,
+        "learned_rel_at_plain_time": learned_at_plain_time,
+        "learned_total_s": t_learned,
+        "learned_overhead_ms_per_pass": prop_ms,
+        "lambda": lam,
+        "noise_rel_plain": float(np.sqrt(noise_plain)) / mean_ref,
+        "noise_rel_learned": float(np.sqrt(noise_learned)) / mean_ref,
+        "variance_ratio": noise_plain / max(noise_learned, 1e-300),
+        "noise_only_speedup": (noise_plain / max(noise_learned, 1e-300)) * (t_plain / t_learned),
+        "curve_plain": [(i, round(t, 4), round(rel(m), 5)) for i, t, m in c_plain],
+        "curve_learned": [(i, round(t, 4), round(rel(m), 5)) for i, t, m in c_learned],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +191,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--gather-mode", choices=["auto", "lanes", "tiled", "wave"], default="auto")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the plain-vs-learned time-to-error leg")
+    ap.add_argument("--rmse-passes", type=int, default=256)
+    ap.add_argument("--rmse-ref-passes", type=int, default=1024)
+    ap.add_argument("--rmse-lambda", type=float, default=0.5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -183,6 +286,9 @@ def main():
         out["copy_calibration"] = {"bytes": args.copy_calibration, "ms": copy_ms,
                                    "GBps": 2 * args.copy_calibration / (copy_ms * 1e-3) / 1e9}
     ctx.close()
+    if not args.no_rmse:
+        out["time_to_rmse"] = error_study(P, np, args.photons, args.width, args.rmse_passes,
+                                          args.rmse_ref_passes, args.rmse_lambda)
     if not args.no_cpu:
         cb, _ = cpu_baseline(scene, args.photons, r1, ray_eps, seed)
         out["cpu_baseline"] = cb

@@ -13,6 +13,8 @@ loss_and_grad       contribution-weighted cross entropy and   central finite dif
                     its analytic backward
 adam                Adam (Kingma & Ba 2015), bias-corrected   first step moves by lr * sign(g)
 cdf_table           sequential FP32 inclusive CDF per anchor  monotone, ends at 1, differences = rows
+used_deposits /     deposits some hit point's gather accepts, brute force over a k-d tree (scipy)
+  contributions     per-photon FP32 sums of their mean flux
 """
 import numpy as np
 
@@ -141,44 +143,31 @@ def cdf_table(prob):
     return out
 
 
-def grid_cells(x, mn, scale):
-    """|trunc((x - min) * scale)| in FP64 (sources/hash_grid_detail.h:35-43)."""
-    return np.abs(np.trunc((np.asarray(x, np.float64) - mn) * scale)).astype(np.int64)
+def used_deposits(ph_pos, ph_nrm, hit_pos, hit_nrm, radius):
+    """Deposits that some hit point's gather accepts: |x - p| <= r and
+    n_p . n_x > 0 (the tests of sources/ppm_probabilistic.cc:300-334 in the
+    fixed-radius form), by a k-d tree over the hit points."""
+    from scipy.spatial import cKDTree
+    hp = np.asarray(hit_pos, np.float64)
+    hn = np.asarray(hit_nrm, np.float64)
+    pp = np.asarray(ph_pos, np.float64)
+    pn = np.asarray(ph_nrm, np.float64)
+    used = np.zeros(len(pp), bool)
+    if len(hp) == 0 or len(pp) == 0:
+        return used
+    lists = cKDTree(hp).query_ball_point(pp, radius)
+    for i, lst in enumerate(lists):
+        if lst:
+            used[i] = bool(np.any(hn[lst] @ pn[i] > 0.0))
+    return used
 
 
-def grid_hash(ix, iy, iz, size):
-    h = ((ix.astype(np.uint64) * 73856093) ^ (iy.astype(np.uint64) * 19349663) ^
-         (iz.astype(np.uint64) * 83492791)) & 0xFFFFFFFF
-    return (h % size).astype(np.int64)
-
-
-def visited_buckets(hit_pos, radius, bmin, scale, size):
-    """Buckets the gather of each hit point visits: per axis the cells of
-    [x - r, x + r], at most two (the second only if it differs)."""
-    hp = np.asarray(hit_pos, np.float32).astype(np.float64)
-    lo = np.stack([grid_cells(hp[:, d] - radius, bmin[d], scale) for d in range(3)], 1)
-    hi = np.stack([grid_cells(hp[:, d] + radius, bmin[d], scale) for d in range(3)], 1)
-    hi = np.minimum(hi, lo + 1)
-    out = []
-    for dz in (0, 1):
-        for dy in (0, 1):
-            for dx in (0, 1):
-                ok = (lo[:, 0] + dx <= hi[:, 0]) & (lo[:, 1] + dy <= hi[:, 1]) & (lo[:, 2] + dz <= hi[:, 2])
-                out.append(grid_hash(lo[ok, 0] + dx, lo[ok, 1] + dy, lo[ok, 2] + dz, size))
-    return np.unique(np.concatenate(out)) if out else np.zeros(0, np.int64)
-
-
-def contributions(ph_pos, ph_flux, ph_key, hit_pos, radius, bmin, scale, size):
+def contributions(ph_flux, ph_key, used):
     """Per emitted photon with a positive value: (photon index, c), c = FP32
-    sum in bounce order of mean(flux) over its deposits in visited buckets."""
-    flags = np.zeros(size, bool)
-    flags[visited_buckets(hit_pos, radius, bmin, scale, size)] = True
-    p = np.asarray(ph_pos, np.float32).astype(np.float64)
-    b = grid_hash(grid_cells(p[:, 0], bmin[0], scale), grid_cells(p[:, 1], bmin[1], scale),
-                  grid_cells(p[:, 2], bmin[2], scale), size)
+    sum in bounce order of mean(flux) over its used deposits."""
     f = np.asarray(ph_flux, np.float32)
     val = ((f[:, 0] + f[:, 1]) + f[:, 2]) * np.float32(1.0 / 3.0)
-    val = np.where(flags[b], val, np.float32(0.0)).astype(np.float32)
+    val = np.where(used, val, np.float32(0.0)).astype(np.float32)
     photon = np.asarray(ph_key, np.uint32) >> 6
     order = np.argsort(np.asarray(ph_key, np.uint32), kind="stable")
     out_p, out_c = [], []

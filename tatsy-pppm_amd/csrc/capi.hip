@@ -105,7 +105,7 @@ struct Ctx {
         bool mlp_ready = false;
     } prop;
     DBuf mlp_params, mlp_m, mlp_v, mlp_wb, mlp_part, mlp_act, mlp_idx, mlp_wsum, mlp_loss, mlp_stage,
-        mlp_grad, prop_ring, prop_prob, prop_cdf, prop_uv, prop_flags, prop_segc, prop_scan, prop_bsums;
+        mlp_grad, prop_ring, prop_prob, prop_cdf, prop_uv, prop_flags, prop_used, prop_segc, prop_scan, prop_bsums;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -873,20 +873,28 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ga.img_comp = ctx->img_comp.as<float>();
     ga.out_rgb = nullptr;
     ga.counters = dc;
+    // contribution records need the photons the gather accepts: the lanes
+    // kernel marks them as it goes, the others through a separate pass
+    const bool mark_in_gather = ctx->prop.enabled && n > 0 && ga.mode == GATHER_LANES;
+    ga.used = mark_in_gather ? ctx->prop_used.as<uint32_t>() : nullptr;
+    if (mark_in_gather) launch_zero_u32(s, ctx->prop_used.p, n);
     launch_gather(s, ga);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[6], s));
     if (ctx->prop.enabled && n > 0) {
-        // contribution records: deposits in buckets that this pass's hit points visit
-        launch_zero_u32(s, ctx->prop_flags.p, S);
-        launch_mark_buckets(s, ga, ctx->prop_flags.as<uint32_t>());
+        // contribution records: deposits that this pass's gather accepted
+        if (mark_in_gather) {
+            launch_scatter_used(s, ctx->prop_used.as<uint32_t>(), sorted, n, ctx->prop_flags.as<uint32_t>());
+        } else {
+            launch_zero_u32(s, ctx->prop_flags.p, n);
+            launch_mark_used(s, ga, sorted, ctx->prop_flags.as<uint32_t>());
+        }
         ContribArgs ca;
         ca.ph_pos = ctx->ph_pos.as<float4>();
         ca.ph_flux = ctx->ph_flux.as<float4>();
         ca.perm = perm1;
         ca.n = n;
-        ca.g = g;
-        ca.flags = ctx->prop_flags.as<uint32_t>();
+        ca.used = ctx->prop_flags.as<uint32_t>();
         ca.seg_c = ctx->prop_segc.as<float>();
         ca.scan = ctx->prop_scan.as<uint64_t>();
         ca.prop = pa.prop;
@@ -1122,6 +1130,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     ga.img_comp = nullptr;
     ga.out_rgb = dout;
     ga.counters = dc;
+    ga.used = nullptr;
     launch_gather(s, ga);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -1187,8 +1196,8 @@ int pppm_proposal_enable(void* ctx_raw, const pppm_proposal_params* p) {
     TRY_HIP(ctx, mlp_upload(ctx, w.data()));
     TRY_HIP(ctx, ensure_train(ctx, p->batch));
     const int64_t cap = ctx->cap > 0 ? ctx->cap : 1;
-    const uint32_t Smax = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)cap;
-    TRY_HIP(ctx, ctx->prop_flags.ensure(sizeof(uint32_t) * (size_t)Smax));
+    TRY_HIP(ctx, ctx->prop_flags.ensure(sizeof(uint32_t) * (size_t)cap));
+    TRY_HIP(ctx, ctx->prop_used.ensure(sizeof(uint32_t) * (size_t)cap));
     TRY_HIP(ctx, ctx->prop_segc.ensure(sizeof(float) * (size_t)cap));
     TRY_HIP(ctx, ctx->prop_scan.ensure(sizeof(uint64_t) * (size_t)cap));
     TRY_HIP(ctx, ctx->prop_bsums.ensure(sizeof(uint64_t) * (size_t)(cap / 1024 + cap / (1024 * 512) + 64)));

@@ -184,6 +184,7 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
                     acc.y += wt * F.y;
                     acc.z += wt * F.z;
                     acc.n++;
+                    if (a.used) a.used[j] = 1u;
                 }
             }
         }
@@ -391,23 +392,55 @@ __global__ void k_gather_layout(const float4* __restrict__ pos, const float4* __
     Cc[k] = make_float4(f.z, f.w, 0.0f, 0.0f);
 }
 
-// flags[h] = 1 for every bucket that some hit point's gather visits
-__global__ __launch_bounds__(256) void k_mark_buckets(GatherArgs a, uint32_t* flags) {
+// used[sorted[j]] = 1 for every photon record j (bucket order) that some hit
+// point's gather accepts: inside the radius with agreeing normals, the same
+// tests as the gather kernels
+__global__ __launch_bounds__(256) void k_mark_used(GatherArgs a, const uint32_t* sorted, uint32_t* used) {
     const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const float4 hp = a.hp_pos[i];
+    const v3f x = xyz(hp);
+    const v3f nx = oct_decode(__float_as_uint(hp.w));
+    const float r2 = (float)(a.radius * a.radius);
+    const bool compact = a.mode == GATHER_LANES;
     uint32_t mb[8];
-    const uint32_t vm = own_buckets(a, a.hp_pos[i], mb);
-#pragma unroll
-    for (int c = 0; c < 8; c++)
-        if ((vm >> c) & 1u) flags[mb[c]] = 1u;
+    const uint32_t todo = own_buckets(a, hp, mb);
+    for (int c = 0; c < 8; c++) {
+        if (!((todo >> c) & 1u)) continue;
+        const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
+        for (uint32_t j = b0; j < b1; j++) {
+            const float4 A = a.ph_a[j];
+            if (dist2(x, A) > r2) continue;
+            v3f np;
+            if (compact) {
+                np = oct_direction(__float_as_uint(A.w));
+            } else {
+                const float4 B = a.ph_b[j];
+                np = mk(A.w, B.x, B.y);
+            }
+            if (np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f) used[sorted ? sorted[j] : j] = 1u;
+        }
+    }
+}
+
+// used_order[sorted[j]] = used[j]: bucket order -> origin order
+__global__ __launch_bounds__(256) void k_scatter_used(const uint32_t* used, const uint32_t* sorted, int64_t n,
+                                                      uint32_t* used_order) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) used_order[sorted[j]] = used[j];
 }
 
 }  // namespace
 
-void launch_mark_buckets(hipStream_t s, const GatherArgs& a, uint32_t* flags) {
+void launch_scatter_used(hipStream_t s, const uint32_t* used, const uint32_t* sorted, int64_t n, uint32_t* used_order) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_used, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, used, sorted, n, used_order);
+}
+
+void launch_mark_used(hipStream_t s, const GatherArgs& a, const uint32_t* sorted, uint32_t* used) {
     if (a.n_hit <= 0) return;
-    hipLaunchKernelGGL(k_mark_buckets, dim3((unsigned)((a.n_hit + 255) / 256)), dim3(256), 0, s, a, flags);
+    hipLaunchKernelGGL(k_mark_used, dim3((unsigned)((a.n_hit + 255) / 256)), dim3(256), 0, s, a, sorted, used);
 }
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }

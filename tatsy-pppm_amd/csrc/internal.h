@@ -140,11 +140,14 @@ struct GatherArgs {
     float* img_comp;
     float* out_rgb;   // [pixel id][3] raw estimates (test hook)
     Counters* counters;
+    uint32_t* used;   // lanes kernel: used[j] = 1 for accepted photon records j (may be null)
 };
 void launch_gather(hipStream_t s, const GatherArgs& a);
-// flags[h] = 1 for each bucket h visited by the gather of a's hit points
-// (flags are not cleared here)
-void launch_mark_buckets(hipStream_t s, const GatherArgs& a, uint32_t* flags);
+// used[sorted[j]] = 1 for each photon record j (bucket order) that the gather
+// of a's hit points accepts (used is not cleared here)
+void launch_mark_used(hipStream_t s, const GatherArgs& a, const uint32_t* sorted, uint32_t* used);
+// used_order[sorted[j]] = used[j] for j < n
+void launch_scatter_used(hipStream_t s, const uint32_t* used, const uint32_t* sorted, int64_t n, uint32_t* used_order);
 // Morton keys of the half cells of the hit points; slots whose pixel id is
 // NO_HITPOINT get key 0xFFFFFFFF and sort behind every hit point
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
@@ -231,8 +234,7 @@ struct ContribArgs {
     const float4* ph_flux;
     const uint32_t* perm;    // origin order
     int64_t n;
-    GridParams g;
-    const uint32_t* flags;   // buckets visited by hit points
+    const uint32_t* used;    // per origin position: a hit point's gather accepted it
     float* seg_c;            // [n]
     uint64_t* scan;          // [n]
     PropDev prop;

@@ -6,9 +6,10 @@
 // pppm_device.h, whose pdf is exact, so the estimator stays unbiased.
 //
 //  * Contribution records: after the gather of a pass, every emitted photon
-//    whose deposits fall in buckets visited by the pass's hit points gets a
-//    record (u, v, cell, c): its light-sample coordinates, the direction cell
-//    it left in and the mean flux of those deposits.  Records are written in
+//    with deposits that some hit point's gather accepted (inside the radius,
+//    normals agreeing) gets a record (u, v, cell, c): its light-sample
+//    coordinates, the direction cell it left in and the summed mean flux of
+//    those deposits.  Records are written in
 //    photon order (a prefix scan, no atomics) into a ring.
 //  * MLP: Fourier encoding of (u, v) (32) -> 64 -> 64 -> 64 (ReLU) -> 1024
 //    logits, BF16 weights and activations on v_mfma_f32_16x16x16_bf16 with
@@ -423,8 +424,9 @@ __device__ __forceinline__ uint32_t photon_of(const float4* ph_flux, const uint3
     return __float_as_uint(ph_flux[perm[k]].w) >> 6;
 }
 
-// At the first record of each photon (records in origin order): the mean flux
-// of its deposits in flagged buckets; flag = 1 where that is positive
+// At the first record of each photon (records in origin order): the summed
+// mean flux of its deposits that some hit point's gather accepted; the scan
+// input is 1 where that is positive
 __global__ __launch_bounds__(256) void k_contrib(ContribArgs a) {
 #pragma clang fp contract(off)  // the sum is restated on the CPU bit for bit
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -434,13 +436,9 @@ __global__ __launch_bounds__(256) void k_contrib(ContribArgs a) {
     float c = 0.0f;
     if (first) {
         for (int64_t j = k; j < a.n; j++) {
-            const uint32_t rj = a.perm[j];
-            const float4 f = a.ph_flux[rj];
+            const float4 f = a.ph_flux[a.perm[j]];
             if ((__float_as_uint(f.w) >> 6) != ph) break;
-            const float4 p = a.ph_pos[rj];
-            const uint32_t b = grid_hash(grid_cell(p.x, a.g.bmin[0], a.g.scale), grid_cell(p.y, a.g.bmin[1], a.g.scale),
-                                         grid_cell(p.z, a.g.bmin[2], a.g.scale), a.g.size);
-            if (a.flags[b]) c += ((f.x + f.y) + f.z) * (1.0f / 3.0f);
+            if (a.used[j]) c += ((f.x + f.y) + f.z) * (1.0f / 3.0f);
         }
     }
     a.seg_c[k] = c;

@@ -355,7 +355,7 @@ class Context:
         return out[:len(a[3])], st.as_dict()
 
     # ---- learned emission proposal ----
-    def proposal_enable(self, lam=0.25, anchors=8, train_interval=8, train_steps=64, batch=16384,
+    def proposal_enable(self, lam=0.5, anchors=8, train_interval=8, train_steps=64, batch=16384,
                         ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED):
         p = ProposalParams(lambda_=lam, learning_rate=learning_rate, ring_capacity=ring_capacity,
                            anchors=anchors, train_interval=train_interval, train_steps=train_steps,

@@ -1,8 +1,9 @@
 """GPU parity of the learned emission proposal through the C ABI.
 
-Bars: the CDF tables and contribution records are bit-exact against the CPU
-restatement on the same inputs (the record's cell may differ only where FP32
-and FP64 disagree at a cell boundary); MLP probabilities within 2 % per cell
+Bars: the CDF tables are bit-exact against the CPU restatement on the same
+inputs, and so are the contribution records except where FP32 and FP64
+disagree at the boundary of the radius test, the normal test or a direction
+cell (at most 0.2 % of records, 1 % of cells); MLP probabilities within 2 % per cell
 and gradients within 3 % (relative L2 per layer) of a float64 restatement that
 rounds weights and activations to BF16 as the device does (the device sums in
 FP32 on the matrix cores); photon records with a fixed table as for plain
@@ -122,11 +123,11 @@ def test_training_lowers_the_loss(gpu_ctx):
     assert np.mean(np.argmax(p, axis=1) == cell[:256]) > 0.9
 
 
-def _setup(ctx, s, photons, passes, r1, seed=0x7A75):
+def _setup(ctx, s, photons, passes, r1, seed=0x7A75, mode=P.GATHER_AUTO):
     ctx.upload(s)
     ctx.build_bvh()
     ctx.set_camera(s.camera)
-    ctx.begin(photons, passes, r1, ray_eps=EPS, seed=seed)
+    ctx.begin(photons, passes, r1, ray_eps=EPS, seed=seed, gather_mode=mode)
 
 
 def test_table_build_is_bit_exact(gpu_ctx):
@@ -187,38 +188,41 @@ def test_lambda_one_equals_plain_emission_bit_for_bit(gpu_ctx, box):
         assert np.array_equal(a, b)
 
 
-def test_contribution_records_match_restatement(gpu_ctx, box):
+def _photon_of_records(rec, photons, seed):
+    """Photon index of each record, from its light-sample coordinates."""
+    idx = {}
+    for p in range(photons):
+        uv = O.emission(None, photons, 1, seed, p, [0.0, -1.0, 0.0])[0]
+        idx[(np.float32(uv[0]), np.float32(uv[1]))] = p
+    return np.array([idx[(r[0], r[1])] for r in rec], np.int64)
+
+
+@pytest.mark.parametrize("mode", [P.GATHER_LANES, P.GATHER_TILED])
+def test_contribution_records_match_restatement(gpu_ctx, box, mode):
+    # lanes marks accepted photons inside the gather, the others in a separate pass
     s, _ = box
-    say("contribution records")
+    say(f"contribution records, gather mode {mode}")
     N = 40000
-    _setup(gpu_ctx, s, N, 1, 0.0866, seed=21)
+    _setup(gpu_ctx, s, N, 1, 0.0866, seed=21, mode=mode)
     gpu_ctx.proposal_enable(lam=0.5, anchors=4, train_interval=1000, train_steps=1, batch=256,
                             ring_capacity=1 << 20)
     st = gpu_ctx.render_pass()
     rec = gpu_ctx.proposal_records()
     assert st["proposal_records"] == len(rec) > 0
     gp, gf, gn, gk = gpu_ctx.last_photons()
-    hp, _, _, _ = gpu_ctx.last_hitpoints()
-    S = st["hash_size"]
-    _, _, _, bmin, scale = gpu_ctx.last_grid(len(gp), S)
-    ph, c = R.contributions(gp, gf, gk, hp, st["radius"], bmin, scale, S)
-    assert len(ph) == len(rec)
-    bad = np.nonzero(rec[:, 3] != c)[0]
-    if len(bad):
-        k = int(bad[0])
-        sel = (gk >> 6) == ph[k]
-        vis = np.zeros(S, bool)
-        vis[R.visited_buckets(hp, st["radius"], bmin, scale, S)] = True
-        b = R.grid_hash(*(R.grid_cells(gp[sel, d].astype(np.float64), bmin[d], scale) for d in range(3)), S)
-        say(f"mismatches {len(bad)}; first photon {ph[k]}: gpu {rec[k, 3]!r} cpu {c[k]!r}; "
-            f"max rel {np.max(np.abs(rec[bad, 3] - c[bad]) / c[bad]):.3e}; deposits {gf[sel].tolist()} "
-            f"buckets {b.tolist()} visited {vis[b].tolist()}")
-    assert len(bad) == 0
+    hp, hn, _, _ = gpu_ctx.last_hitpoints()
+    used = R.used_deposits(gp, gn, hp, hn, st["radius"])
+    ph, c = R.contributions(gf, gk, used)
+    # the radius and normal tests can fall differently (FP32 against FP64) only at
+    # their boundaries: the records agree bit for bit elsewhere
+    common, gi, oi = np.intersect1d(_photon_of_records(rec, N, 21), ph, return_indices=True)
+    assert len(common) > 0.998 * max(len(ph), len(rec)), (len(common), len(ph), len(rec))
+    assert np.mean(rec[gi, 3] == c[oi]) > 0.998
     same_cell = []
-    for k in range(0, len(ph), max(1, len(ph) // 300)):
-        uv, _, _, cell = O.emission(None, N, 1, 21, int(ph[k]), [0.0, -1.0, 0.0])
-        assert rec[k, 0] == np.float32(uv[0]) and rec[k, 1] == np.float32(uv[1])
-        same_cell.append(rec[k, 2] == cell)
+    for t in range(0, len(common), max(1, len(common) // 300)):
+        uv, _, _, cell = O.emission(None, N, 1, 21, int(common[t]), [0.0, -1.0, 0.0])
+        assert rec[gi[t], 0] == np.float32(uv[0]) and rec[gi[t], 1] == np.float32(uv[1])
+        same_cell.append(rec[gi[t], 2] == cell)
     # the cosine cell is computed in FP32 on the device: a boundary may round either way
     assert np.mean(same_cell) > 0.99
 
