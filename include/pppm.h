@@ -136,6 +136,12 @@ typedef struct {
     int64_t proposal_records;  /* records written this pass */
     int32_t proposal_active;   /* photons of this pass used the learned mixture */
     int32_t retrained;         /* the MLP was retrained after this pass */
+    /* subsurface scattering (zero without BSSRDF materials) */
+    double irradiance_ms;      /* irradiance photons: order, grid, gather at the surface points */
+    double dipole_ms;          /* dipole diffusion sum of the transmitted eye paths */
+    int64_t irradiance_photons;/* photons stored on BSSRDF surfaces */
+    int64_t sss_hits;          /* eye paths ending in the dipole estimate */
+    int64_t sss_points;        /* surface points of the irradiance cache */
 } pppm_pass_stats;
 
 /* ---- context ---- */
@@ -209,6 +215,33 @@ int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const floa
  * reference counterpart): the known-traffic kernel that scales profiler
  * FETCH_SIZE / WRITE_SIZE counters, and a measured copy bandwidth. */
 int pppm_stream_copy(void* ctx, int64_t bytes, double* ms);
+
+/* ---- subsurface scattering: replaces SubsurfaceIntegrator (sources/
+ * subsurface_integrator.cc:180-365, called per pass from ppm_probabilistic.cc:
+ * 83-85 and 272-296) with DipoleBSSRDF (sources/bssrdf.cc:33-104).  A BSSRDF
+ * material (PPPM_BSDF_REFRACTION | PPPM_BSDF_BSSRDF, sigma_a, sigmap_s, eta)
+ * gets fixed surface points (each standing for an equal share of the area);
+ * every pass stores the photons that reach those surfaces, estimates the
+ * irradiance E_j at the points with the fixed-radius gather, and adds
+ * (1 - F_dr) / pi * sum_j R_d(|x - x_j|^2) E_j A_j for eye paths transmitted
+ * into the surface.  The exponential is the true one, not the reference's
+ * fast_exp (sources/vector3d.cc:11-16). ---- */
+/* Surface points of later renders (0: the reference's density, at most 2^20);
+ * takes effect at pppm_render_begin. */
+int pppm_subsurface_configure(void* ctx, int64_t points);
+/* Points of the current render [n][3], their normals, and the area per point. */
+int pppm_subsurface_points(void* ctx, float* pos, float* normal, int64_t cap, int64_t* n, double* area);
+/* Irradiance E_j [n][3] of the last pass. */
+int pppm_subsurface_irradiance(void* ctx, float* E, int64_t cap, int64_t* n);
+/* Photons stored on BSSRDF surfaces in the last pass (append order). */
+int pppm_last_irradiance_photons(void* ctx, float* pos, float* flux, float* normal, uint32_t* key, int64_t cap,
+                                 int64_t* n);
+/* The dipole sum alone (DipoleBSSRDF::operator() of sources/bssrdf.cc:92-104
+ * summed as in subsurface_integrator.cc:147-167): out [m][3] =
+ * (1 - F_dr) / pi * area * sum_j R_d(|q - p_j|^2) E_j for the material's
+ * sigma_a, sigmap_s and eta. */
+int pppm_dipole_eval(void* ctx, const pppm_bsdf* material, const float* pts, const float* E, int64_t n_pts,
+                     double area, const float* query, int64_t n_query, float* out);
 
 /* ---- learned emission proposal.  No reference counterpart: it replaces the
  * cosine emission of sources/sampler.cc:17-32 (the light contract of

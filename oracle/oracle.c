@@ -22,6 +22,9 @@
  * orc_eye_pass                  sources/ppm_probabilistic.cc:253-387       mirror-chain / direct-emission closed form
  * orc_gather                    sources/ppm_probabilistic.cc:343-377       brute force; uniform-density closed form
  * orc_render                    sources/ppm_probabilistic.cc:67-164        furnace-style energy check; convergence
+ * orc_dipole / orc_dipole_sum   sources/bssrdf.cc:33-104;                  compiled reference DipoleBSSRDF (_ref golden,
+ *                               subsurface_integrator.cc:147-167, 234-238  fast exp); closed forms of R_d at d = 0 and
+ *                                                                          of its clamped area integral
  * emit_direction /              no reference counterpart (the learned      uniform-table and one-hot-table closed
  *   orc_trace_photons_prop        mixture replacing sampler.cc:17-32)        forms; lambda = 1 equals on_hemisphere;
  *                                                                          E[factor] = 1 (unbiasedness)
@@ -1080,4 +1083,64 @@ void orc_emission(const orc_proposal* pr, const orc_pass_params* P, int64_t idx,
     out_dir[1] = d.y;
     out_dir[2] = d.z;
     *cell = c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dipole BSSRDF (sources/bssrdf.cc:33-104) and its surface sum              */
+/* (sources/subsurface_integrator.cc:147-167, 234-238)                      */
+/* ------------------------------------------------------------------------ */
+
+/* Schraudolph's exponential as the reference writes it (vector3d.cc:11-16):
+ * only the high word of the double is set */
+static double schraudolph_exp(double y) {
+    int32_t hi = (int32_t)(1512775 * y + 1072632447);
+    uint64_t bits = ((uint64_t)(uint32_t)hi) << 32;
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    return d;
+}
+
+double orc_dipole_fdr(double eta) {
+    double e2 = eta * eta, e3 = e2 * eta;
+    if (eta >= 1.0) return (-1.439 + 0.7099 * eta + 0.6681 * e2 + 0.0636 * e3) / e2;
+    return (-0.4399 * e3 + 0.7099 * e2 - 0.3319 * eta + 0.0636) / e3;
+}
+
+/* R_d(d2) per channel, clamped at 0; fast = 1 uses the reference's exp */
+void orc_dipole(const double sigma_a[3], const double sigmap_s[3], double eta, double d2, int32_t fast,
+                double out[3]) {
+    double fdr = orc_dipole_fdr(eta);
+    double A = (1.0 + fdr) / (1.0 - fdr);
+    for (int c = 0; c < 3; c++) {
+        double st = sigma_a[c] + sigmap_s[c];
+        double tr = sqrt(3.0 * sigma_a[c] * st);
+        double ap = sigmap_s[c] / st;
+        double zp = 1.0 / st;
+        double zn = zp * (1.0 + (4.0 / 3.0) * A);
+        double dp = sqrt(d2 + zp * zp), dn = sqrt(d2 + zn * zn);
+        double ep = fast ? schraudolph_exp(-tr * dp) : exp(-tr * dp);
+        double en = fast ? schraudolph_exp(-tr * dn) : exp(-tr * dn);
+        double pos = zp * (dp * tr + 1.0) * ep;
+        double neg = zn * (dn * tr + 1.0) * en;
+        double r = (ap * (pos * dn * dn * dn - neg * dp * dp * dp)) / ((4.0 * PI) * tr * dp * dp * dp * dn * dn * dn);
+        out[c] = r > 0.0 ? r : 0.0;
+    }
+}
+
+/* out[i] = (1 - F_dr) / pi * area * sum_j R_d(|q_i - p_j|^2) E_j, FP64 brute force */
+void orc_dipole_sum(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
+                    const float* E, int64_t n_pts, double area, const float* q, int64_t n_q, double* out) {
+    double f = (1.0 - orc_dipole_fdr(eta)) / PI * area;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n_q; i++) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int64_t j = 0; j < n_pts; j++) {
+            double dx = (double)q[3 * i] - pts[3 * j], dy = (double)q[3 * i + 1] - pts[3 * j + 1];
+            double dz = (double)q[3 * i + 2] - pts[3 * j + 2];
+            double r[3];
+            orc_dipole(sigma_a, sigmap_s, eta, dx * dx + dy * dy + dz * dz, 0, r);
+            for (int c = 0; c < 3; c++) acc[c] += r[c] * (double)E[3 * j + c];
+        }
+        for (int c = 0; c < 3; c++) out[3 * i + c] = f * acc[c];
+    }
 }

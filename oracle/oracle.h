@@ -135,6 +135,15 @@ int64_t orc_trace_photons_prop(const orc_scene* s, const orc_pass_params* p, con
 void orc_emission(const orc_proposal* pr, const orc_pass_params* p, int64_t idx, const double nl[3],
                   double out_uv[2], double out_dir[3], double* weight, uint32_t* cell);
 
+/* Dipole BSSRDF (sources/bssrdf.cc:33-104): F_dr, R_d(d2) per channel
+ * (clamped at 0; fast = 1: the reference's Schraudolph exp), and the surface
+ * sum (1 - F_dr) / pi * area * sum_j R_d(|q_i - p_j|^2) E_j in FP64. */
+double orc_dipole_fdr(double eta);
+void orc_dipole(const double sigma_a[3], const double sigmap_s[3], double eta, double d2, int32_t fast,
+                double out[3]);
+void orc_dipole_sum(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
+                    const float* E, int64_t n_pts, double area, const float* q, int64_t n_q, double* out);
+
 /* Eye pass: one path per pixel.  For each pixel writes either a hit point
  * (has_hit=1: pos, oriented normal, weight = throughput * rho / pi) or a
  * direct contribution (emission seen through specular chains). */

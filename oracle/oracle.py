@@ -53,6 +53,9 @@ _ODECL = {
     "orc_trace_photons_prop": (C.c_int64, [_P, C.POINTER(PassParams), C.POINTER(Proposal), _P, _P, _P, _P,
                                            C.c_int64]),
     "orc_emission": (None, [C.POINTER(Proposal), C.POINTER(PassParams), C.c_int64, _P, _P, _P, _P, _P]),
+    "orc_dipole_fdr": (C.c_double, [C.c_double]),
+    "orc_dipole": (None, [_P, _P, C.c_double, C.c_double, C.c_int32, _P]),
+    "orc_dipole_sum": (None, [_P, _P, C.c_double, _P, _P, C.c_int64, C.c_double, _P, C.c_int64, _P]),
     "orc_eye_pass": (None, [_P, _P, C.POINTER(PassParams), _P, _P, _P, _P, _P]),
     "orc_gather": (None, [_P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double, C.c_int32, _P, _P, _P]),
     "orc_render": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
@@ -249,3 +252,22 @@ def emission(prop, photons, pass_index, seed, idx, normal, scene_id=0):
     orc().orc_emission(C.byref(prop) if prop is not None else None, C.byref(p), idx, _ptr(nl), _ptr(uv),
                        _ptr(d), C.byref(w), C.byref(c))
     return uv, d, w.value, c.value
+
+
+def dipole(sigma_a, sigmap_s, eta, d2, fast=False):
+    sa = np.ascontiguousarray(sigma_a, np.float64)
+    ss = np.ascontiguousarray(sigmap_s, np.float64)
+    out = np.zeros(3)
+    orc().orc_dipole(_ptr(sa), _ptr(ss), eta, d2, 1 if fast else 0, _ptr(out))
+    return out
+
+
+def dipole_sum(sigma_a, sigmap_s, eta, pts, E, area, query):
+    sa = np.ascontiguousarray(sigma_a, np.float64)
+    ss = np.ascontiguousarray(sigmap_s, np.float64)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(E, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(query, np.float32).reshape(-1, 3)
+    out = np.zeros((len(q), 3))
+    orc().orc_dipole_sum(_ptr(sa), _ptr(ss), eta, _ptr(p), _ptr(e), len(p), area, _ptr(q), len(q), _ptr(out))
+    return out

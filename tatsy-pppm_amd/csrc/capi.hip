@@ -90,7 +90,16 @@ struct Ctx {
     double last_scale = 0.0;
     uint32_t* last_perm1 = nullptr;
     uint32_t* last_sorted = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[11] = {};
+    // subsurface scattering (BSSRDF materials)
+    DBuf dips, sss_pts, sss_ptw, sss_E, irr_pos, irr_flux, sss_pos, sss_w;
+    std::vector<int32_t> h_sss_index;  // material -> dipole index (-1: none)
+    int64_t sss_points_req = 0;        // 0: the default count
+    int64_t sss_n = 0;                 // surface points of the current render
+    double sss_area = 0.0;             // area per point
+    bool sss_present = false;
+    int32_t n_dips = 0;
+    int64_t last_n_irr = 0;
     // learned emission proposal
     struct Proposal {
         bool enabled = false;  // records + retraining
@@ -141,6 +150,25 @@ int bits_for(uint64_t max_value) {
     int b = 0;
     while (b < 64 && (max_value >> b) != 0) b++;
     return b;
+}
+
+// host copy of the device octahedral encode (pppm_device.h), returned as the
+// float whose bits hold the code
+float oct_encode_host(v3f n) {
+    const float s = std::fabs(n.x) + std::fabs(n.y) + std::fabs(n.z);
+    float x = n.x / s, y = n.y / s;
+    if (n.z < 0.0f) {
+        const float ox = (1.0f - std::fabs(y)) * (x < 0.0f ? -1.0f : 1.0f);
+        const float oy = (1.0f - std::fabs(x)) * (y < 0.0f ? -1.0f : 1.0f);
+        x = ox;
+        y = oy;
+    }
+    const int qx = (int)std::rint(std::fmin(std::fmax(x, -1.0f), 1.0f) * 32767.0f);
+    const int qy = (int)std::rint(std::fmin(std::fmax(y, -1.0f), 1.0f) * 32767.0f);
+    const uint32_t e = ((uint32_t)(qx & 0xFFFF)) | ((uint32_t)(qy & 0xFFFF) << 16);
+    float f;
+    std::memcpy(&f, &e, 4);
+    return f;
 }
 
 // host copy of the device octahedral decode (pppm_device.h)
@@ -202,6 +230,138 @@ hipError_t ensure_hit_sort(Ctx* c, int64_t n) {
     for (auto& b : c->hk)
         if ((e = b.ensure(sizeof(uint32_t) * m)) != hipSuccess) return e;
     return c->hhist.ensure(sizeof(uint32_t) * (size_t)radix_hist_words((int64_t)m));
+}
+
+// ---------------------------------------------------------------------------
+// Subsurface scattering: host side
+// ---------------------------------------------------------------------------
+// F_dr, the rational fit of sources/bssrdf.cc:33-42
+double dipole_fdr(double eta) {
+    const double e2 = eta * eta, e3 = e2 * eta;
+    if (eta >= 1.0) return (-1.439 + 0.7099 * eta + 0.6681 * e2 + 0.0636 * e3) / e2;
+    return (-0.4399 * e3 + 0.7099 * e2 - 0.3319 * eta + 0.0636) / e3;
+}
+
+uint32_t morton_bits(uint32_t v) {  // 10 bits spread 3 apart
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Surface points on the BSSRDF triangles (the reference draws a Poisson disk
+// every pass, sources/sampler.cc:36-101, with dA = pi (r / 2)^2): n points
+// placed once by systematic sampling of the area CDF, uniform barycentrics
+// inside each triangle, so every point stands for area A / n.  The default
+// count gives the reference's dA (r = sqrt(sum of BSSRDF areas / all
+// triangles), ppm_probabilistic.cc:37-46), at most 2^20.  Points are stored in
+// Morton order.
+uint64_t splitmix64(uint64_t* st);
+
+hipError_t subsurface_setup(Ctx* c, const float* tri_h /* may be null: read back */) {
+    c->sss_present = false;
+    c->sss_n = 0;
+    if (c->n_dips == 0) return hipSuccess;
+    std::vector<float> tri((size_t)c->n_tris * 9);
+    std::vector<int32_t> mat((size_t)c->n_tris);
+    hipError_t e;
+    if (tri_h) std::memcpy(tri.data(), tri_h, sizeof(float) * tri.size());
+    else if ((e = hipMemcpy(tri.data(), c->tri9.p, sizeof(float) * tri.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpy(mat.data(), c->tri_mat.p, sizeof(int32_t) * mat.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+        return e;
+    std::vector<int64_t> ids;
+    std::vector<double> cdf;
+    double total = 0.0;
+    for (int64_t t = 0; t < c->n_tris; t++) {
+        if (c->h_sss_index[(size_t)mat[(size_t)t]] < 0) continue;
+        const float* v = &tri[(size_t)t * 9];
+        const double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const double e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        double cr[3];
+        cross3(e1, e2, cr);
+        const double a = 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        if (!(a > 0.0)) continue;
+        total += a;
+        ids.push_back(t);
+        cdf.push_back(total);
+    }
+    if (ids.empty()) return hipSuccess;
+    int64_t n = c->sss_points_req;
+    if (n <= 0) {
+        const double r = std::sqrt(total / (double)c->n_tris);
+        const double dA = M_PI * 0.25 * r * r;
+        n = std::min<int64_t>((int64_t)std::ceil(total / dA), 1LL << 20);
+    }
+    std::vector<float4> pts((size_t)n);
+    uint64_t st = 0x55D1CE5EEDull;
+    size_t cur = 0;
+    for (int64_t k = 0; k < n; k++) {
+        const double target = ((double)k + 0.5) / (double)n * total;
+        while (cur + 1 < cdf.size() && cdf[cur] < target) cur++;
+        const float* v = &tri[(size_t)ids[cur] * 9];
+        double u1 = (double)(splitmix64(&st) >> 11) * (1.0 / 9007199254740992.0);
+        double u2 = (double)(splitmix64(&st) >> 11) * (1.0 / 9007199254740992.0);
+        if (u1 + u2 >= 1.0) {
+            u1 = 1.0 - u1;
+            u2 = 1.0 - u2;
+        }
+        double p[3];
+        for (int d = 0; d < 3; d++) p[d] = v[d] + u1 * (v[3 + d] - v[d]) + u2 * (v[6 + d] - v[d]);
+        const double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const double e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        double nn[3];
+        cross3(e1, e2, nn);
+        const double ln = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+        const v3f nf = mk((float)(nn[0] / ln), (float)(nn[1] / ln), (float)(nn[2] / ln));
+        pts[(size_t)k] = make_float4((float)p[0], (float)p[1], (float)p[2], oct_encode_host(nf));
+    }
+    // Morton order over the bounding box of the points
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (const float4& q : pts) {
+        const float xs[3] = {q.x, q.y, q.z};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], xs[d]);
+            hi[d] = std::max(hi[d], xs[d]);
+        }
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> key((size_t)n);
+    for (int64_t k = 0; k < n; k++) {
+        const float xs[3] = {pts[(size_t)k].x, pts[(size_t)k].y, pts[(size_t)k].z};
+        uint32_t code = 0;
+        for (int d = 0; d < 3; d++) {
+            const float ext = hi[d] - lo[d];
+            const uint32_t q = ext > 0.0f ? (uint32_t)std::min(1023.0f, (xs[d] - lo[d]) / ext * 1024.0f) : 0u;
+            code |= morton_bits(q) << d;
+        }
+        key[(size_t)k] = {code, (uint32_t)k};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint32_t, uint32_t>& x, const std::pair<uint32_t, uint32_t>& y) {
+                         return x.first < y.first;
+                     });
+    std::vector<float4> sorted((size_t)n);
+    for (int64_t k = 0; k < n; k++) sorted[(size_t)k] = pts[(size_t)key[(size_t)k].second];
+    std::vector<float4> wts((size_t)n);
+    for (int64_t k = 0; k < n; k++) {
+        const uint32_t id = (uint32_t)k;
+        float idf;
+        std::memcpy(&idf, &id, 4);
+        wts[(size_t)k] = make_float4((float)(1.0 / M_PI), (float)(1.0 / M_PI), (float)(1.0 / M_PI), idf);
+    }
+    if ((e = c->sss_ptw.ensure(sizeof(float4) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->sss_ptw.p, wts.data(), sizeof(float4) * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    if ((e = c->sss_pts.ensure(sizeof(float4) * (size_t)n)) != hipSuccess) return e;
+    if ((e = c->sss_E.ensure(sizeof(float) * 3 * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->sss_pts.p, sorted.data(), sizeof(float4) * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    c->sss_n = n;
+    c->sss_area = total / (double)n;
+    c->sss_present = true;
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------
@@ -351,6 +511,48 @@ hipError_t proposal_retrain(Ctx* c, double* loss) {
     return hipSuccess;
 }
 
+// E_j of the surface points from the n_irr irradiance photons of this pass:
+// ordered by origin key, a hash grid at the pass radius, the compact gather
+// layout (in ph_a / ph_b, rewritten later by the main photons), then the
+// lanes gather with weight 1 / pi writing E [point][3]
+hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius) {
+    hipStream_t s = c->stream;
+    if (n_irr <= 0) {
+        launch_zero_u32(s, c->sss_E.p, 3 * c->sss_n);
+        return hipGetLastError();
+    }
+    uint32_t* u0 = c->u[0].as<uint32_t>();
+    uint32_t* u1 = c->u[1].as<uint32_t>();
+    launch_origin_keys(s, c->irr_flux.as<float4>(), n_irr, u0, u1);
+    const int obits = bits_for(((uint64_t)(c->rp.photons - 1) << 6) | 63u);
+    const bool alt = radix_sort_pairs(s, u0, u1, c->u[2].as<uint32_t>(), c->u[3].as<uint32_t>(),
+                                      c->hist.as<uint32_t>(), n_irr, obits);
+    const int keep = alt ? 3 : 1;
+    const uint32_t* perm = c->u[keep].as<uint32_t>();
+    GridParams g;
+    for (int d = 0; d < 3; d++) g.bmin[d] = c->grid_bmin[d];
+    g.scale = 1.0 / (2.0 * radius);
+    g.size = c->rp.hash_size > 0 ? (uint32_t)c->rp.hash_size : (uint32_t)n_irr;
+    const uint32_t* sorted = build_grid(c, c->irr_pos.as<float4>(), nullptr, perm, n_irr, g, keep);
+    launch_gather_layout(s, c->irr_pos.as<float4>(), c->irr_flux.as<float4>(), perm, sorted, n_irr,
+                         c->ph_a.as<float4>(), c->ph_b.as<float4>(), nullptr);
+    GatherArgs ga;
+    std::memset(&ga, 0, sizeof(ga));
+    ga.mode = GATHER_LANES;
+    ga.ph_a = c->ph_a.as<float4>();
+    ga.ph_b = c->ph_b.as<float4>();
+    ga.off = c->off.as<uint32_t>();
+    ga.hp_pos = c->sss_pts.as<float4>();
+    ga.hp_w = c->sss_ptw.as<float4>();
+    ga.hp_count_src = nullptr;
+    ga.n_hit = c->sss_n;
+    ga.g = g;
+    ga.radius = radius;
+    ga.out_rgb = c->sss_E.as<float>();
+    launch_gather(s, ga);
+    return hipGetLastError();
+}
+
 // The hit points among the cap slots of (hp, hw), compacted into hp_pos_s /
 // hp_w_s in Morton order of their half cells (edge r); a stable sort, so the
 // order is a function of the slot order alone.
@@ -449,6 +651,11 @@ int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_i
     for (int32_t i = 0; i < nb; i++) {
         int t = table[i].type;
         if (t & PPPM_BSDF_PHONG) return ctx->fail(PPPM_ERR_INVALID, "Phong BRDF is not supported");
+        if ((t & PPPM_BSDF_BSSRDF) &&
+            (!(t & PPPM_BSDF_REFRACTION) || !(table[i].sigma_a[0] > 0.0f && table[i].sigma_a[1] > 0.0f &&
+                                              table[i].sigma_a[2] > 0.0f && table[i].sigmap_s[0] > 0.0f &&
+                                              table[i].sigmap_s[1] > 0.0f && table[i].sigmap_s[2] > 0.0f)))
+            return ctx->fail(PPPM_ERR_INVALID, "a BSSRDF needs the refraction boundary and positive sigma_a, sigma'_s");
         if (!(t & (PPPM_BSDF_LAMBERTIAN | PPPM_BSDF_SPECULAR | PPPM_BSDF_REFRACTION | PPPM_BSDF_EMITTER)))
             return ctx->fail(PPPM_ERR_INVALID, "bsdf " + std::to_string(i) + " has no supported type");
     }
@@ -480,6 +687,37 @@ int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_i
         m[i].rr_mean = (float)(((double)b.reflectance[0] + (double)b.reflectance[1] + (double)b.reflectance[2]) / 3.0);
         m[i].rr_max = std::fmax(b.reflectance[0], std::fmax(b.reflectance[1], b.reflectance[2]));
         m[i].ior = b.ior > 0.0f ? b.ior : 1.5f;
+        m[i].sss = -1;
+    }
+    // dipole constants of the BSSRDF materials (sources/bssrdf.cc:33-62, FP64)
+    std::vector<DipoleDev> dv;
+    ctx->h_sss_index.assign((size_t)nb, -1);
+    for (int32_t i = 0; i < nb; i++) {
+        const pppm_bsdf& b = table[i];
+        if (!(b.type & PPPM_BSDF_BSSRDF)) continue;
+        const double eta = b.eta > 0.0f ? b.eta : 1.3;
+        DipoleDev d;
+        std::memset(&d, 0, sizeof(d));
+        const double fdr = dipole_fdr(eta);
+        const double A = (1.0 + fdr) / (1.0 - fdr);
+        for (int k = 0; k < 3; k++) {
+            const double sa = b.sigma_a[k], ss = b.sigmap_s[k];
+            const double st = sa + ss;
+            const double tr = std::sqrt(3.0 * sa * st);
+            d.zp[k] = (float)(1.0 / st);
+            d.zn[k] = (float)((1.0 / st) * (1.0 + (4.0 / 3.0) * A));
+            d.tr[k] = (float)tr;
+            d.k[k] = (float)((ss / st) / (4.0 * M_PI * tr));
+        }
+        d.out = (float)((1.0 - fdr) / M_PI);
+        m[i].sss = (int32_t)dv.size();
+        ctx->h_sss_index[(size_t)i] = m[i].sss;
+        dv.push_back(d);
+    }
+    ctx->n_dips = (int32_t)dv.size();
+    if (!dv.empty()) {
+        TRY_HIP(ctx, ctx->dips.ensure(sizeof(DipoleDev) * dv.size()));
+        TRY_HIP(ctx, hipMemcpy(ctx->dips.p, dv.data(), sizeof(DipoleDev) * dv.size(), hipMemcpyHostToDevice));
     }
     TRY_HIP(ctx, ctx->mats.ensure(sizeof(Mat) * (size_t)nb));
     TRY_HIP(ctx, hipMemcpy(ctx->mats.p, m.data(), sizeof(Mat) * (size_t)nb, hipMemcpyHostToDevice));
@@ -737,6 +975,13 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     launch_zero_u32(ctx->stream, ctx->img_comp.p, 3 * npix);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    TRY_HIP(ctx, subsurface_setup(ctx, nullptr));
+    if (ctx->sss_present) {
+        TRY_HIP(ctx, ctx->irr_pos.ensure(sizeof(float4) * (size_t)cap));
+        TRY_HIP(ctx, ctx->irr_flux.ensure(sizeof(float4) * (size_t)cap));
+        TRY_HIP(ctx, ctx->sss_pos.ensure(sizeof(float4) * (size_t)npix));
+        TRY_HIP(ctx, ctx->sss_w.ensure(sizeof(float4) * (size_t)npix));
+    }
     ctx->passes_done = 0;
     ctx->rendering = true;
     ctx->last_valid = false;
@@ -776,6 +1021,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     pa.prop.cdf = prop_active ? ctx->prop_cdf.as<float>() : nullptr;
     pa.prop.lambda = prop_active ? ctx->prop.lambda : 1.0f;
     pa.prop.side = prop_active ? ctx->prop.side : 0;
+    pa.irr_pos = ctx->sss_present ? ctx->irr_pos.as<float4>() : nullptr;
+    pa.irr_flux = ctx->sss_present ? ctx->irr_flux.as<float4>() : nullptr;
+    pa.irr_cap = ctx->sss_present ? ctx->cap : 0;
     launch_photon_pass(s, pa, grid_limit);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -793,6 +1041,18 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     if (n > ctx->cap)
         return ctx->fail(PPPM_ERR_CAPACITY, "photon buffer holds " + std::to_string(ctx->cap) +
                                                 " records, the pass stored " + std::to_string(n));
+    const int64_t n_irr = (int64_t)hc.irr_stored;
+    if (n_irr > ctx->cap)
+        return ctx->fail(PPPM_ERR_CAPACITY, "irradiance photon buffer holds " + std::to_string(ctx->cap) +
+                                                " records, the pass stored " + std::to_string(n_irr));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[8], s));
+    if (ctx->sss_present) {
+        // irradiance at the surface points: the fixed-radius gather over the
+        // photons stored on BSSRDF surfaces, with weight 1 / pi (the E_j of
+        // sources/subsurface_integrator.cc:329-365)
+        TRY_HIP(ctx, subsurface_irradiance(ctx, n_irr, radius));
+    }
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[9], s));
     // deterministic photon order: stable sort by (photon index, bounce)
     uint32_t* u0 = ctx->u[0].as<uint32_t>();
     uint32_t* u1 = ctx->u[1].as<uint32_t>();
@@ -851,6 +1111,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ea.img_sum = ctx->img_sum.as<float>();
     ea.img_comp = ctx->img_comp.as<float>();
     ea.counters = dc;
+    ea.sss_pos = ctx->sss_present ? ctx->sss_pos.as<float4>() : nullptr;
+    ea.sss_w = ctx->sss_present ? ctx->sss_w.as<float4>() : nullptr;
+    ea.sss_cap = ctx->sss_present ? npix : 0;
     launch_eye_pass(s, ea, grid_limit);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s));
@@ -909,6 +1172,25 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         TRY_HIP(ctx, hipGetLastError());
     }
     TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
+    if (ctx->sss_present) {
+        // eye paths that entered a translucent surface: the dipole sum
+        DipoleArgs da;
+        da.pts = ctx->sss_pts.as<float4>();
+        da.E = ctx->sss_E.as<float>();
+        da.n_pts = ctx->sss_n;
+        da.area = (float)ctx->sss_area;
+        da.dip = ctx->dips.as<DipoleDev>();
+        da.hit_pos = ctx->sss_pos.as<float4>();
+        da.hit_w = ctx->sss_w.as<float4>();
+        da.count_src = dc;
+        da.n_hit = npix;
+        da.img_sum = ctx->img_sum.as<float>();
+        da.img_comp = ctx->img_comp.as<float>();
+        da.out_rgb = nullptr;
+        launch_dipole(s, da);
+        TRY_HIP(ctx, hipGetLastError());
+    }
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[10], s));
     TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, s));
     TRY_HIP(ctx, hipStreamSynchronize(s));
     if (hc.stack_overflow) return ctx->fail(PPPM_ERR_HIP, "traversal stack overflow");
@@ -929,6 +1211,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ctx->passes_done = pass;
     ctx->last_valid = true;
     ctx->last_n = n;
+    ctx->last_n_irr = n_irr;
     ctx->last_S = (int32_t)S;
     ctx->last_scale = g.scale;
     ctx->last_perm1 = perm1;
@@ -952,6 +1235,17 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         st->hitsort_ms = t[4];
         st->gather_ms = t[5];
         st->total_ms = tot;
+        float t_irr = 0.0f, t_dip = 0.0f, t_end = 0.0f;
+        (void)hipEventElapsedTime(&t_irr, ctx->ev[8], ctx->ev[9]);
+        (void)hipEventElapsedTime(&t_dip, ctx->ev[7], ctx->ev[10]);
+        (void)hipEventElapsedTime(&t_end, ctx->ev[0], ctx->ev[10]);
+        st->order_ms = t[1] - t_irr;
+        st->irradiance_ms = t_irr;
+        st->dipole_ms = t_dip;
+        st->total_ms = t_end;
+        st->irradiance_photons = n_irr;
+        st->sss_hits = (int64_t)hc.sss_hits;
+        st->sss_points = ctx->sss_n;
         float tp = 0.0f;
         (void)hipEventElapsedTime(&tp, ctx->ev[6], ctx->ev[7]);
         st->proposal_ms = ctx->prop.enabled ? (double)tp + train_ms : 0.0;
@@ -1179,6 +1473,157 @@ int pppm_stream_copy(void* ctx_raw, int64_t bytes, double* ms) {
     (void)hipFree(b);
     if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
     if (ms) *ms = t;
+    return PPPM_OK;
+}
+
+int pppm_subsurface_configure(void* ctx_raw, int64_t points) {
+    CHECK_CTX(ctx);
+    if (points < 0 || points > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "points: 0 (default) .. 2^26");
+    ctx->sss_points_req = points;
+    ctx->rendering = false;
+    return PPPM_OK;
+}
+
+int pppm_subsurface_points(void* ctx_raw, float* pos, float* normal, int64_t cap, int64_t* n, double* area) {
+    CHECK_CTX(ctx);
+    if (!ctx->rendering || !ctx->sss_present) return ctx->fail(PPPM_ERR_STATE, "no subsurface points");
+    if (n) *n = ctx->sss_n;
+    if (area) *area = ctx->sss_area;
+    const int64_t m = std::min(cap, ctx->sss_n);
+    if (m <= 0) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    std::vector<float4> p((size_t)ctx->sss_n);
+    TRY_HIP(ctx, hipMemcpy(p.data(), ctx->sss_pts.p, sizeof(float4) * p.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; i++) {
+        if (pos) {
+            pos[3 * i] = p[(size_t)i].x;
+            pos[3 * i + 1] = p[(size_t)i].y;
+            pos[3 * i + 2] = p[(size_t)i].z;
+        }
+        if (normal) {
+            uint32_t e;
+            std::memcpy(&e, &p[(size_t)i].w, 4);
+            oct_decode_host(e, normal + 3 * i);
+        }
+    }
+    return PPPM_OK;
+}
+
+int pppm_subsurface_irradiance(void* ctx_raw, float* E, int64_t cap, int64_t* n) {
+    CHECK_CTX(ctx);
+    if (!ctx->last_valid || !ctx->sss_present) return ctx->fail(PPPM_ERR_STATE, "no completed pass with subsurface points");
+    if (n) *n = ctx->sss_n;
+    const int64_t m = std::min(cap, ctx->sss_n);
+    if (m <= 0 || !E) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    TRY_HIP(ctx, hipMemcpy(E, ctx->sss_E.p, sizeof(float) * 3 * (size_t)m, hipMemcpyDeviceToHost));
+    return PPPM_OK;
+}
+
+int pppm_last_irradiance_photons(void* ctx_raw, float* pos, float* flux, float* normal, uint32_t* key, int64_t cap,
+                                 int64_t* n) {
+    CHECK_CTX(ctx);
+    if (!ctx->last_valid || !ctx->sss_present) return ctx->fail(PPPM_ERR_STATE, "no completed pass with subsurface points");
+    if (n) *n = ctx->last_n_irr;
+    const int64_t m = std::min(cap, ctx->last_n_irr);
+    if (m <= 0) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    std::vector<float4> p((size_t)m), f((size_t)m);
+    TRY_HIP(ctx, hipMemcpy(p.data(), ctx->irr_pos.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    TRY_HIP(ctx, hipMemcpy(f.data(), ctx->irr_flux.p, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; i++) {
+        if (pos) {
+            pos[3 * i] = p[(size_t)i].x;
+            pos[3 * i + 1] = p[(size_t)i].y;
+            pos[3 * i + 2] = p[(size_t)i].z;
+        }
+        if (flux) {
+            flux[3 * i] = f[(size_t)i].x;
+            flux[3 * i + 1] = f[(size_t)i].y;
+            flux[3 * i + 2] = f[(size_t)i].z;
+        }
+        uint32_t e;
+        if (normal) {
+            std::memcpy(&e, &p[(size_t)i].w, 4);
+            oct_decode_host(e, normal + 3 * i);
+        }
+        if (key) std::memcpy(&key[i], &f[(size_t)i].w, 4);
+    }
+    return PPPM_OK;
+}
+
+int pppm_dipole_eval(void* ctx_raw, const pppm_bsdf* material, const float* pts, const float* E, int64_t n_pts,
+                     double area, const float* query, int64_t n_query, float* out) {
+    CHECK_CTX(ctx);
+    if (!material || !(material->type & PPPM_BSDF_BSSRDF) || n_pts < 0 || n_query < 0 ||
+        (n_pts > 0 && (!pts || !E)) || (n_query > 0 && (!query || !out)) || n_pts > (1LL << 26) ||
+        n_query > (1LL << 26))
+        return ctx->fail(PPPM_ERR_INVALID, "bad dipole arguments");
+    if (n_query == 0) return PPPM_OK;
+    (void)hipSetDevice(ctx->device);
+    const double eta = material->eta > 0.0f ? material->eta : 1.3;
+    const double fdr = dipole_fdr(eta), A = (1.0 + fdr) / (1.0 - fdr);
+    DipoleDev d;
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; k++) {
+        const double sa = material->sigma_a[k], ss = material->sigmap_s[k], st = sa + ss;
+        const double tr = std::sqrt(3.0 * sa * st);
+        d.zp[k] = (float)(1.0 / st);
+        d.zn[k] = (float)((1.0 / st) * (1.0 + (4.0 / 3.0) * A));
+        d.tr[k] = (float)tr;
+        d.k[k] = (float)((ss / st) / (4.0 * M_PI * tr));
+    }
+    d.out = (float)((1.0 - fdr) / M_PI);
+    std::vector<float4> p4((size_t)std::max<int64_t>(n_pts, 1)), q4((size_t)n_query), w4((size_t)n_query);
+    for (int64_t i = 0; i < n_pts; i++) p4[(size_t)i] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], 0.0f);
+    for (int64_t i = 0; i < n_query; i++) {
+        q4[(size_t)i] = make_float4(query[3 * i], query[3 * i + 1], query[3 * i + 2], 0.0f);  // dipole index 0
+        const uint32_t id = (uint32_t)i;
+        float idf;
+        std::memcpy(&idf, &id, 4);
+        w4[(size_t)i] = make_float4(1.0f, 1.0f, 1.0f, idf);
+    }
+    void* buf = nullptr;
+    const size_t bytes = sizeof(DipoleDev) + sizeof(float4) * (p4.size() + 2 * q4.size()) +
+                         sizeof(float) * 3 * (size_t)(std::max<int64_t>(n_pts, 1) + n_query);
+    TRY_HIP(ctx, hipMalloc(&buf, bytes));
+    char* c = static_cast<char*>(buf);
+    DipoleDev* dd = reinterpret_cast<DipoleDev*>(c);
+    float4* dp = reinterpret_cast<float4*>(c + sizeof(DipoleDev));
+    float4* dq = dp + p4.size();
+    float4* dw = dq + q4.size();
+    float* dE = reinterpret_cast<float*>(dw + q4.size());
+    float* dout = dE + 3 * std::max<int64_t>(n_pts, 1);
+    hipError_t e = hipMemcpy(dd, &d, sizeof(d), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, p4.data(), sizeof(float4) * p4.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dq, q4.data(), sizeof(float4) * q4.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dw, w4.data(), sizeof(float4) * w4.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_pts > 0) e = hipMemcpy(dE, E, sizeof(float) * 3 * (size_t)n_pts, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        DipoleArgs a;
+        a.pts = dp;
+        a.E = dE;
+        a.n_pts = n_pts;
+        a.area = (float)area;
+        a.dip = dd;
+        a.hit_pos = dq;
+        a.hit_w = dw;
+        a.count_src = nullptr;
+        a.n_hit = n_query;
+        a.img_sum = nullptr;
+        a.img_comp = nullptr;
+        a.out_rgb = dout;
+        if (n_pts > 0) {
+            launch_dipole(ctx->stream, a);
+        } else {
+            launch_zero_u32(ctx->stream, dout, 3 * n_query);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * 3 * (size_t)n_query, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("dipole: ") + hipGetErrorString(e));
     return PPPM_OK;
 }
 

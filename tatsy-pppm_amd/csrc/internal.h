@@ -78,6 +78,8 @@ struct Counters {
     unsigned long long gather_inradius;
     unsigned long long stack_overflow;
     unsigned long long proposal_records;
+    unsigned long long irr_stored;   // photons stored on BSSRDF surfaces
+    unsigned long long sss_hits;     // eye paths ending in the subsurface estimate
 };
 
 struct PhotonPassArgs {
@@ -93,6 +95,10 @@ struct PhotonPassArgs {
     float4* ph_flux;  // rgb, origin key bits (photon << 6 | bounce)
     Counters* counters;
     PropDev prop;     // emission proposal (side 0: plain cosine emission)
+    // irradiance photons at BSSRDF surfaces (null: none stored)
+    float4* irr_pos;  // xyz, oct normal bits
+    float4* irr_flux; // rgb, origin key bits
+    int64_t irr_cap;
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 
@@ -113,6 +119,11 @@ struct EyePassArgs {
     float* img_sum;
     float* img_comp;
     Counters* counters;
+    // eye paths transmitted into a BSSRDF surface: (xyz, dipole index bits)
+    // (weight rgb, pixel id bits), appended (null: none recorded)
+    float4* sss_pos;
+    float4* sss_w;
+    int64_t sss_cap;
 };
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 
@@ -254,3 +265,25 @@ void launch_mlp_prep(hipStream_t s, const float4* ring, int64_t n_valid, int64_t
                      int identity, uint32_t* idx, float* inv_wsum);
 void launch_prop_table(hipStream_t s, const float* prob, int32_t anchors, float* cdf);
 void launch_contrib(hipStream_t s, const ContribArgs& a, uint64_t* block_sums);
+
+// ---------------------------------------------------------------------------
+// Subsurface scattering (subsurface.hip)
+// ---------------------------------------------------------------------------
+struct DipoleArgs {
+    const float4* pts;        // surface points: xyz, -
+    const float* E;           // irradiance [n_pts][3]
+    int64_t n_pts;
+    float area;               // surface area per point
+    const DipoleDev* dip;     // per BSSRDF material
+    const float4* hit_pos;    // xyz, dipole index bits
+    const float4* hit_w;      // weight rgb, pixel id bits
+    const Counters* count_src;  // sss_hits, or null: n_hit
+    int64_t n_hit;
+    float* img_sum;           // Kahan image (null: write out_rgb [hit][3])
+    float* img_comp;
+    float* out_rgb;
+};
+// L = W (1 - F_dr) / pi * area * sum_j R_d(|x - x_j|^2) E_j for every hit
+void launch_dipole(hipStream_t s, const DipoleArgs& a);
+// stratified points on triangles: n points over the area CDF, uniform inside
+// each triangle (host, deterministic)

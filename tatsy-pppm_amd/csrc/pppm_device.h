@@ -142,7 +142,17 @@ struct Mat {
     float rr_mean;  // (rho.x + rho.y + rho.z) / 3: photon Russian roulette
     float rr_max;   // max(rho): eye Russian roulette
     float ior;
-    float pad[2];
+    int32_t sss;   // index into the dipole table (BSSRDF materials), else -1
+    float pad;
+};
+
+// Dipole BSSRDF constants of one material (sources/bssrdf.cc:46-62), per
+// channel: z+ = 1 / sigma'_t, z- = z+ (1 + 4A/3), sigma_tr = sqrt(3 sigma_a
+// sigma'_t), k = alpha' / (4 pi sigma_tr); out = (1 - F_dr) / pi
+struct DipoleDev {
+    float zp[3], zn[3], tr[3], k[3];
+    float out;
+    float pad[3];
 };
 
 struct LightDev {

@@ -128,6 +128,16 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
             const v3f ng = tri_normal(sc, h.tri);
             const v3f on = dot(ng, dir) < 0.0f ? ng : -ng;
             if (m.type & PPPM_BSDF_EMITTER) break;
+            if ((m.type & PPPM_BSDF_BSSRDF) && a.irr_pos && dot(ng, dir) < 0.0f) {
+                // irradiance photon (arriving from outside) for the subsurface
+                // estimate; the path goes on through the dielectric boundary below
+                unsigned long long slot = wave_append(&a.counters->irr_stored);
+                if (slot < (unsigned long long)a.irr_cap) {
+                    a.irr_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                    a.irr_flux[slot] = make_float4(flux.x, flux.y, flux.z,
+                                                   __uint_as_float(((uint32_t)i << 6) | (uint32_t)bounce));
+                }
+            }
             if (m.type & PPPM_BSDF_LAMBERTIAN) {
                 unsigned long long slot = wave_append(&a.counters->photons_stored);
                 if (slot < (unsigned long long)a.cap) {
@@ -221,6 +231,17 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
             v3f nd;
             if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(d, on);
             else nd = refraction_sample(d, ng, m.ior, r1, &pdf);
+            if ((m.type & PPPM_BSDF_BSSRDF) && a.sss_pos && (dot(nd, ng) < 0.0f) == (dot(d, ng) < 0.0f)) {
+                // transmitted into the translucent body: the path ends in the
+                // dipole estimate at hp, weighted F_t / (1 - P) (ppm_probabilistic.cc:272-296)
+                const v3f w = thr * (1.0f / (pdf * roulette));
+                unsigned long long slot = wave_append(&a.counters->sss_hits);
+                if (slot < (unsigned long long)a.sss_cap) {
+                    a.sss_pos[slot] = make_float4(hp.x, hp.y, hp.z, __int_as_float(m.sss));
+                    a.sss_w[slot] = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
+                }
+                break;
+            }
             thr = thr * rho * (1.0f / (pdf * roulette));
             o = spawn(hp, ng, nd, a.ray_eps);
             d = nd;

@@ -301,7 +301,19 @@ int pppm_scenegen(int32_t kind, uint64_t seed, int32_t frequency, int32_t width,
         b.add_quad({6, 2, 2.9}, {0, 4, 0}, {0, 0, 0.2}, wall);
         b.add_ceiling_light(3.0, 6.0 - 1e-3, 3.0, 1.0, le);
         int f = frequency > 0 ? frequency : (kind == PPPM_SCENE_TWO_ROOMS ? 100 : 316);
-        int sm = b.add_bsdf(PPPM_BSDF_LAMBERTIAN, albedo(rng));
+        int sm;
+        if (kind == PPPM_SCENE_TWO_ROOMS_SSS) {
+            // translucent: specular dielectric boundary over a dipole BSSRDF
+            // (the marble set-up of sources/main/appmain.cc:43-66 with the
+            // coefficients drawn per channel)
+            sm = b.add_bsdf(PPPM_BSDF_REFRACTION | PPPM_BSDF_BSSRDF, {0.99, 0.99, 0.99});
+            for (int k = 0; k < 3; k++) {
+                b.bsdfs[(size_t)sm].sigma_a[k] = (float)(0.001 + 0.049 * rng.uniform());
+                b.bsdfs[(size_t)sm].sigmap_s[k] = (float)(0.5 + 1.5 * rng.uniform());
+            }
+        } else {
+            sm = b.add_bsdf(PPPM_BSDF_LAMBERTIAN, albedo(rng));
+        }
         b.add_icosphere({9.0, 1.3, 3.0}, 1.3, f, sm, &noise, 0.05, 4.0);
         finish(b, out, make_cam({11.9, 3.0, 3.0}, {-1.0, -0.15, 0.0}, 60.0, width, height));
         return PPPM_OK;

@@ -57,7 +57,9 @@ class PassStats(C.Structure):
                 ("photon_ms", C.c_double), ("order_ms", C.c_double), ("grid_ms", C.c_double),
                 ("eye_ms", C.c_double), ("gather_ms", C.c_double), ("total_ms", C.c_double),
                 ("hitsort_ms", C.c_double), ("proposal_ms", C.c_double), ("train_loss", C.c_double),
-                ("proposal_records", C.c_int64), ("proposal_active", C.c_int32), ("retrained", C.c_int32)]
+                ("proposal_records", C.c_int64), ("proposal_active", C.c_int32), ("retrained", C.c_int32),
+                ("irradiance_ms", C.c_double), ("dipole_ms", C.c_double), ("irradiance_photons", C.c_int64),
+                ("sss_hits", C.c_int64), ("sss_points", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -103,6 +105,11 @@ _DECLS = {
     "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
                               C.c_int32, _P, C.c_int32, _P, C.POINTER(PassStats)]),
     "pppm_stream_copy": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
+    "pppm_subsurface_configure": (C.c_int, [_P, C.c_int64]),
+    "pppm_subsurface_points": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "pppm_subsurface_irradiance": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "pppm_last_irradiance_photons": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "pppm_dipole_eval": (C.c_int, [_P, C.POINTER(Bsdf), _P, _P, C.c_int64, C.c_double, _P, C.c_int64, _P]),
     "pppm_proposal_enable": (C.c_int, [_P, C.POINTER(ProposalParams)]),
     "pppm_proposal_set_table": (C.c_int, [_P, _P, C.c_int32, C.c_double]),
     "pppm_proposal_get_table": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int32)]),
@@ -402,3 +409,44 @@ class Context:
         self._chk(self.L.pppm_mlp_train_step(self.h, _ptr(rec), len(rec), learning_rate, step, _ptr(grad),
                                              C.byref(loss)))
         return grad, loss.value
+
+    # ---- subsurface scattering ----
+    def subsurface_configure(self, points):
+        self._chk(self.L.pppm_subsurface_configure(self.h, points))
+
+    def subsurface_points(self):
+        n = C.c_int64()
+        area = C.c_double()
+        self._chk(self.L.pppm_subsurface_points(self.h, None, None, 0, C.byref(n), C.byref(area)))
+        pos = np.zeros((max(n.value, 1), 3), np.float32)
+        nrm = np.zeros((max(n.value, 1), 3), np.float32)
+        self._chk(self.L.pppm_subsurface_points(self.h, _ptr(pos), _ptr(nrm), n.value, C.byref(n), C.byref(area)))
+        return pos[:n.value], nrm[:n.value], area.value
+
+    def subsurface_irradiance(self):
+        n = C.c_int64()
+        self._chk(self.L.pppm_subsurface_irradiance(self.h, None, 0, C.byref(n)))
+        E = np.zeros((max(n.value, 1), 3), np.float32)
+        self._chk(self.L.pppm_subsurface_irradiance(self.h, _ptr(E), n.value, C.byref(n)))
+        return E[:n.value]
+
+    def last_irradiance_photons(self):
+        n = C.c_int64()
+        self._chk(self.L.pppm_last_irradiance_photons(self.h, None, None, None, None, 0, C.byref(n)))
+        m = n.value
+        pos = np.zeros((max(m, 1), 3), np.float32)
+        flux = np.zeros((max(m, 1), 3), np.float32)
+        nrm = np.zeros((max(m, 1), 3), np.float32)
+        key = np.zeros(max(m, 1), np.uint32)
+        self._chk(self.L.pppm_last_irradiance_photons(self.h, _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), m,
+                                                      C.byref(n)))
+        return pos[:m], flux[:m], nrm[:m], key[:m]
+
+    def dipole_eval(self, material, pts, E, area, query):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        E = np.ascontiguousarray(E, np.float32).reshape(-1, 3)
+        q = np.ascontiguousarray(query, np.float32).reshape(-1, 3)
+        out = np.zeros((max(len(q), 1), 3), np.float32)
+        self._chk(self.L.pppm_dipole_eval(self.h, C.byref(material), _ptr(pts), _ptr(E), len(pts), area, _ptr(q),
+                                          len(q), _ptr(out)))
+        return out[:len(q)]
