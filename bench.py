@@ -180,6 +180,85 @@ def error_study(P, np, photons, width, passes, ref_passes, lam, probe_every=8):
     }
 
 
+# Dipole sum issue model (gfx950, wave64): the inner loop of k_dipole issues
+# DIPOLE_VALU full-rate and DIPOLE_TRANS transcendental (v_rsq_f32, v_exp_f32)
+# VALU instructions per pair (counted in its AMD assembly); their cycle costs
+# come from pppm_issue_probe, measured in the same run.
+DIPOLE_VALU = 65
+DIPOLE_TRANS = 12
+
+
+def dipole_peaks(cus, probe):
+    lanes_per_s = cus * 4 * probe["clock_hz"] * 64
+    issue = lanes_per_s / (probe["valu_cycles"] * DIPOLE_VALU + probe["trans_cycles"] * DIPOLE_TRANS)
+    trans = lanes_per_s / (probe["trans_cycles"] * DIPOLE_TRANS)
+    return issue, trans
+
+
+def subsurface_study(P, np, width, photons, passes, points, frequency):
+    """The translucent two-room scene: displaced icosphere (frequency f gives
+    20 f^2 triangles) with a dipole BSSRDF, `points` surface points, the
+    learned proposal on.  Reports the per-pass stages and the dipole sum's
+    pair evaluations per second against the issue model."""
+    scene = P.scenegen(P.SCENE_TWO_ROOMS_SSS, 1, frequency, width, width)
+    r1 = 0.01 * scene.diagonal()
+    ctx = P.Context(0)
+    cus = ctx.device_info()["cu"]
+    probe = ctx.issue_probe()
+    ctx.upload(scene)
+    bvh = ctx.build_bvh()
+    ctx.set_camera(scene.camera)
+    ctx.subsurface_configure(points)
+    t0 = time.perf_counter()
+    ctx.begin(photons, passes + 1, r1, seed=0x7A75)
+    setup_s = time.perf_counter() - t0
+    ctx.proposal_enable(lam=0.5, anchors=8, train_interval=8, train_steps=64, batch=16384,
+                        ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
+    ctx.render_pass()  # warm-up
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        sts.append(ctx.render_pass())
+    wall = time.perf_counter() - t0
+    img = ctx.image()
+    ctx.close()
+    assert np.all(np.isfinite(img))
+    pairs = sum(s["sss_hits"] * s["sss_points"] for s in sts)
+    dip_s = sum(s["dipole_ms"] for s in sts) * 1e-3
+    issue, trans = dipole_peaks(cus, probe)
+    achieved = pairs / dip_s if dip_s > 0 else 0.0
+    return {
+        "workload": "two rooms, translucent displaced icosphere (dipole BSSRDF), learned proposal on",
+        "triangles": int(scene.n_tris),
+        "bvh_build_ms": bvh["build_ms"],
+        "resolution": [width, width],
+        "photons_per_pass": photons,
+        "passes": passes,
+        "surface_points": int(sts[0]["sss_points"]),
+        "points_setup_s": setup_s,
+        "ms_per_pass": 1e3 * wall / passes,
+        "photons_per_s": photons * passes / wall,
+        "stage_ms": {k: sum(s[k] for s in sts) / passes
+                     for k in ("photon_ms", "irradiance_ms", "order_ms", "grid_ms", "eye_ms", "gather_ms",
+                               "proposal_ms", "dipole_ms", "total_ms")},
+        "irradiance_photons_per_pass": sum(s["irradiance_photons"] for s in sts) / passes,
+        "sss_hits_per_pass": sum(s["sss_hits"] for s in sts) / passes,
+        "dipole_roofline": {
+            "bound": "valu",
+            "kernel": "k_dipole",
+            "achieved": achieved,
+            "peak": issue,
+            "unit": "pair evaluations/s",
+            "frac": achieved / issue,
+            "transcendental_peak": trans,
+            "frac_of_transcendental_peak": achieved / trans,
+            "model": (f"{DIPOLE_VALU} full-rate VALU x {probe['valu_cycles']:.2f} cycles + {DIPOLE_TRANS} "
+                      f"transcendental x {probe['trans_cycles']:.2f} cycles per pair and wave64 (measured), "
+                      f"{cus} CUs x 4 SIMDs at {probe['clock_hz'] / 1e9:.2f} GHz"),
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +274,11 @@ def main():
     ap.add_argument("--rmse-passes", type=int, default=256)
     ap.add_argument("--rmse-ref-passes", type=int, default=1024)
     ap.add_argument("--rmse-lambda", type=float, default=0.5)
+    ap.add_argument("--no-sss", action="store_true", help="skip the subsurface (dipole BSSRDF) leg")
+    ap.add_argument("--sss-passes", type=int, default=2)
+    ap.add_argument("--sss-photons", type=int, default=8_000_000)
+    ap.add_argument("--sss-points", type=int, default=1 << 20)
+    ap.add_argument("--sss-frequency", type=int, default=316)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -289,6 +373,11 @@ def main():
     if not args.no_rmse:
         out["time_to_rmse"] = error_study(P, np, args.photons, args.width, args.rmse_passes,
                                           args.rmse_ref_passes, args.rmse_lambda)
+    if not args.no_sss:
+        out["subsurface"] = subsurface_study(P, np, args.width, args.sss_photons, args.sss_passes,
+                                             args.sss_points, args.sss_frequency)
+    else:
+        out["subsurface"] = None
     if not args.no_cpu:
         cb, _ = cpu_baseline(scene, args.photons, r1, ray_eps, seed)
         out["cpu_baseline"] = cb

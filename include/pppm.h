@@ -215,6 +215,11 @@ int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const floa
  * reference counterpart): the known-traffic kernel that scales profiler
  * FETCH_SIZE / WRITE_SIZE counters, and a measured copy bandwidth. */
 int pppm_stream_copy(void* ctx, int64_t bytes, double* ms);
+/* Issue-rate calibration (no reference counterpart): cycles per wave64
+ * instruction on one SIMD at the reported clock for a full-rate v_fma_f32
+ * and for v_exp_f32, measured with independent chains; the roofline of the
+ * dipole sum is built from them. */
+int pppm_issue_probe(void* ctx, double* valu_cycles, double* trans_cycles, double* clock_hz);
 
 /* ---- subsurface scattering: replaces SubsurfaceIntegrator (sources/
  * subsurface_integrator.cc:180-365, called per pass from ppm_probabilistic.cc:

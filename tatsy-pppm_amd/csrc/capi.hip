@@ -1476,6 +1476,41 @@ int pppm_stream_copy(void* ctx_raw, int64_t bytes, double* ms) {
     return PPPM_OK;
 }
 
+int pppm_issue_probe(void* ctx_raw, double* valu_cycles, double* trans_cycles, double* clock_hz) {
+    CHECK_CTX(ctx);
+    (void)hipSetDevice(ctx->device);
+    int clk_khz = 0;
+    TRY_HIP(ctx, hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, ctx->device));
+    const int blocks = (ctx->cu > 0 ? ctx->cu : 256) * 16;
+    float* out = nullptr;
+    TRY_HIP(ctx, hipMalloc(&out, sizeof(float) * 256 * (size_t)blocks));
+    double cyc[2] = {0.0, 0.0};
+    hipError_t e = hipSuccess;
+    for (int which = 0; which < 2 && e == hipSuccess; which++) {
+        float best = 1e30f;
+        for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+            (void)hipEventRecord(ctx->ev[0], ctx->stream);
+            launch_issue_probe(ctx->stream, out, blocks, which);
+            (void)hipEventRecord(ctx->ev[1], ctx->stream);
+            e = hipEventSynchronize(ctx->ev[1]);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+            if (ms > 0.0f) best = std::min(best, ms);
+        }
+        // cycles per wave instruction on one SIMD at the reported clock
+        const double waves = (double)blocks * 4.0;
+        const double rate = waves * (double)issue_probe_wave_instr(which) / (best * 1e-3);
+        cyc[which] = (double)(ctx->cu > 0 ? ctx->cu : 256) * 4.0 * clk_khz * 1e3 / rate;
+    }
+    (void)hipFree(out);
+    if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("probe: ") + hipGetErrorString(e));
+    if (valu_cycles) *valu_cycles = cyc[0];
+    // the exp probe alternates v_exp_f32 and a full-rate v_fma_f32
+    if (trans_cycles) *trans_cycles = 2.0 * cyc[1] - cyc[0];
+    if (clock_hz) *clock_hz = clk_khz * 1e3;
+    return PPPM_OK;
+}
+
 int pppm_subsurface_configure(void* ctx_raw, int64_t points) {
     CHECK_CTX(ctx);
     if (points < 0 || points > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "points: 0 (default) .. 2^26");

@@ -285,5 +285,9 @@ struct DipoleArgs {
 };
 // L = W (1 - F_dr) / pi * area * sum_j R_d(|x - x_j|^2) E_j for every hit
 void launch_dipole(hipStream_t s, const DipoleArgs& a);
+// issue-rate probes (which 0: v_fma_f32 chains, 1: v_exp_f32 + v_fma_f32
+// chains) over blocks x 256 threads writing out[blocks * 256]
+void launch_issue_probe(hipStream_t s, float* out, int blocks, int which);
+int64_t issue_probe_wave_instr(int which);  // wave instructions per wave
 // stratified points on triangles: n points over the area CDF, uniform inside
 // each triangle (host, deterministic)

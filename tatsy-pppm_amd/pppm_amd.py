@@ -105,6 +105,7 @@ _DECLS = {
     "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
                               C.c_int32, _P, C.c_int32, _P, C.POINTER(PassStats)]),
     "pppm_stream_copy": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
+    "pppm_issue_probe": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pppm_subsurface_configure": (C.c_int, [_P, C.c_int64]),
     "pppm_subsurface_points": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "pppm_subsurface_irradiance": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),
@@ -409,6 +410,11 @@ class Context:
         self._chk(self.L.pppm_mlp_train_step(self.h, _ptr(rec), len(rec), learning_rate, step, _ptr(grad),
                                              C.byref(loss)))
         return grad, loss.value
+
+    def issue_probe(self):
+        v, t, c = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.pppm_issue_probe(self.h, C.byref(v), C.byref(t), C.byref(c)))
+        return {"valu_cycles": v.value, "trans_cycles": t.value, "clock_hz": c.value}
 
     # ---- subsurface scattering ----
     def subsurface_configure(self, points):
