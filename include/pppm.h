@@ -296,6 +296,32 @@ int pppm_mlp_forward(void* ctx, const float* uv, int64_t n, float* prob);
 int pppm_mlp_train_step(void* ctx, const float* rec, int64_t n, double learning_rate, int32_t step,
                         float* grad, double* loss);
 
+/* ---- batched scenes: many renders in one hipGraph-captured pass loop.  No
+ * reference counterpart (the reference renders one scene per process,
+ * sources/main/appmain.cc:11-26); each scene's pass is pppm_render_pass's
+ * computation (same image bit for bit, for the same hash size and gather
+ * kernel), enqueued without host synchronisation: counts stay on the device,
+ * sorts run over the buffer capacity, pass index and radius live in device
+ * memory.  One pass of every scene is captured once into a hipGraph whose
+ * branches (streams) run concurrently, scenes spread over them longest
+ * first; each replay renders one more pass of every scene.  Plain PPM. ---- */
+int pppm_batch_create(void** batch, int32_t device, int32_t branches);
+int pppm_batch_destroy(void* batch);
+const char* pppm_batch_last_error(const void* batch);
+/* A scene with its camera and render parameters (hash_size > 0 required). */
+int pppm_batch_add(void* batch, const float* tri_xyz, const int32_t* bsdf_id, int64_t n, const pppm_bsdf* table,
+                   int32_t nb, const pppm_light* lights, int32_t nl, const pppm_camera* camera,
+                   const pppm_render_params* params, int32_t* index);
+/* Autotuner: per scene, times one pass of each gather kernel (tune_gather)
+ * and each photon-kernel grid of 16, 32, 64 and 128 waves per CU (tune_grid)
+ * and keeps the fastest; the renders restart. */
+int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid);
+/* Runs `passes` more passes of every scene (graph replays); *ms = GPU time. */
+int pppm_batch_run(void* batch, int32_t passes, double* ms);
+int pppm_batch_image(void* batch, int32_t index, double* rgb, int64_t n_values);
+int pppm_batch_scene_info(void* batch, int32_t index, int32_t* gather_mode, int32_t* photon_grid, int32_t* branch,
+                          double* cost_ms);
+
 /* ---- procedural scenes (no reference counterpart: the reference loads PLY
  * meshes, sources/trimesh.cc:125-217) ---- */
 #define PPPM_SCENE_BOX 1        /* box + two icospheres */

@@ -24,11 +24,12 @@ namespace {
 // Cells of edge 2r met by [x - r, x + r] per axis: at most two (a third can
 // only come from FP64 rounding at an exact cell boundary, and is dropped).
 __device__ __forceinline__ void hit_cells(const GatherArgs& a, float4 hp, int32_t lo[3], int32_t hi[3]) {
-    const double r = a.radius;
+    const double r = gather_radius(a);
+    const double sc = grid_scale(a.g);
     const double xs[3] = {hp.x, hp.y, hp.z};
     for (int d = 0; d < 3; d++) {
-        lo[d] = grid_cell(xs[d] - r, a.g.bmin[d], a.g.scale);
-        hi[d] = min(grid_cell(xs[d] + r, a.g.bmin[d], a.g.scale), lo[d] + 1);
+        lo[d] = grid_cell(xs[d] - r, a.g.bmin[d], sc);
+        hi[d] = min(grid_cell(xs[d] + r, a.g.bmin[d], sc), lo[d] + 1);
     }
 }
 
@@ -79,7 +80,7 @@ __device__ __forceinline__ float cone_norm(double r) {
 }
 
 __device__ __forceinline__ void write_result(const GatherArgs& a, int64_t pix, v3f w, float ax, float ay, float az) {
-    const float inv_norm = cone_norm(a.radius);
+    const float inv_norm = cone_norm(gather_radius(a));
     const float Lx = w.x * ax * inv_norm, Ly = w.y * ay * inv_norm, Lz = w.z * az * inv_norm;
     if (a.img_sum) {
         kahan_add(a.img_sum, a.img_comp, 3 * pix + 0, Lx);
@@ -161,8 +162,8 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
         const v3f nx = oct_decode(__float_as_uint(hp.w));
         uint32_t mb[8];
         uint32_t todo = own_buckets(a, hp, mb);
-        const float r2 = (float)(a.radius * a.radius);
-        const float inv_kr = (float)(1.0 / (1.1 * a.radius));
+        const float r2 = (float)(gather_radius(a) * gather_radius(a));
+        const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
         // slots in the reference's z, y, x cell order: lanes of neighbouring
         // half cells then walk their shared buckets in step
 #pragma unroll
@@ -244,8 +245,8 @@ __global__ __launch_bounds__(256) void k_gather_wave(GatherArgs a) {
     LaneHit h;
     lane_hit(a, n, i, h);
     uint32_t pending = h.vm;
-    const float r2 = (float)(a.radius * a.radius);
-    const float inv_kr = (float)(1.0 / (1.1 * a.radius));
+    const float r2 = (float)(gather_radius(a) * gather_radius(a));
+    const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
     const const_f4 pa = as_const(a.ph_a), pb = as_const(a.ph_b), pc = as_const(a.ph_c);
     const __attribute__((address_space(4))) uint32_t* off =
         (const __attribute__((address_space(4))) uint32_t*)a.off;
@@ -286,8 +287,8 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
     LaneHit h;
     lane_hit(a, n, i, h);
     uint32_t pending = h.vm;
-    const float r2 = (float)(a.radius * a.radius);
-    const float inv_kr = (float)(1.0 / (1.1 * a.radius));
+    const float r2 = (float)(gather_radius(a) * gather_radius(a));
+    const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
     Acc acc;
     if (lane == 0) s_pre[0] = 0;
     while (true) {
@@ -361,7 +362,7 @@ __global__ void k_hit_keys(const float4* __restrict__ hp_pos, const float4* __re
         return;
     }
     const float4 p = hp_pos[i];
-    const double inv = 1.0 / radius;
+    const double inv = 1.0 / (g.pd ? g.pd->radius : radius);
     const double c[3] = {((double)p.x - g.bmin[0]) * inv, ((double)p.y - g.bmin[1]) * inv,
                          ((double)p.z - g.bmin[2]) * inv};
     uint32_t q[3];
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(256) void k_mark_used(GatherArgs a, const uint32_t*
     const float4 hp = a.hp_pos[i];
     const v3f x = xyz(hp);
     const v3f nx = oct_decode(__float_as_uint(hp.w));
-    const float r2 = (float)(a.radius * a.radius);
+    const float r2 = (float)(gather_radius(a) * gather_radius(a));
     const bool compact = a.mode == GATHER_LANES;
     uint32_t mb[8];
     const uint32_t todo = own_buckets(a, hp, mb);

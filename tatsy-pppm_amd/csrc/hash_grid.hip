@@ -25,9 +25,15 @@ __device__ inline uint32_t hash3(int32_t ix, int32_t iy, int32_t iz, uint32_t si
 
 __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __restrict__ pos3,
                              const uint32_t* __restrict__ perm, int64_t n, GridParams g,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                             const unsigned long long* __restrict__ count, uint32_t sentinel) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (count && i >= (int64_t)min((unsigned long long)n, *count)) {
+        keys[i] = sentinel;
+        vals[i] = (uint32_t)i;
+        return;
+    }
     int64_t j = perm ? (int64_t)perm[i] : i;
     double x, y, z;
     if (pos4) {
@@ -36,19 +42,21 @@ __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __res
     } else {
         x = pos3[3 * j]; y = pos3[3 * j + 1]; z = pos3[3 * j + 2];
     }
-    keys[i] = hash3(cell_of(x, g.bmin[0], g.scale), cell_of(y, g.bmin[1], g.scale),
-                    cell_of(z, g.bmin[2], g.scale), g.size);
+    const double sc = grid_scale(g);
+    keys[i] = hash3(cell_of(x, g.bmin[0], sc), cell_of(y, g.bmin[1], sc), cell_of(z, g.bmin[2], sc), g.size);
     vals[i] = (uint32_t)i;
 }
 
 // off[h] = lower_bound(sorted_keys, h) for h in [0, size]; each key boundary
-// fills the empty buckets up to it.
+// fills the empty buckets up to it.  Keys >= size (sentinels of unused slots)
+// may follow the real ones.
 __global__ void k_bucket_offsets(const uint32_t* __restrict__ keys, int64_t n, uint32_t size,
                                  uint32_t* __restrict__ off) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > n) return;
     int64_t lo = (j == 0) ? 0 : (int64_t)keys[j - 1] + 1;
     int64_t hi = (j == n) ? (int64_t)size : (int64_t)keys[j];
+    if (hi > (int64_t)size) hi = size;  // sentinel keys (>= size) close the last bucket
     for (int64_t h = lo; h <= hi; h++) off[h] = (uint32_t)j;
 }
 
@@ -129,10 +137,11 @@ __global__ void k_scan_u64_add(uint64_t* data, int64_t n, const uint64_t* block_
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
-                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals) {
+                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals, const unsigned long long* count,
+                       uint32_t sentinel) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_point_keys, dim3(grid1(n, 256)), dim3(256), 0, s, pos4, pos3, perm, n, g,
-                       keys, vals);
+                       keys, vals, count, sentinel);
 }
 
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,

@@ -29,14 +29,28 @@ bool radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* k
 // ---------------------------------------------------------------------------
 // Hash grid kernels (hash_grid.hip)
 // ---------------------------------------------------------------------------
+// Pass constants in device memory: a pass enqueued without host
+// synchronisation (batched, graph-captured loops) reads its pass index and
+// radius here, and k_pass_advance moves them to the next pass.
+struct PassDev {
+    double radius;   // r_i
+    double scale;    // 1 / (2 r_i)
+    uint32_t pass;   // i, 1-based
+    uint32_t overflow;  // set when a pass stored more photons than the buffers hold
+};
+
 struct GridParams {
     double bmin[3];
     double scale;
     uint32_t size;
+    const PassDev* pd = nullptr;  // when set, the scale is read from it
 };
-// keys[i] = hash(cell(pos[perm ? perm[i] : i])), vals[i] = i
+__device__ __forceinline__ double grid_scale(const GridParams& g) { return g.pd ? g.pd->scale : g.scale; }
+// keys[i] = hash(cell(pos[perm ? perm[i] : i])), vals[i] = i; with a device
+// count, slots i >= min(*count, n) get key `sentinel` (>= size)
 void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
-                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals);
+                       int64_t n, GridParams g, uint32_t* keys, uint32_t* vals,
+                       const unsigned long long* count = nullptr, uint32_t sentinel = 0xFFFFFFFFu);
 // off[h] = number of sorted keys < h, for h = 0..size
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
                            uint32_t* off);
@@ -94,6 +108,7 @@ struct PhotonPassArgs {
     float4* ph_pos;   // xyz, oct normal bits
     float4* ph_flux;  // rgb, origin key bits (photon << 6 | bounce)
     Counters* counters;
+    const PassDev* pd = nullptr;  // when set, the pass index is read from it
     PropDev prop;     // emission proposal (side 0: plain cosine emission)
     // irradiance photons at BSSRDF surfaces (null: none stored)
     float4* irr_pos;  // xyz, oct normal bits
@@ -119,6 +134,7 @@ struct EyePassArgs {
     float* img_sum;
     float* img_comp;
     Counters* counters;
+    const PassDev* pd = nullptr;  // when set, the pass index is read from it
     // eye paths transmitted into a BSSRDF surface: (xyz, dipole index bits)
     // (weight rgb, pixel id bits), appended (null: none recorded)
     float4* sss_pos;
@@ -153,6 +169,7 @@ struct GatherArgs {
     Counters* counters;
     uint32_t* used;   // lanes kernel: used[j] = 1 for accepted photon records j (may be null)
 };
+__device__ __forceinline__ double gather_radius(const GatherArgs& a) { return a.g.pd ? a.g.pd->radius : a.radius; }
 void launch_gather(hipStream_t s, const GatherArgs& a);
 // used[sorted[j]] = 1 for each photon record j (bucket order) that the gather
 // of a's hit points accepts (used is not cleared here)
@@ -161,6 +178,7 @@ void launch_mark_used(hipStream_t s, const GatherArgs& a, const uint32_t* sorted
 void launch_scatter_used(hipStream_t s, const uint32_t* used, const uint32_t* sorted, int64_t n, uint32_t* used_order);
 // Morton keys of the half cells of the hit points; slots whose pixel id is
 // NO_HITPOINT get key 0xFFFFFFFF and sort behind every hit point
+// (radius from g.pd when set)
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
                      double radius, uint32_t* keys, uint32_t* vals);
 // gather layout of n photons: record k = photon perm[sorted[k]] (either may be
@@ -168,9 +186,15 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
                           const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C);
 
-// origin keys of the stored photons, and the permutation applied to the records
+// origin keys of the stored photons, and the permutation applied to the records;
+// with a device count, slots i >= min(*count, n) get key `sentinel`
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
-                        uint32_t* vals);
+                        uint32_t* vals, const unsigned long long* count = nullptr,
+                        uint32_t sentinel = 0xFFFFFFFFu);
+// pd->pass += 1, radius = radii[pass - 1], scale = 1 / (2 radius); sets
+// pd->overflow when counters->photons_stored exceeds cap (counters may be null)
+void launch_pass_advance(hipStream_t s, PassDev* pd, const double* radii);
+void launch_capacity_check(hipStream_t s, PassDev* pd, const Counters* counters, int64_t cap);
 void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,
                      const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out);
 void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,

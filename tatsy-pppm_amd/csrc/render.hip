@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n_photons;
          i += (int64_t)gridDim.x * 256) {
         Rng rng;
-        rng.init(a.seed, KIND_PHOTON, a.pass, (uint32_t)i, a.scene_id);
+        rng.init(a.seed, KIND_PHOTON, a.pd ? a.pd->pass : a.pass, (uint32_t)i, a.scene_id);
         // emission: light by power, uniform point, cosine direction
         float ul = rng.at(0);
         int l = 0;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
          p += (int64_t)gridDim.x * 256) {
         const int x = (int)(p % W), row = (int)(p / W);
         Rng rng;
-        rng.init(a.seed, KIND_EYE, a.pass, (uint32_t)p, a.scene_id);
+        rng.init(a.seed, KIND_EYE, a.pd ? a.pd->pass : a.pass, (uint32_t)p, a.scene_id);
         const float px = (float)x + rng.at(0);
         const float py = (float)(H - 1 - row) + rng.at(1);
         v3f d = normalize(D + U * (2.0f * px / (float)W - 1.0f) + V * (2.0f * py / (float)H - 1.0f));
@@ -267,11 +267,26 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
 }
 
 __global__ void k_origin_keys(const float4* __restrict__ ph_flux, int64_t n, uint32_t* keys,
-                              uint32_t* vals) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                              uint32_t* vals, const unsigned long long* count, uint32_t sentinel) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = __float_as_uint(ph_flux[i].w);
+    const bool real = !count || i < (int64_t)min((unsigned long long)n, *count);
+    keys[i] = real ? __float_as_uint(ph_flux[i].w) : sentinel;
     vals[i] = (uint32_t)i;
+}
+
+__global__ void k_pass_advance(PassDev* pd, const double* radii) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t p = pd->pass + 1;
+    const double r = radii[p - 1];
+    pd->pass = p;
+    pd->radius = r;
+    pd->scale = 1.0 / (2.0 * r);
+}
+
+__global__ void k_capacity_check(PassDev* pd, const Counters* c, int64_t cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if ((int64_t)c->photons_stored > cap || (int64_t)c->irr_stored > cap) pd->overflow = 1u;
 }
 
 __global__ void k_permute2(const float4* __restrict__ a_in, const float4* __restrict__ b_in,
@@ -327,9 +342,17 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
 }
 
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
-                        uint32_t* vals) {
+                        uint32_t* vals, const unsigned long long* count, uint32_t sentinel) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_origin_keys, dim3(grid1(n, 256)), dim3(256), 0, s, ph_flux, n, keys, vals);
+    hipLaunchKernelGGL(k_origin_keys, dim3(grid1(n, 256)), dim3(256), 0, s, ph_flux, n, keys, vals, count, sentinel);
+}
+
+void launch_pass_advance(hipStream_t s, PassDev* pd, const double* radii) {
+    hipLaunchKernelGGL(k_pass_advance, dim3(1), dim3(64), 0, s, pd, radii);
+}
+
+void launch_capacity_check(hipStream_t s, PassDev* pd, const Counters* counters, int64_t cap) {
+    hipLaunchKernelGGL(k_capacity_check, dim3(1), dim3(64), 0, s, pd, counters, cap);
 }
 
 void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,

@@ -119,6 +119,16 @@ _DECLS = {
     "pppm_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),
     "pppm_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
     "pppm_mlp_train_step": (C.c_int, [_P, _P, C.c_int64, C.c_double, C.c_int32, _P, C.POINTER(C.c_double)]),
+    "pppm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
+    "pppm_batch_last_error": (C.c_char_p, [_P]),
+    "pppm_batch_destroy": (C.c_int, [_P]),
+    "pppm_batch_add": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(Bsdf), C.c_int32, C.POINTER(Light), C.c_int32,
+                                 C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(C.c_int32)]),
+    "pppm_batch_tune": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "pppm_batch_run": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "pppm_batch_image": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "pppm_batch_scene_info": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pppm_scenegen": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SceneDesc)]),
     "pppm_scenegen_free": (None, [C.POINTER(SceneDesc)]),
 }
@@ -456,3 +466,65 @@ class Context:
         self._chk(self.L.pppm_dipole_eval(self.h, C.byref(material), _ptr(pts), _ptr(E), len(pts), area, _ptr(q),
                                           len(q), _ptr(out)))
         return out[:len(q)]
+
+
+class Batch:
+    """Many scenes rendered by one hipGraph-captured pass loop (pppm_batch_*)."""
+
+    def __init__(self, device=0, branches=4):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.pppm_batch_create(C.byref(h), device, branches)
+        if rc != OK:
+            raise PPPMError(rc, self.L.pppm_batch_last_error(None).decode())
+        self.h = h
+        self.cams = []
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise PPPMError(rc, self.L.pppm_batch_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.pppm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, scene, photons, passes, radius1, hash_size, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
+            max_bounces=64, capacity_factor=4.0, gather_mode=GATHER_LANES):
+        tri = np.ascontiguousarray(scene.tri, np.float32)
+        ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
+        p = RenderParams(photons=photons, passes=passes, max_bounces=max_bounces, alpha=alpha, radius1=radius1,
+                         ray_eps=ray_eps, seed=seed, scene_id=scene_id, hash_size=hash_size,
+                         capacity_factor=capacity_factor, gather_mode=gather_mode)
+        idx = C.c_int32()
+        self._chk(self.L.pppm_batch_add(self.h, _ptr(tri), _ptr(ids), len(ids), scene.bsdf_array(), len(scene.bsdfs),
+                                        scene.light_array(), len(scene.lights), C.byref(scene.camera), C.byref(p),
+                                        C.byref(idx)))
+        self.cams.append(scene.camera)
+        return idx.value
+
+    def tune(self, gather=True, grid=True):
+        self._chk(self.L.pppm_batch_tune(self.h, 1 if gather else 0, 1 if grid else 0))
+
+    def run(self, passes):
+        ms = C.c_double()
+        self._chk(self.L.pppm_batch_run(self.h, passes, C.byref(ms)))
+        return ms.value
+
+    def image(self, index):
+        cam = self.cams[index]
+        out = np.zeros((cam.height, cam.width, 3), np.float64)
+        self._chk(self.L.pppm_batch_image(self.h, index, _ptr(out), out.size))
+        return out
+
+    def info(self, index):
+        m, g, b = C.c_int32(), C.c_int32(), C.c_int32()
+        c = C.c_double()
+        self._chk(self.L.pppm_batch_scene_info(self.h, index, C.byref(m), C.byref(g), C.byref(b), C.byref(c)))
+        return {"gather_mode": m.value, "photon_grid": g.value, "branch": b.value, "cost_ms": c.value}
