@@ -259,6 +259,80 @@ def subsurface_study(P, np, width, photons, passes, points, frequency):
     }
 
 
+def batch_study(P, np, n_scenes, passes, seed, branches=4):
+    """Batched plain PPM (config C3): n_scenes procedural scenes, triangle
+    counts log-uniform in [1k, 500k], resolutions from {128, 256, 512, 1024}^2,
+    8 photons per pixel, in one hipGraph-captured pass loop after the
+    autotuner; against the same scenes rendered one after another with
+    pppm_render_pass."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n_scenes):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        kind = P.SCENE_RANDOM if k % 2 else P.SCENE_BOX
+        per = 20.0 if kind == P.SCENE_RANDOM else 40.0   # 20 f^2 triangles per icosphere
+        f = max(1, int(round((tris / per) ** 0.5)))
+        specs.append((kind, 1000 + k, f, res))
+    t0 = time.perf_counter()
+    b = P.Batch(branches=branches)
+    scenes = []
+    for k, (kind, sd, f, res) in enumerate(specs):
+        s = P.scenegen(kind, sd, f, res, res)
+        n = 8 * res * res
+        r1 = 0.01 * s.diagonal()
+        b.add(s, n, passes + 1, r1, hash_size=n, seed=0x900 + k, capacity_factor=4.0)
+        scenes.append((s, n, r1, 0x900 + k))
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.tune()
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.run(1)  # captures the graph and warms up
+    capture_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gpu_ms = b.run(passes)
+    wall = time.perf_counter() - t0
+    infos = [b.info(k) for k in range(n_scenes)]
+    for k in (0, n_scenes - 1):
+        assert np.all(np.isfinite(b.image(k)))
+    b.close()
+    photons = sum(n for _, n, _, _ in scenes)
+    # one scene at a time through pppm_render_pass, one warm-up and one timed pass
+    ctx = P.Context(0)
+    solo_ms = 0.0
+    for k, (s, n, r1, sd) in enumerate(scenes):
+        ctx.upload(s)
+        ctx.build_bvh()
+        ctx.set_camera(s.camera)
+        ctx.begin(n, 2, r1, seed=sd, hash_size=n, gather_mode=infos[k]["gather_mode"], capacity_factor=4.0)
+        ctx.render_pass()
+        t1 = time.perf_counter()
+        ctx.render_pass()
+        solo_ms += 1e3 * (time.perf_counter() - t1)
+    ctx.close()
+    modes = {P.GATHER_LANES: "lanes", P.GATHER_TILED: "tiled", P.GATHER_WAVE: "wave"}
+    return {
+        "workload": (f"{n_scenes} scenes, triangles log-uniform in [1k, 500k], resolutions from "
+                     "{128,256,512,1024}^2, 8 photons per pixel"),
+        "branches": branches,
+        "passes_timed": passes,
+        "photons_per_batch_pass": photons,
+        "triangles_total": int(sum(s.n_tris for s, _, _, _ in scenes)),
+        "pixels_total": int(sum(s.camera.width * s.camera.height for s, _, _, _ in scenes)),
+        "ms_per_batch_pass": 1e3 * wall / passes,
+        "gpu_ms_per_batch_pass": gpu_ms / passes,
+        "photons_per_s": photons * passes / wall,
+        "solo_ms_per_batch_pass": solo_ms,
+        "speedup_vs_solo": solo_ms / (1e3 * wall / passes),
+        "setup_s": setup_s,
+        "tune_s": tune_s,
+        "capture_and_first_pass_s": capture_s,
+        "tuned_gather": {v: sum(1 for i in infos if i["gather_mode"] == k) for k, v in modes.items()},
+        "tuned_photon_grid_waves_per_cu": sorted({i["photon_grid"] // 256 for i in infos}),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +353,9 @@ def main():
     ap.add_argument("--sss-photons", type=int, default=8_000_000)
     ap.add_argument("--sss-points", type=int, default=1 << 20)
     ap.add_argument("--sss-frequency", type=int, default=316)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-passes", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -378,6 +455,10 @@ def main():
                                              args.sss_points, args.sss_frequency)
     else:
         out["subsurface"] = None
+    if not args.no_batch:
+        out["batch"] = batch_study(P, np, args.batch_scenes, args.batch_passes, 7)
+    else:
+        out["batch"] = None
     if not args.no_cpu:
         cb, _ = cpu_baseline(scene, args.photons, r1, ray_eps, seed)
         out["cpu_baseline"] = cb
