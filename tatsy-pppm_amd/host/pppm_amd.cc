@@ -57,6 +57,20 @@ static BSDF make_bsdf(int type, const Vector3D& r) {
     b.eta = 1.3f;
     return BSDF(b);
 }
+BSSRDF DipoleBSSRDF::factory(const Vector3D& sigma_a, const Vector3D& sigmap_s, double eta, double scale) {
+    if (scale != 1.0) throw Error(PPPM_ERR_INVALID, "DipoleBSSRDF: only scale 1 is supported");
+    return BSSRDF(sigma_a, sigmap_s, eta);
+}
+
+void BSDF::setBssrdf(const BSSRDF& bssrdf) {
+    for (int k = 0; k < 3; k++) {
+        _b.sigma_a[k] = (float)bssrdf.sigma_a()[k];
+        _b.sigmap_s[k] = (float)bssrdf.sigmap_s()[k];
+    }
+    _b.eta = (float)bssrdf.eta();
+    _b.type |= PPPM_BSDF_BSSRDF;
+}
+
 BSDF LambertianBRDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_LAMBERTIAN, r); }
 BSDF SpecularBRDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_SPECULAR, r); }
 BSDF RefractionBSDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_REFRACTION, r); }
@@ -289,7 +303,10 @@ void ProgressivePhotonMappingProb::render(const Scene& scene, const Camera& came
     p.hash_size = _settings.hashSize;
     p.capacity_factor = 8.0;
     p.gather_mode = _settings.gatherMode;
+    dev->check(pppm_subsurface_configure(ctx, _settings.subsurfacePoints));
+    dev->check(pppm_camera_set(ctx, &camera.raw()));  // configure ends a running render
     dev->check(pppm_render_begin(ctx, &p));
+    if (_settings.learnedProposal) dev->check(pppm_proposal_enable(ctx, &_settings.proposal));
     _result = Image(camera.width(), camera.height());
     _stats.clear();
     for (int t = 1; t <= params.spp(); t++) {

@@ -149,6 +149,25 @@ enum BsdfType {
     BSDF_TYPE_EMITTER = PPPM_BSDF_EMITTER
 };
 
+// ---- sources/bssrdf.h:17-76 (DipoleBSSRDF) ---------------------------------
+// The dipole parameters; scale must be 1 (the reference's only use,
+// sources/main/appmain.cc:64).
+class BSSRDF {
+    Vector3D _sigma_a, _sigmap_s;
+    double _eta = 1.3;
+
+public:
+    BSSRDF() {}
+    BSSRDF(const Vector3D& sigma_a, const Vector3D& sigmap_s, double eta)
+        : _sigma_a(sigma_a), _sigmap_s(sigmap_s), _eta(eta) {}
+    const Vector3D& sigma_a() const { return _sigma_a; }
+    const Vector3D& sigmap_s() const { return _sigmap_s; }
+    double eta() const { return _eta; }
+};
+struct DipoleBSSRDF {
+    static BSSRDF factory(const Vector3D& sigma_a, const Vector3D& sigmap_s, double eta = 1.3, double scale = 1.0);
+};
+
 // Value type holding the material parameters the GPU table needs.
 class BSDF {
     pppm_bsdf _b;
@@ -159,6 +178,8 @@ public:
     Vector3D reflectance() const;
     BsdfType type() const { return (BsdfType)_b.type; }
     const pppm_bsdf& raw() const { return _b; }
+    // sources/bsdf.cc:70-73: attaches a BSSRDF and adds BSDF_TYPE_BSSRDF
+    void setBssrdf(const BSSRDF& bssrdf);
 };
 
 struct LambertianBRDF {
@@ -215,6 +236,9 @@ struct PPMSettings {
     int bounceLimit = 64;
     int hashSize = 0;
     int gatherMode = PPPM_GATHER_AUTO;
+    int64_t subsurfacePoints = 0;     // BSSRDF surface points (0: the reference's density, <= 2^20)
+    bool learnedProposal = false;     // the learned emission mixture (one light)
+    pppm_proposal_params proposal = {0.5, 2.0e-3, 16 << 20, 8, 8, 64, 16384, 0x5EED};
 };
 
 // ---- sources/image.h (FP64 RGB buffer) --------------------------------------

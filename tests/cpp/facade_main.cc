@@ -2,11 +2,12 @@
 // the way the reference's application and tests drive theirs
 // (sources/main/appmain.cc:11-26; tests/test_trimesh.cc:62-117).
 //
-//   facade_main <out_image.bin>
+//   facade_main <out_image.bin> [<sss_image.bin> <proposal_image.bin>]
 // Exits 0 when every check passes; prints one line per check.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 
 #include "../../tatsy-pppm_amd/host/pppm_amd.h"
@@ -147,6 +148,59 @@ int main(int argc, char** argv) {
         }
         std::printf("render: mean %.6f, %lld photons stored in pass 2\n", sum / img.data().size(),
                     (long long)ppm.passStats()[1].photons_stored);
+    }
+    // --- translucent material, set up as appmain.cc:63-65 does ----------------
+    if (argc > 2) {
+        Scene s3;
+        PerspectiveCamera cam = s3.loadProcedural(PPPM_SCENE_TWO_ROOMS_SSS, 1, 12, 48, 48);
+        // the scene's translucent material rebuilt through the factories
+        const BSDF& orig = s3.getBsdf((int)s3.numTriangles() - 1);
+        BSDF rebuilt = RefractionBSDF::factory(orig.reflectance());
+        rebuilt.setBssrdf(DipoleBSSRDF::factory(
+            Vector3D(orig.raw().sigma_a[0], orig.raw().sigma_a[1], orig.raw().sigma_a[2]),
+            Vector3D(orig.raw().sigmap_s[0], orig.raw().sigmap_s[1], orig.raw().sigmap_s[2]), orig.raw().eta));
+        CHECK(rebuilt.type() == (BSDF_TYPE_REFRACTION | BSDF_TYPE_BSSRDF) &&
+                  std::memcmp(&rebuilt.raw(), &orig.raw(), sizeof(pppm_bsdf)) == 0,
+              "DipoleBSSRDF::factory + setBssrdf rebuild the scene's material");
+        bool threw = false;
+        try {
+            DipoleBSSRDF::factory(Vector3D(0.01, 0.01, 0.01), Vector3D(1, 1, 1), 1.3, 2.0);
+        } catch (const Error&) {
+            threw = true;
+        }
+        CHECK(threw, "a dipole scale other than 1 is rejected");
+        PPMSettings set;
+        set.subsurfacePoints = 20000;
+        ProgressivePhotonMappingProb ppm(set);
+        ppm.render(s3, cam, RenderParameters(100000, 2, 0, 0.0));
+        CHECK(ppm.passStats()[1].sss_hits > 0 && ppm.passStats()[1].sss_points == 20000, "subsurface estimate ran");
+        FILE* f = std::fopen(argv[2], "wb");
+        if (f) {
+            std::fwrite(ppm.result().data().data(), sizeof(double), ppm.result().data().size(), f);
+            std::fclose(f);
+        }
+        std::printf("subsurface: %lld eye paths in the dipole sum\n", (long long)ppm.passStats()[1].sss_hits);
+    }
+    // --- the learned emission proposal ---------------------------------------
+    if (argc > 3) {
+        Scene s4;
+        PerspectiveCamera cam = s4.loadProcedural(PPPM_SCENE_TWO_ROOMS, 1, 8, 48, 48);
+        PPMSettings set;
+        set.learnedProposal = true;
+        set.proposal.anchors = 4;
+        set.proposal.train_interval = 2;
+        set.proposal.train_steps = 8;
+        set.proposal.batch = 4096;
+        set.proposal.ring_capacity = 1 << 20;
+        ProgressivePhotonMappingProb ppm(set);
+        ppm.render(s4, cam, RenderParameters(50000, 4, 0, 0.0));
+        CHECK(ppm.passStats()[1].retrained == 1 && ppm.passStats()[2].proposal_active == 1, "proposal trained and used");
+        FILE* f = std::fopen(argv[3], "wb");
+        if (f) {
+            std::fwrite(ppm.result().data().data(), sizeof(double), ppm.result().data().size(), f);
+            std::fclose(f);
+        }
+        std::printf("proposal: loss %.4f after pass 2\n", ppm.passStats()[1].train_loss);
     }
     std::printf(failures ? "FAILED\n" : "OK\n");
     return failures ? 1 : 0;
