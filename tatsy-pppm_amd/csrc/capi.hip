@@ -90,7 +90,8 @@ struct Ctx {
     double last_scale = 0.0;
     uint32_t* last_perm1 = nullptr;
     uint32_t* last_sorted = nullptr;
-    hipEvent_t ev[11] = {};
+    hipEvent_t ev[13] = {};
+    hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
     // subsurface scattering (BSSRDF materials)
     DBuf dips, sss_pts, sss_ptw, sss_E, irr_pos, irr_flux, sss_pos, sss_w;
     std::vector<int32_t> h_sss_index;  // material -> dipole index (-1: none)
@@ -561,14 +562,13 @@ hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius) {
 // The hit points among the cap slots of (hp, hw), compacted into hp_pos_s /
 // hp_w_s in Morton order of their half cells (edge r); a stable sort, so the
 // order is a function of the slot order alone.
-void sort_hitpoints(Ctx* c, const float4* hp, const float4* hw, int64_t cap, const GridParams& g,
+void sort_hitpoints(Ctx* c, hipStream_t s, const float4* hp, const float4* hw, int64_t cap, const GridParams& g,
                     double radius) {
     uint32_t* k[4];
     for (int i = 0; i < 4; i++) k[i] = c->hk[i].as<uint32_t>();
-    launch_hit_keys(c->stream, hp, hw, cap, g, radius, k[0], k[1]);
-    bool alt = radix_sort_pairs(c->stream, k[0], k[1], k[2], k[3], c->hhist.as<uint32_t>(), cap, 32);
-    launch_permute2(c->stream, hp, hw, nullptr, alt ? k[3] : k[1], cap, c->hp_pos_s.as<float4>(),
-                    c->hp_w_s.as<float4>());
+    launch_hit_keys(s, hp, hw, cap, g, radius, k[0], k[1]);
+    bool alt = radix_sort_pairs(s, k[0], k[1], k[2], k[3], c->hhist.as<uint32_t>(), cap, 32);
+    launch_permute2(s, hp, hw, nullptr, alt ? k[3] : k[1], cap, c->hp_pos_s.as<float4>(), c->hp_w_s.as<float4>());
 }
 
 void copy_ranges(const std::vector<uint32_t>& off, int32_t S, int32_t* cs, int32_t* ce) {
@@ -650,6 +650,12 @@ int pppm_create(void** out, int32_t device) {
         delete c;
         return PPPM_ERR_HIP;
     }
+    if (hipStreamCreate(&c->stream2) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PPPM_ERR_HIP;
+    }
     for (auto& ev : c->ev) (void)hipEventCreate(&ev);
     *out = c;
     return PPPM_OK;
@@ -666,7 +672,9 @@ int pppm_destroy(void* ctx_raw) {
     for (auto& b : c->u) b.release();
     c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
     c->img_sum.release(); c->img_comp.release(); c->counters.release(); c->scratch.release();
+    (void)hipStreamSynchronize(c->stream2);
     for (auto& ev : c->ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PPPM_OK;
@@ -1051,6 +1059,25 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
 
     launch_zero_u32(s, dc, sizeof(Counters) / 4);
     TRY_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    // the eye pass and the hit point sort need no photons: they run on the
+    // second stream, concurrent with the photon pass, so each kernel's tail of
+    // long paths overlaps the other's work
+    hipStream_t s2 = ctx->stream2;
+    TRY_HIP(ctx, hipStreamWaitEvent(s2, ctx->ev[0], 0));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[11], s2));
+    {
+        EyePassArgs ea = eye_args(ctx, (uint32_t)pass);
+        launch_eye_pass(s2, ea, grid_limit);
+        TRY_HIP(ctx, hipGetLastError());
+        TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s2));
+        GridParams gh;
+        for (int d = 0; d < 3; d++) gh.bmin[d] = ctx->grid_bmin[d];
+        gh.scale = 1.0 / (2.0 * radius);
+        gh.size = 1;
+        sort_hitpoints(ctx, s2, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, gh, radius);
+        TRY_HIP(ctx, hipGetLastError());
+        TRY_HIP(ctx, hipEventRecord(ctx->ev[5], s2));
+    }
     PhotonPassArgs pa;
     pa.sc = ctx->scene_dev();
     pa.n_photons = ctx->rp.photons;
@@ -1123,22 +1150,17 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    // eye pass
-    EyePassArgs ea = eye_args(ctx, (uint32_t)pass);
-    launch_eye_pass(s, ea, grid_limit);
-    TRY_HIP(ctx, hipGetLastError());
-    TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    // gather
+    // gather, once the hit points are sorted on the second stream
+    TRY_HIP(ctx, hipStreamWaitEvent(s, ctx->ev[5], 0));
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[12], s));
     GatherArgs ga;
     ga.mode = ctx->rp.gather_mode;
     ga.ph_a = ctx->ph_a.as<float4>();
     ga.ph_b = ctx->ph_b.as<float4>();
     ga.ph_c = ctx->ph_c.as<float4>();
     ga.off = ctx->off.as<uint32_t>();
-    sort_hitpoints(ctx, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, radius);
     ga.hp_pos = ctx->hp_pos_s.as<float4>();
     ga.hp_w = ctx->hp_w_s.as<float4>();
-    TRY_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     ga.hp_count_src = dc;
     ga.n_hit = npix;
     ga.g = g;
@@ -1228,8 +1250,15 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ctx->last_perm1 = perm1;
     ctx->last_sorted = sorted;
     if (st) {
+        // stream 1: photon (0-1), order (1-2), grid (2-3), gather (12-6);
+        // stream 2: eye (11-4), hit point sort (4-5)
         float t[6] = {0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < 6; k++) (void)hipEventElapsedTime(&t[k], ctx->ev[k], ctx->ev[k + 1]);
+        (void)hipEventElapsedTime(&t[0], ctx->ev[0], ctx->ev[1]);
+        (void)hipEventElapsedTime(&t[1], ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&t[2], ctx->ev[2], ctx->ev[3]);
+        (void)hipEventElapsedTime(&t[3], ctx->ev[11], ctx->ev[4]);
+        (void)hipEventElapsedTime(&t[4], ctx->ev[4], ctx->ev[5]);
+        (void)hipEventElapsedTime(&t[5], ctx->ev[12], ctx->ev[6]);
         float tot = 0.0f;
         (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[6]);
         st->hash_size = (int32_t)S;
@@ -1419,7 +1448,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     ga.off = ctx->off.as<uint32_t>();
     if (n_hit > 0) {  // every kernel reads the hit points in Morton order, as in a pass
         TRY_HIP(ctx, ensure_hit_sort(ctx, n_hit));
-        sort_hitpoints(ctx, h4, w4, n_hit, g, radius);
+        sort_hitpoints(ctx, ctx->stream, h4, w4, n_hit, g, radius);
         ga.hp_pos = ctx->hp_pos_s.as<float4>();
         ga.hp_w = ctx->hp_w_s.as<float4>();
     } else {
@@ -1919,7 +1948,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
     launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * 64 : 16384);
-    sort_hitpoints(ctx, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, 0.0);
+    sort_hitpoints(ctx, ctx->stream, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, 0.0);
     GatherArgs ga;
     std::memset(&ga, 0, sizeof(ga));
     ga.mode = ctx->rp.gather_mode;
@@ -2078,7 +2107,7 @@ int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid) {
         else modes = {base.gather_mode};
         std::vector<int> grids;
         const int cu = c->cu > 0 ? c->cu : 256;
-        if (tune_grid) grids = {cu * 16, cu * 32, cu * 64, cu * 128};
+        if (tune_grid) grids = {cu * 4, cu * 8, cu * 16, cu * 32, cu * 64};
         else grids = {sc.photon_grid};
         double best = 1e30;
         int best_mode = base.gather_mode, best_grid = sc.photon_grid;
