@@ -383,10 +383,10 @@ def main():
     ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed, gather_mode=mode)
     for _ in range(args.warmup):
         ctx.render_pass()
-    stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(ctx.render_pass())  # synchronous: returns after the pass completes
+    # pipelined: the gather of pass i overlaps the photons of pass i + 1;
+    # returns after the last pass completes
+    stats = ctx.render_passes(args.steps)
     elapsed = time.perf_counter() - t0
     img = ctx.image()
     assert np.all(np.isfinite(img))

@@ -191,6 +191,12 @@ int pppm_camera_set(void* ctx, const pppm_camera* cam);
 int pppm_render_begin(void* ctx, const pppm_render_params* params);
 /* Runs pass i = (passes done) + 1 with r_i from the schedule. */
 int pppm_render_pass(void* ctx, pppm_pass_stats* stats);
+/* Runs `count` passes pipelined on three streams: the gather of pass i
+ * overlaps the photon tracing of pass i + 1 (the image equals `count`
+ * pppm_render_pass calls bit for bit).  stats (may be NULL) gets one entry per
+ * pass.  With the learned proposal or BSSRDF materials the passes run one by
+ * one. */
+int pppm_render_passes(void* ctx, int32_t count, pppm_pass_stats* stats);
 /* FP64 image [height][width][3], row 0 at the top, averaged over the passes
  * done: (sum - compensation) / passes. */
 int pppm_image_read(void* ctx, double* rgb, int64_t n_values);

@@ -79,7 +79,10 @@ struct Ctx {
     double grid_bmin[3] = {0, 0, 0};
     // ph_pos / ph_flux: photons as the photon pass appended them; ph_a/b/c: the
     // gather layout in bucket order
-    DBuf ph_pos, ph_flux, ph_a, ph_b, ph_c, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp,
+    // img_sum / img_comp: the gather (and the dipole sum); img_dsum / img_dcomp:
+    // emission seen directly by eye paths (its own Kahan pair, so the eye pass
+    // never writes the gather's accumulator)
+    DBuf ph_pos, ph_flux, ph_a, ph_b, ph_c, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp, img_dsum, img_dcomp,
         counters, scratch;
     // hit points in Morton order of their half cells (tiled gather)
     DBuf hp_pos_s, hp_w_s, hk[4], hhist;
@@ -92,6 +95,10 @@ struct Ctx {
     uint32_t* last_sorted = nullptr;
     hipEvent_t ev[13] = {};
     hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
+    hipStream_t stream3 = nullptr;  // the gather of pipelined passes (pppm_render_passes)
+    hipEvent_t pev[2][11] = {};     // per buffer set: events of a pipelined pass
+    // the second buffer set of pipelined passes: what a pass's gather reads
+    DBuf ph_a_alt, ph_b_alt, ph_c_alt, off_alt, hp_pos_s_alt, hp_w_s_alt, counters_alt, pass_counters;
     // subsurface scattering (BSSRDF materials)
     DBuf dips, sss_pts, sss_ptw, sss_E, irr_pos, irr_flux, sss_pos, sss_w;
     std::vector<int32_t> h_sss_index;  // material -> dipole index (-1: none)
@@ -609,8 +616,8 @@ EyePassArgs eye_args(Ctx* ctx, uint32_t pass) {
     ea.ray_eps = (float)ctx->rp.ray_eps;
     ea.hp_pos = ctx->hp_pos.as<float4>();
     ea.hp_w = ctx->hp_w.as<float4>();
-    ea.img_sum = ctx->img_sum.as<float>();
-    ea.img_comp = ctx->img_comp.as<float>();
+    ea.img_sum = ctx->img_dsum.as<float>();
+    ea.img_comp = ctx->img_dcomp.as<float>();
     ea.counters = ctx->counters.as<Counters>();
     ea.pd = nullptr;
     ea.sss_pos = ctx->sss_present ? ctx->sss_pos.as<float4>() : nullptr;
@@ -650,13 +657,16 @@ int pppm_create(void** out, int32_t device) {
         delete c;
         return PPPM_ERR_HIP;
     }
-    if (hipStreamCreate(&c->stream2) != hipSuccess) {
+    if (hipStreamCreate(&c->stream2) != hipSuccess || hipStreamCreate(&c->stream3) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PPPM_ERR_HIP;
     }
     for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+    for (auto& set : c->pev)
+        for (auto& ev : set) (void)hipEventCreate(&ev);
     *out = c;
     return PPPM_OK;
 }
@@ -671,9 +681,14 @@ int pppm_destroy(void* ctx_raw) {
     c->ph_pos.release(); c->ph_flux.release(); c->ph_a.release(); c->ph_b.release(); c->ph_c.release();
     for (auto& b : c->u) b.release();
     c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
-    c->img_sum.release(); c->img_comp.release(); c->counters.release(); c->scratch.release();
+    c->img_sum.release(); c->img_comp.release(); c->img_dsum.release(); c->img_dcomp.release();
+    c->counters.release(); c->scratch.release();
     (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream3);
     for (auto& ev : c->ev) (void)hipEventDestroy(ev);
+    for (auto& set : c->pev)
+        for (auto& ev : set) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream3);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1024,9 +1039,13 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     TRY_HIP(ctx, ensure_hit_sort(ctx, npix));
     TRY_HIP(ctx, ctx->img_sum.ensure(sizeof(float) * 3 * (size_t)npix));
     TRY_HIP(ctx, ctx->img_comp.ensure(sizeof(float) * 3 * (size_t)npix));
+    TRY_HIP(ctx, ctx->img_dsum.ensure(sizeof(float) * 3 * (size_t)npix));
+    TRY_HIP(ctx, ctx->img_dcomp.ensure(sizeof(float) * 3 * (size_t)npix));
     TRY_HIP(ctx, ctx->counters.ensure(sizeof(Counters)));
     launch_zero_u32(ctx->stream, ctx->img_sum.p, 3 * npix);
     launch_zero_u32(ctx->stream, ctx->img_comp.p, 3 * npix);
+    launch_zero_u32(ctx->stream, ctx->img_dsum.p, 3 * npix);
+    launch_zero_u32(ctx->stream, ctx->img_dcomp.p, 3 * npix);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     TRY_HIP(ctx, subsurface_setup(ctx, nullptr));
@@ -1297,18 +1316,247 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     return PPPM_OK;
 }
 
+// Pipelined passes: three streams -- photons and sorts (stream), eye pass and
+// hit point sort (stream2), gather (stream3) -- so that the gather of pass i
+// runs while the photons of pass i + 1 are traced.  What a gather reads (the
+// gather layout, bucket offsets, sorted hit points, counters) alternates
+// between two buffer sets; everything else is single and ordered by its
+// stream.  The gathers run in pass order, so the image equals that of the
+// same number of pppm_render_pass calls bit for bit.
+int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
+    CHECK_CTX(ctx);
+    if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "pppm_render_begin not called");
+    if (count < 0 || ctx->passes_done + count > ctx->rp.passes)
+        return ctx->fail(PPPM_ERR_STATE, "more passes than the render was begun with");
+    if (count == 0) return PPPM_OK;
+    if (ctx->prop.enabled || ctx->prop.active || ctx->sss_present) {
+        // the proposal and the subsurface estimate work between passes on the
+        // host: one pass at a time
+        for (int32_t k = 0; k < count; k++) {
+            int rc = pppm_render_pass(ctx_raw, stats ? &stats[k] : nullptr);
+            if (rc != PPPM_OK) return rc;
+        }
+        return PPPM_OK;
+    }
+    (void)hipSetDevice(ctx->device);
+    TRY_HIP(ctx, ctx->ph_a_alt.ensure(ctx->ph_a.n));
+    TRY_HIP(ctx, ctx->ph_b_alt.ensure(ctx->ph_b.n));
+    TRY_HIP(ctx, ctx->ph_c_alt.ensure(ctx->ph_c.n));
+    TRY_HIP(ctx, ctx->off_alt.ensure(ctx->off.n));
+    TRY_HIP(ctx, ctx->hp_pos_s_alt.ensure(ctx->hp_pos_s.n));
+    TRY_HIP(ctx, ctx->hp_w_s_alt.ensure(ctx->hp_w_s.n));
+    TRY_HIP(ctx, ctx->counters_alt.ensure(ctx->counters.n));
+    TRY_HIP(ctx, ctx->pass_counters.ensure(sizeof(Counters) * (size_t)count));
+    // the alternate set must be sized as the current one for every later pass
+    if (ctx->off_alt.n < ctx->off.n || ctx->ph_a_alt.n < ctx->ph_a.n)
+        return ctx->fail(PPPM_ERR_NOMEM, "second buffer set");
+    auto swap_buf = [](DBuf& a, DBuf& b) {
+        std::swap(a.p, b.p);
+        std::swap(a.n, b.n);
+    };
+    hipStream_t A = ctx->stream, B = ctx->stream2, Cs = ctx->stream3;
+    enum { ZERO, P0, P1, O0, ODONE, E0, E1, HDONE, G0, GEND, GDONE };
+    const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
+    const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    std::vector<Counters> hc_pass((size_t)count);
+    std::vector<float> times((size_t)count * 6, 0.0f);
+    std::vector<int32_t> sizes((size_t)count, 0);
+    std::vector<double> rads((size_t)count, 0.0);
+    auto read_times = [&](int32_t k) {
+        hipEvent_t* E = ctx->pev[k & 1];
+        (void)hipEventSynchronize(E[GDONE]);
+        float* t = &times[(size_t)k * 6];
+        (void)hipEventElapsedTime(&t[0], E[P0], E[P1]);
+        (void)hipEventElapsedTime(&t[1], E[O0], E[ODONE]);
+        (void)hipEventElapsedTime(&t[2], E[E0], E[E1]);
+        (void)hipEventElapsedTime(&t[3], E[E1], E[HDONE]);
+        (void)hipEventElapsedTime(&t[4], E[G0], E[GEND]);
+        (void)hipEventElapsedTime(&t[5], E[ZERO], E[GDONE]);
+    };
+    for (int32_t k = 0; k < count; k++) {
+        if (k > 0) {
+            swap_buf(ctx->ph_a, ctx->ph_a_alt);
+            swap_buf(ctx->ph_b, ctx->ph_b_alt);
+            swap_buf(ctx->ph_c, ctx->ph_c_alt);
+            swap_buf(ctx->off, ctx->off_alt);
+            swap_buf(ctx->hp_pos_s, ctx->hp_pos_s_alt);
+            swap_buf(ctx->hp_w_s, ctx->hp_w_s_alt);
+            swap_buf(ctx->counters, ctx->counters_alt);
+        }
+        hipEvent_t* E = ctx->pev[k & 1];
+        if (k >= 2) {
+            // this buffer set was last read by the gather of pass k - 2
+            read_times(k - 2);
+            TRY_HIP(ctx, hipStreamWaitEvent(A, E[GDONE], 0));
+            TRY_HIP(ctx, hipStreamWaitEvent(B, E[GDONE], 0));
+        }
+        const int32_t pass = ctx->passes_done + 1;
+        const double radius = ctx->radii[(size_t)pass - 1];
+        rads[(size_t)k] = radius;
+        Counters* dc = ctx->counters.as<Counters>();
+        launch_zero_u32(A, dc, sizeof(Counters) / 4);
+        TRY_HIP(ctx, hipEventRecord(E[ZERO], A));
+        // eye side
+        TRY_HIP(ctx, hipStreamWaitEvent(B, E[ZERO], 0));
+        TRY_HIP(ctx, hipEventRecord(E[E0], B));
+        EyePassArgs ea = eye_args(ctx, (uint32_t)pass);
+        launch_eye_pass(B, ea, grid_limit);
+        TRY_HIP(ctx, hipEventRecord(E[E1], B));
+        GridParams gh;
+        for (int d = 0; d < 3; d++) gh.bmin[d] = ctx->grid_bmin[d];
+        gh.scale = 1.0 / (2.0 * radius);
+        gh.size = 1;
+        sort_hitpoints(ctx, B, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, gh, radius);
+        TRY_HIP(ctx, hipEventRecord(E[HDONE], B));
+        // photon side
+        TRY_HIP(ctx, hipEventRecord(E[P0], A));
+        PhotonPassArgs pa;
+        pa.sc = ctx->scene_dev();
+        pa.n_photons = ctx->rp.photons;
+        pa.cap = ctx->cap;
+        pa.seed = ctx->rp.seed;
+        pa.pass = (uint32_t)pass;
+        pa.scene_id = ctx->rp.scene_id;
+        pa.max_bounces = ctx->rp.max_bounces;
+        pa.ray_eps = (float)ctx->rp.ray_eps;
+        pa.ph_pos = ctx->ph_pos.as<float4>();
+        pa.ph_flux = ctx->ph_flux.as<float4>();
+        pa.counters = dc;
+        pa.prop.cdf = nullptr;
+        pa.prop.lambda = 1.0f;
+        pa.prop.side = 0;
+        pa.irr_pos = nullptr;
+        pa.irr_flux = nullptr;
+        pa.irr_cap = 0;
+        launch_photon_pass(A, pa, grid_limit);
+        TRY_HIP(ctx, hipGetLastError());
+        TRY_HIP(ctx, hipEventRecord(E[P1], A));
+        // the stored count sizes the sorts; meanwhile the previous gather runs
+        Counters& hc = hc_pass[(size_t)k];
+        TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, A));
+        TRY_HIP(ctx, hipEventSynchronize(E[P1]));
+        TRY_HIP(ctx, hipStreamSynchronize(A));
+        const int64_t n = (int64_t)hc.photons_stored;
+        if (n > ctx->cap) {
+            (void)hipDeviceSynchronize();
+            return ctx->fail(PPPM_ERR_CAPACITY, "photon buffer holds " + std::to_string(ctx->cap) +
+                                                    " records, the pass stored " + std::to_string(n));
+        }
+        TRY_HIP(ctx, hipEventRecord(E[O0], A));
+        uint32_t* u0 = ctx->u[0].as<uint32_t>();
+        uint32_t* u1 = ctx->u[1].as<uint32_t>();
+        launch_origin_keys(A, ctx->ph_flux.as<float4>(), n, u0, u1);
+        const int obits = bits_for(((uint64_t)(ctx->rp.photons - 1) << 6) | 63u);
+        hipStream_t keep_stream = ctx->stream;
+        const bool alt = radix_sort_pairs(A, u0, u1, ctx->u[2].as<uint32_t>(), ctx->u[3].as<uint32_t>(),
+                                          ctx->hist.as<uint32_t>(), n, obits);
+        const int keep = alt ? 3 : 1;
+        uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
+        const uint32_t S = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
+        sizes[(size_t)k] = (int32_t)S;
+        GridParams g;
+        for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
+        g.scale = 1.0 / (2.0 * radius);
+        g.size = S;
+        ctx->stream = A;
+        uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
+        ctx->stream = keep_stream;
+        launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
+                             ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
+                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
+        TRY_HIP(ctx, hipGetLastError());
+        TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
+        // gather side, in pass order
+        TRY_HIP(ctx, hipStreamWaitEvent(Cs, E[ODONE], 0));
+        TRY_HIP(ctx, hipStreamWaitEvent(Cs, E[HDONE], 0));
+        TRY_HIP(ctx, hipEventRecord(E[G0], Cs));
+        GatherArgs ga;
+        ga.mode = ctx->rp.gather_mode;
+        ga.ph_a = ctx->ph_a.as<float4>();
+        ga.ph_b = ctx->ph_b.as<float4>();
+        ga.ph_c = ctx->ph_c.as<float4>();
+        ga.off = ctx->off.as<uint32_t>();
+        ga.hp_pos = ctx->hp_pos_s.as<float4>();
+        ga.hp_w = ctx->hp_w_s.as<float4>();
+        ga.hp_count_src = dc;
+        ga.n_hit = npix;
+        ga.g = g;
+        ga.radius = radius;
+        ga.img_sum = ctx->img_sum.as<float>();
+        ga.img_comp = ctx->img_comp.as<float>();
+        ga.out_rgb = nullptr;
+        ga.counters = dc;
+        ga.used = nullptr;
+        launch_gather(Cs, ga);
+        TRY_HIP(ctx, hipGetLastError());
+        TRY_HIP(ctx, hipEventRecord(E[GEND], Cs));
+        TRY_HIP(ctx, hipMemcpyAsync(ctx->pass_counters.as<Counters>() + k, dc, sizeof(Counters),
+                                    hipMemcpyDeviceToDevice, Cs));
+        TRY_HIP(ctx, hipEventRecord(E[GDONE], Cs));
+        ctx->passes_done = pass;
+        ctx->last_n = n;
+        ctx->last_n_irr = 0;
+        ctx->last_S = (int32_t)S;
+        ctx->last_scale = g.scale;
+        ctx->last_perm1 = perm1;
+        ctx->last_sorted = sorted;
+    }
+    for (int32_t k = std::max(0, count - 2); k < count; k++) read_times(k);
+    TRY_HIP(ctx, hipStreamSynchronize(Cs));
+    TRY_HIP(ctx, hipStreamSynchronize(B));
+    std::vector<Counters> dcs((size_t)count);
+    TRY_HIP(ctx, hipMemcpy(dcs.data(), ctx->pass_counters.p, sizeof(Counters) * (size_t)count, hipMemcpyDeviceToHost));
+    for (int32_t k = 0; k < count; k++)
+        if (dcs[(size_t)k].stack_overflow) return ctx->fail(PPPM_ERR_HIP, "traversal stack overflow");
+    ctx->last_valid = true;
+    if (stats) {
+        for (int32_t k = 0; k < count; k++) {
+            pppm_pass_stats& st = stats[k];
+            const Counters& c = dcs[(size_t)k];
+            const float* t = &times[(size_t)k * 6];
+            std::memset(&st, 0, sizeof(st));
+            st.pass_index = ctx->passes_done - count + k + 1;
+            st.hash_size = sizes[(size_t)k];
+            st.radius = rads[(size_t)k];
+            st.photons_emitted = ctx->rp.photons;
+            st.photons_stored = (int64_t)c.photons_stored;
+            st.photon_segments = (int64_t)c.photon_segments;
+            st.eye_segments = (int64_t)c.eye_segments;
+            st.hitpoints = (int64_t)c.hitpoints;
+            st.gather_buckets = (int64_t)c.gather_buckets;
+            st.gather_candidates = (int64_t)c.gather_candidates;
+            st.gather_inradius = (int64_t)c.gather_inradius;
+            st.photon_ms = t[0];
+            st.order_ms = t[1];
+            st.grid_ms = 0.0;
+            st.eye_ms = t[2];
+            st.hitsort_ms = t[3];
+            st.gather_ms = t[4];
+            st.total_ms = t[5];
+        }
+    }
+    return PPPM_OK;
+}
+
 int pppm_image_read(void* ctx_raw, double* rgb, int64_t n_values) {
     CHECK_CTX(ctx);
     if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "no render in progress");
     const int64_t nv = 3 * (int64_t)ctx->cam.width * ctx->cam.height;
     if (!rgb || n_values < nv) return ctx->fail(PPPM_ERR_INVALID, "image buffer too small");
     (void)hipSetDevice(ctx->device);
-    std::vector<float> s((size_t)nv), c((size_t)nv);
+    std::vector<float> s((size_t)nv), c((size_t)nv), ds((size_t)nv), dc((size_t)nv);
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream2));
     TRY_HIP(ctx, hipMemcpyAsync(s.data(), ctx->img_sum.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipMemcpyAsync(c.data(), ctx->img_comp.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(ds.data(), ctx->img_dsum.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipMemcpyAsync(dc.data(), ctx->img_dcomp.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // FP64: the compensated sums of both accumulators, averaged over the passes
     const double inv = ctx->passes_done > 0 ? 1.0 / (double)ctx->passes_done : 0.0;
-    for (int64_t i = 0; i < nv; i++) rgb[i] = ((double)s[(size_t)i] - (double)c[(size_t)i]) * inv;
+    for (int64_t i = 0; i < nv; i++) {
+        const size_t k = (size_t)i;
+        rgb[i] = (((double)s[k] - (double)c[k]) + ((double)ds[k] - (double)dc[k])) * inv;
+    }
     return PPPM_OK;
 }
 

@@ -98,6 +98,7 @@ _DECLS = {
     "pppm_camera_set": (C.c_int, [_P, C.POINTER(Camera)]),
     "pppm_render_begin": (C.c_int, [_P, C.POINTER(RenderParams)]),
     "pppm_render_pass": (C.c_int, [_P, C.POINTER(PassStats)]),
+    "pppm_render_passes": (C.c_int, [_P, C.c_int32, _P]),
     "pppm_image_read": (C.c_int, [_P, _P, C.c_int64]),
     "pppm_last_photons": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pppm_last_grid": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, C.POINTER(C.c_double)]),
@@ -319,6 +320,11 @@ class Context:
         st = PassStats()
         self._chk(self.L.pppm_render_pass(self.h, C.byref(st)))
         return st.as_dict()
+
+    def render_passes(self, count):
+        sts = (PassStats * max(count, 1))()
+        self._chk(self.L.pppm_render_passes(self.h, count, sts))
+        return [sts[k].as_dict() for k in range(count)]
 
     def image(self):
         w, h = self.camera.width, self.camera.height

@@ -333,3 +333,32 @@ def test_full_size_pass_properties(gpu_ctx):
     # within each bucket the insertion order is kept (stable)
     same = bucket_of_pos[1:] == bucket_of_pos[:-1]
     assert np.all(sidx[1:][same] > sidx[:-1][same])
+
+
+@pytest.mark.parametrize("mode", [P.GATHER_LANES, P.GATHER_TILED])
+def test_pipelined_passes_equal_single_passes(gpu_ctx, mode):
+    # pppm_render_passes overlaps the gather of pass i with the photons of pass
+    # i + 1 on three streams; image, photons and counts equal single passes
+    say(f"pipelined passes, gather mode {mode}")
+    s = P.scenegen(P.SCENE_BOX_CAUSTIC, 3, 10, 96, 64)
+    outs = []
+    for pipelined in (False, True):
+        gpu_ctx.upload(s)
+        gpu_ctx.build_bvh()
+        gpu_ctx.set_camera(s.camera)
+        gpu_ctx.begin(60000, 7, 0.01 * s.diagonal(), ray_eps=EPS, gather_mode=mode)
+        if pipelined:
+            sts = gpu_ctx.render_passes(2) + gpu_ctx.render_passes(4) + gpu_ctx.render_passes(1)
+        else:
+            sts = [gpu_ctx.render_pass() for _ in range(7)]
+        outs.append((gpu_ctx.image(), gpu_ctx.last_photons(), sts, gpu_ctx.last_hitpoints()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(a, b)
+    for a, b in zip(outs[0][3], outs[1][3]):
+        assert np.array_equal(a, b)
+    for x, y in zip(outs[0][2], outs[1][2]):
+        for k in ("pass_index", "photons_stored", "photon_segments", "eye_segments", "hitpoints", "gather_candidates",
+                  "gather_inradius", "hash_size"):
+            assert x[k] == y[k], k
+        assert y["gather_ms"] > 0 and y["photon_ms"] > 0
