@@ -390,6 +390,11 @@ def main():
     elapsed = time.perf_counter() - t0
     img = ctx.image()
     assert np.all(np.isfinite(img))
+    # the gather alone (single passes: nothing runs beside it), untimed
+    ctx.begin(args.photons, 3, r1, ray_eps=ray_eps, seed=seed + 1, gather_mode=mode)
+    iso = [ctx.render_pass() for _ in range(3)][1:]
+    iso_ms = sum(s["gather_ms"] for s in iso) / len(iso)
+    iso_bytes = sum(gather_bytes(s) for s in iso) / len(iso)
 
     photons_per_s = args.photons * args.steps / elapsed
     segs = sum(s["photon_segments"] + s["eye_segments"] for s in stats)
@@ -430,6 +435,9 @@ def main():
             "traffic": read_traffic(),
             "algorithmic_bytes_per_launch": gbytes,
             "avg_launch_ms": gather_ms,
+            "note": "timed region runs passes pipelined: the gather shares the GPU with the next pass's photons",
+            "isolated": {"achieved": iso_bytes / (iso_ms * 1e-3) / 1e9, "avg_launch_ms": iso_ms,
+                         "frac": iso_bytes / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
         },
         "rays_per_s": segs / (trace_ms * 1e-3),
         "photon_kernel_photons_per_s": args.photons / (photon_kernel_ms * 1e-3),
