@@ -310,6 +310,7 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
         t1 = time.perf_counter()
         ctx.render_pass()
         solo_ms += 1e3 * (time.perf_counter() - t1)
+    cu = max(1, ctx.device_info()["cu"])
     ctx.close()
     modes = {P.GATHER_LANES: "lanes", P.GATHER_TILED: "tiled", P.GATHER_WAVE: "wave"}
     return {
@@ -329,7 +330,7 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
         "tune_s": tune_s,
         "capture_and_first_pass_s": capture_s,
         "tuned_gather": {v: sum(1 for i in infos if i["gather_mode"] == k) for k, v in modes.items()},
-        "tuned_photon_grid_waves_per_cu": sorted({i["photon_grid"] // 256 for i in infos}),
+        "tuned_photon_blocks_per_cu": sorted({i["photon_grid"] // cu for i in infos}),
     }
 
 

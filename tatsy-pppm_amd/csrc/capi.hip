@@ -1073,6 +1073,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     const double radius = ctx->radii[(size_t)pass - 1];
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
     const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
     Counters* dc = ctx->counters.as<Counters>();
     ctx->last_valid = false;
 
@@ -1116,7 +1117,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     pa.irr_pos = ctx->sss_present ? ctx->irr_pos.as<float4>() : nullptr;
     pa.irr_flux = ctx->sss_present ? ctx->irr_flux.as<float4>() : nullptr;
     pa.irr_cap = ctx->sss_present ? ctx->cap : 0;
-    launch_photon_pass(s, pa, grid_limit);
+    launch_photon_pass(s, pa, photon_grid);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     Counters hc;
@@ -1358,6 +1359,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     enum { ZERO, P0, P1, O0, ODONE, E0, E1, HDONE, G0, GEND, GDONE };
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
     const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
     std::vector<Counters> hc_pass((size_t)count);
     std::vector<float> times((size_t)count * 6, 0.0f);
     std::vector<int32_t> sizes((size_t)count, 0);
@@ -1428,7 +1430,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         pa.irr_pos = nullptr;
         pa.irr_flux = nullptr;
         pa.irr_cap = 0;
-        launch_photon_pass(A, pa, grid_limit);
+        launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
         // the stored count sizes the sorts; meanwhile the previous gather runs
@@ -2333,7 +2335,7 @@ int pppm_batch_add(void* batch, const float* tri_xyz, const int32_t* bsdf_id, in
     }
     BatchScene sc;
     sc.ctx = c;
-    sc.photon_grid = c->cu > 0 ? c->cu * 64 : 16384;
+    sc.photon_grid = c->cu > 0 ? c->cu * PHOTON_BLOCKS_PER_CU : 2048;
     b->scenes.push_back(sc);
     b->drop_graph();
     if (index) *index = (int32_t)b->scenes.size() - 1;
@@ -2355,7 +2357,7 @@ int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid) {
         else modes = {base.gather_mode};
         std::vector<int> grids;
         const int cu = c->cu > 0 ? c->cu : 256;
-        if (tune_grid) grids = {cu * 4, cu * 8, cu * 16, cu * 32, cu * 64};
+        if (tune_grid) grids = {cu, cu * 2, cu * 4, cu * 8};
         else grids = {sc.photon_grid};
         double best = 1e30;
         int best_mode = base.gather_mode, best_grid = sc.photon_grid;

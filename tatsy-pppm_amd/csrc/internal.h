@@ -94,7 +94,12 @@ struct Counters {
     unsigned long long proposal_records;
     unsigned long long irr_stored;   // photons stored on BSSRDF surfaces
     unsigned long long sss_hits;     // eye paths ending in the subsurface estimate
+    unsigned long long photon_next;  // next photon index handed to a photon-pass lane
 };
+
+// Persistent photon-pass blocks per CU: at least the blocks one CU holds at
+// once (the kernel regenerates paths, so extra blocks find no work).
+#define PHOTON_BLOCKS_PER_CU 8
 
 struct PhotonPassArgs {
     SceneDev sc;

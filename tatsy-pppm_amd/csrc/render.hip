@@ -86,86 +86,126 @@ __device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
 
 #define TRACE_TMAX 1.0e30f
 
+// Emission of photon i: light by power, uniform point, cosine direction (or
+// the learned proposal's mixture); returns the ray origin, direction, flux.
+__device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, Rng& rng, v3f& o, v3f& dir,
+                                   v3f& flux) {
+    float ul = rng.at(0);
+    int l = 0;
+    while (l < sc.n_lights - 1 && sc.lights[l].cdf_hi <= ul) l++;
+    const LightDev L = sc.lights[l];
+    const float pu = rng.at(1), pv = rng.at(2);
+    const v3f pos = L.corner + L.eu * pu + L.ev * pv;
+    flux = L.flux;
+    const float r3 = rng.at(3), r4 = rng.at(4);
+    if (a.prop.side == 0) {
+        dir = on_hemisphere(L.normal, r3, r4);
+    } else {
+        const float sel = rng.at(PH_DIM_PROP_SEL);
+        const float j6 = rng.at(PH_DIM_PROP_JIT), j7 = rng.at(PH_DIM_PROP_JIT + 1);
+        const PropPick k = prop_pick(a.prop, pu, pv, r3, r4, sel, j6, j7);
+        dir = k.phi_u < 0.0f ? on_hemisphere(L.normal, r3, r4) : cell_direction(L.normal, k.cos_t, k.phi_u);
+        flux = flux * prop_weight(a.prop, pu, pv, k);
+    }
+    o = pos + L.normal * a.ray_eps;
+}
+
+// One lane per photon path, with path regeneration: every loop iteration
+// traces one segment on every lane, and a lane whose path has ended takes the
+// next photon index from a pass-wide counter (one atomic per wave), so lanes
+// do not idle behind the longest path of their wave.  A photon's random
+// numbers are keyed by its index, so its path and records do not depend on
+// the lane that traces it.
 __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
     unsigned long long segs = 0;
     bool ovf = false;
     const SceneDev& sc = a.sc;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n_photons;
-         i += (int64_t)gridDim.x * 256) {
-        Rng rng;
-        rng.init(a.seed, KIND_PHOTON, a.pd ? a.pd->pass : a.pass, (uint32_t)i, a.scene_id);
-        // emission: light by power, uniform point, cosine direction
-        float ul = rng.at(0);
-        int l = 0;
-        while (l < sc.n_lights - 1 && sc.lights[l].cdf_hi <= ul) l++;
-        const LightDev L = sc.lights[l];
-        const float pu = rng.at(1), pv = rng.at(2);
-        v3f pos = L.corner + L.eu * pu + L.ev * pv;
-        v3f dir;
-        v3f flux = L.flux;
-        const float r3 = rng.at(3), r4 = rng.at(4);
-        if (a.prop.side == 0) {
-            dir = on_hemisphere(L.normal, r3, r4);
-        } else {
-            const float sel = rng.at(PH_DIM_PROP_SEL);
-            const float j6 = rng.at(PH_DIM_PROP_JIT), j7 = rng.at(PH_DIM_PROP_JIT + 1);
-            const PropPick k = prop_pick(a.prop, pu, pv, r3, r4, sel, j6, j7);
-            dir = k.phi_u < 0.0f ? on_hemisphere(L.normal, r3, r4) : cell_direction(L.normal, k.cos_t, k.phi_u);
-            flux = flux * prop_weight(a.prop, pu, pv, k);
-        }
-        v3f o = pos + L.normal * a.ray_eps;
-        for (int bounce = 0;; bounce++) {
-            if (bounce >= a.max_bounces || vmax3(flux) <= 0.0f) break;
-            Hit h = trace_closest<256>(sc, o, dir, 0.0f, TRACE_TMAX, stk, &ovf);
-            segs++;
-            if (h.tri < 0) break;
-            const int dd = PH_DIM_BOUNCE + 3 * bounce;
-            const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
-            const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
-            const v3f hp = o + dir * h.t;
-            const v3f ng = tri_normal(sc, h.tri);
-            const v3f on = dot(ng, dir) < 0.0f ? ng : -ng;
-            if (m.type & PPPM_BSDF_EMITTER) break;
-            if ((m.type & PPPM_BSDF_BSSRDF) && a.irr_pos && dot(ng, dir) < 0.0f) {
-                // irradiance photon (arriving from outside) for the subsurface
-                // estimate; the path goes on through the dielectric boundary below
-                unsigned long long slot = wave_append(&a.counters->irr_stored);
-                if (slot < (unsigned long long)a.irr_cap) {
-                    a.irr_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
-                    a.irr_flux[slot] = make_float4(flux.x, flux.y, flux.z,
-                                                   __uint_as_float(((uint32_t)i << 6) | (uint32_t)bounce));
+    const uint32_t pass = a.pd ? a.pd->pass : a.pass;
+    const int lane = threadIdx.x & 63;
+    Rng rng;
+    v3f o = mk(0.0f, 0.0f, 0.0f), dir = o, flux = o;
+    uint32_t i = 0;
+    int bounce = 0;
+    bool live = false;
+    while (true) {
+        const uint64_t need = __ballot(!live);
+        if (need != 0) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&a.counters->photon_next, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (!live) {
+                const unsigned long long k = base + (unsigned long long)__popcll(need & lanemask_lt());
+                if (k < (unsigned long long)a.n_photons) {
+                    i = (uint32_t)k;
+                    rng.init(a.seed, KIND_PHOTON, pass, i, a.scene_id);
+                    emit_photon(a, sc, rng, o, dir, flux);
+                    bounce = 0;
+                    live = true;
                 }
             }
-            if (m.type & PPPM_BSDF_LAMBERTIAN) {
-                unsigned long long slot = wave_append(&a.counters->photons_stored);
-                if (slot < (unsigned long long)a.cap) {
-                    a.ph_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
-                    a.ph_flux[slot] = make_float4(flux.x, flux.y, flux.z,
-                                                  __uint_as_float(((uint32_t)i << 6) | (uint32_t)bounce));
-                }
-                if (r0 < m.rr_mean) {
-                    dir = on_hemisphere(on, r1, r2);
-                    o = spawn(hp, ng, dir, a.ray_eps);
-                    flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / m.rr_mean);
-                } else {
-                    break;
-                }
+        }
+        if (!live) break;  // no photons left for this lane
+        if (bounce >= a.max_bounces || vmax3(flux) <= 0.0f) {
+            live = false;
+            continue;
+        }
+        Hit h = trace_closest<256>(sc, o, dir, 0.0f, TRACE_TMAX, stk, &ovf);
+        segs++;
+        if (h.tri < 0) {
+            live = false;
+            continue;
+        }
+        const int dd = PH_DIM_BOUNCE + 3 * bounce;
+        const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
+        const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
+        const v3f hp = o + dir * h.t;
+        const v3f ng = tri_normal(sc, h.tri);
+        const v3f on = dot(ng, dir) < 0.0f ? ng : -ng;
+        if (m.type & PPPM_BSDF_EMITTER) {
+            live = false;
+            continue;
+        }
+        if ((m.type & PPPM_BSDF_BSSRDF) && a.irr_pos && dot(ng, dir) < 0.0f) {
+            // irradiance photon (arriving from outside) for the subsurface
+            // estimate; the path goes on through the dielectric boundary below
+            unsigned long long slot = wave_append(&a.counters->irr_stored);
+            if (slot < (unsigned long long)a.irr_cap) {
+                a.irr_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                a.irr_flux[slot] = make_float4(flux.x, flux.y, flux.z,
+                                               __uint_as_float((i << 6) | (uint32_t)bounce));
+            }
+        }
+        if (m.type & PPPM_BSDF_LAMBERTIAN) {
+            unsigned long long slot = wave_append(&a.counters->photons_stored);
+            if (slot < (unsigned long long)a.cap) {
+                a.ph_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                a.ph_flux[slot] = make_float4(flux.x, flux.y, flux.z, __uint_as_float((i << 6) | (uint32_t)bounce));
+            }
+            if (r0 < m.rr_mean) {
+                dir = on_hemisphere(on, r1, r2);
+                o = spawn(hp, ng, dir, a.ray_eps);
+                flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / m.rr_mean);
             } else {
-                float pdf = 1.0f;
-                v3f nd;
-                if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(dir, on);
-                else nd = refraction_sample(dir, ng, m.ior, r0, &pdf);
-                o = spawn(hp, ng, nd, a.ray_eps);
-                dir = nd;
-                flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / pdf);
+                live = false;
+                continue;
             }
+        } else {
+            float pdf = 1.0f;
+            v3f nd;
+            if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(dir, on);
+            else nd = refraction_sample(dir, ng, m.ior, r0, &pdf);
+            o = spawn(hp, ng, nd, a.ray_eps);
+            dir = nd;
+            flux = flux * mk(m.rho[0], m.rho[1], m.rho[2]) * (1.0f / pdf);
         }
+        bounce++;
     }
     segs = wave_sum(segs);
     uint64_t any_ovf = __ballot(ovf);
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         atomicAdd(&a.counters->photon_segments, segs);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
     }
