@@ -126,10 +126,16 @@ __device__ __forceinline__ float dist2(v3f x, float4 A) {
     return __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, __fmul_rn(dz, dz)));
 }
 
+// cone filter weight 1 - |x - p| / (1.1 r) from the squared distance, with the
+// hardware square root (v_sqrt_f32) that every gather kernel shares
+__device__ __forceinline__ float cone_weight(float d2, float inv_kr) {
+    return 1.0f - __builtin_amdgcn_sqrtf(d2) * inv_kr;
+}
+
 // one candidate inside the radius: normal agreement, cone weight
 __device__ __forceinline__ void accumulate(Acc& acc, v3f nx, float d2, float inv_kr, float4 A, float4 B, float4 C) {
     if (A.w * nx.x + B.x * nx.y + B.y * nx.z > 0.0f) {
-        const float wt = 1.0f - sqrtf(d2) * inv_kr;
+        const float wt = cone_weight(d2, inv_kr);
         acc.x += wt * B.z;
         acc.y += wt * B.w;
         acc.z += wt * C.x;
@@ -154,7 +160,8 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
     unsigned long long nb = 0, nc = 0;
-    Acc acc;
+    uint32_t ni = 0;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
     if (valid) {
         const float4 hp = a.hp_pos[i];
         const float4 hw = a.hp_w[i];
@@ -164,6 +171,8 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
         uint32_t todo = own_buckets(a, hp, mb);
         const float r2 = (float)(gather_radius(a) * gather_radius(a));
         const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
+        const float4* __restrict__ pa = a.ph_a;
+        const float4* __restrict__ pb = a.ph_b;
         // slots in the reference's z, y, x cell order: lanes of neighbouring
         // half cells then walk their shared buckets in step
 #pragma unroll
@@ -172,26 +181,48 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
             const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
             nb++;
             nc += b1 - b0;
-            // compact layout: (x, y, z, octahedral normal) (Phi_r, Phi_g, Phi_b, key)
-            for (uint32_t j = b0; j < b1; j++) {
-                const float4 A = a.ph_a[j];
-                const float d2 = dist2(x, A);
-                if (d2 > r2) continue;
-                const v3f np = oct_direction(__float_as_uint(A.w));
-                const float4 F = a.ph_b[j];
-                if (np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f) {
-                    const float wt = 1.0f - sqrtf(d2) * inv_kr;
-                    acc.x += wt * F.x;
-                    acc.y += wt * F.y;
-                    acc.z += wt * F.z;
-                    acc.n++;
-                    if (a.used) a.used[j] = 1u;
+            // compact layout: (x, y, z, octahedral normal) (Phi_r, Phi_g, Phi_b, key);
+            // four records in flight per wait, the bucket's last record
+            // repeated past its end and masked out
+            for (uint32_t j = b0; j < b1; j += 4) {
+                const uint32_t last = b1 - 1;
+                float4 A[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) A[q] = pa[min(j + q, last)];
+                // the normal word is read with the position (one 16-byte load)
+#pragma unroll
+                for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
+                float d2[4];
+                bool in[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    d2[q] = dist2(x, A[q]);
+                    in[q] = j + q < b1 && d2[q] <= r2;
+                    if (in[q]) {
+                        const v3f np = oct_direction(__float_as_uint(A[q].w));
+                        in[q] = np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f;
+                    }
+                }
+                float4 F[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (in[q]) F[q] = pb[j + q];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (in[q]) {
+                        const float wt = cone_weight(d2[q], inv_kr);
+                        ax += wt * F[q].x;
+                        ay += wt * F[q].y;
+                        az += wt * F[q].z;
+                        ni++;
+                        if (a.used) a.used[j + q] = 1u;
+                    }
                 }
             }
         }
-        write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), acc.x, acc.y, acc.z);
+        write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
     }
-    flush_counters(a, valid, nb, nc, acc.n);
+    flush_counters(a, valid, nb, nc, ni);
 }
 
 // Per-lane setup shared by the wave-cooperative kernels.
