@@ -126,7 +126,7 @@ typedef struct {
     int64_t eye_segments;      /* ray queries of the eye pass */
     int64_t hitpoints;
     int64_t gather_buckets;    /* sum over hit points of de-duplicated buckets */
-    int64_t gather_candidates; /* sum of photon records in them */
+    int64_t gather_candidates; /* sum of photon records in them (the reference's candidates) */
     int64_t gather_inradius;
     double photon_ms, order_ms, grid_ms, eye_ms, gather_ms, total_ms;
     double hitsort_ms;         /* Morton sort of the hit points (tiled gather) */
@@ -142,6 +142,9 @@ typedef struct {
     int64_t irradiance_photons;/* photons stored on BSSRDF surfaces */
     int64_t sss_hits;          /* eye paths ending in the dipole estimate */
     int64_t sss_points;        /* surface points of the irradiance cache */
+    /* gather work beyond the reference convention (gather_candidates) */
+    int64_t gather_tested;     /* photon records the kernel distance-tested */
+    int64_t gather_blocks;     /* 32-record block boxes tested (lanes kernel) */
 } pppm_pass_stats;
 
 /* ---- context ---- */

@@ -93,12 +93,15 @@ struct Ctx {
     double last_scale = 0.0;
     uint32_t* last_perm1 = nullptr;
     uint32_t* last_sorted = nullptr;
+    int32_t last_sub_bits = 0;  // the last pass ordered each bucket by a cell-local Morton code
     hipEvent_t ev[13] = {};
     hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
     hipStream_t stream3 = nullptr;  // the gather of pipelined passes (pppm_render_passes)
     hipEvent_t pev[2][11] = {};     // per buffer set: events of a pipelined pass
     // the second buffer set of pipelined passes: what a pass's gather reads
-    DBuf ph_a_alt, ph_b_alt, ph_c_alt, off_alt, hp_pos_s_alt, hp_w_s_alt, counters_alt, pass_counters;
+    DBuf ph_a_alt, ph_b_alt, ph_c_alt, off_alt, hp_pos_s_alt, hp_w_s_alt, counters_alt, pass_counters, boxes_alt;
+    // block boxes of the lanes gather over ph_a (GATHER_BLOCK records each)
+    DBuf boxes;
     // subsurface scattering (BSSRDF materials)
     DBuf dips, sss_pts, sss_ptw, sss_E, irr_pos, irr_flux, sss_pos, sss_w;
     std::vector<int32_t> h_sss_index;  // material -> dipole index (-1: none)
@@ -209,19 +212,36 @@ void cross3(const double a[3], const double b[3], double o[3]) {
 // Returns the buffer holding the sorted values (grid sorted_idx).
 // With a device count the n slots hold *count records followed by unused
 // ones, whose keys sort behind every bucket.
+// Bits of a bucket id in the grid sort key (with a device count, one more
+// value for the sentinel of unused slots).
+int grid_bucket_bits(uint32_t size, bool sentinel) {
+    return sentinel ? bits_for((uint64_t)size) : bits_for(size > 0 ? (uint64_t)(size - 1) : 0);
+}
+
+// Cell-local Morton bits per axis below the bucket in the render path's grid
+// sort (at most 3, keeping the key within 32 bits): the lanes gather walks a
+// bucket in blocks whose boxes are then small.  Other gather kernels walk whole
+// buckets and keep the reference's insertion order.
+int render_sub_bits(const Ctx* c, uint32_t size, bool sentinel) {
+    if (c->rp.gather_mode != GATHER_LANES) return 0;
+    return std::max(0, std::min(3, (32 - grid_bucket_bits(size, sentinel)) / 3));
+}
+
 uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32_t* perm, int64_t n,
                      const GridParams& g, int keep, const unsigned long long* count = nullptr) {
     uint32_t* b[4];
     int k = 0;
     for (int i = 0; i < 5 && k < 4; i++)
         if (i != keep) b[k++] = c->u[i].as<uint32_t>();
-    int bits = count ? bits_for((uint64_t)g.size) : bits_for(g.size > 0 ? (uint64_t)(g.size - 1) : 0);
+    const int bucket_bits = grid_bucket_bits(g.size, count != nullptr);
+    const int shift = 3 * g.sub_bits;
+    const int bits = bucket_bits + shift;
     const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
     launch_point_keys(c->stream, pos4, pos3, perm, n, g, b[0], b[1], count, sentinel);
     bool alt = radix_sort_pairs(c->stream, b[0], b[1], b[2], b[3], c->hist.as<uint32_t>(), n, bits);
     uint32_t* keys = alt ? b[2] : b[0];
     uint32_t* vals = alt ? b[3] : b[1];
-    launch_bucket_offsets(c->stream, keys, n, g.size, c->off.as<uint32_t>());
+    launch_bucket_offsets(c->stream, keys, n, g.size, c->off.as<uint32_t>(), shift);
     return vals;
 }
 
@@ -1032,6 +1052,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     TRY_HIP(ctx, ctx->ph_a.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_b.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_c.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->boxes.ensure(sizeof(float4) * 2 * (size_t)gather_box_count(cap)));
     uint32_t Smax = p->hash_size > 0 ? (uint32_t)p->hash_size : (uint32_t)cap;
     TRY_HIP(ctx, ensure_sort_buffers(ctx, cap, Smax));
     TRY_HIP(ctx, ctx->hp_pos.ensure(sizeof(float4) * (size_t)npix));
@@ -1163,11 +1184,13 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
     g.scale = 1.0 / (2.0 * radius);
     g.size = S;
+    g.sub_bits = render_sub_bits(ctx, S, false);
     TRY_HIP(ctx, ctx->off.ensure(sizeof(uint32_t) * ((size_t)S + 1)));
     uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
+    if (g.sub_bits > 0) launch_gather_boxes(s, ctx->ph_a.as<float4>(), n, ctx->boxes.as<float4>());
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     // gather, once the hit points are sorted on the second stream
@@ -1191,6 +1214,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ga.counters = dc;
     // contribution records need the photons the gather accepts: the lanes
     // kernel marks them as it goes, the others through a separate pass
+    ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
     const bool mark_in_gather = ctx->prop.enabled && n > 0 && ga.mode == GATHER_LANES;
     ga.used = mark_in_gather ? ctx->prop_used.as<uint32_t>() : nullptr;
     if (mark_in_gather) launch_zero_u32(s, ctx->prop_used.p, n);
@@ -1269,6 +1293,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     ctx->last_scale = g.scale;
     ctx->last_perm1 = perm1;
     ctx->last_sorted = sorted;
+    ctx->last_sub_bits = g.sub_bits;
     if (st) {
         // stream 1: photon (0-1), order (1-2), grid (2-3), gather (12-6);
         // stream 2: eye (11-4), hit point sort (4-5)
@@ -1288,6 +1313,8 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         st->gather_buckets = (int64_t)hc.gather_buckets;
         st->gather_candidates = (int64_t)hc.gather_candidates;
         st->gather_inradius = (int64_t)hc.gather_inradius;
+        st->gather_tested = (int64_t)hc.gather_tested;
+        st->gather_blocks = (int64_t)hc.gather_blocks;
         st->photon_ms = t[0];
         st->order_ms = t[1];
         st->grid_ms = t[2];
@@ -1347,9 +1374,10 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     TRY_HIP(ctx, ctx->hp_pos_s_alt.ensure(ctx->hp_pos_s.n));
     TRY_HIP(ctx, ctx->hp_w_s_alt.ensure(ctx->hp_w_s.n));
     TRY_HIP(ctx, ctx->counters_alt.ensure(ctx->counters.n));
+    TRY_HIP(ctx, ctx->boxes_alt.ensure(ctx->boxes.n));
     TRY_HIP(ctx, ctx->pass_counters.ensure(sizeof(Counters) * (size_t)count));
     // the alternate set must be sized as the current one for every later pass
-    if (ctx->off_alt.n < ctx->off.n || ctx->ph_a_alt.n < ctx->ph_a.n)
+    if (ctx->off_alt.n < ctx->off.n || ctx->ph_a_alt.n < ctx->ph_a.n || ctx->boxes_alt.n < ctx->boxes.n)
         return ctx->fail(PPPM_ERR_NOMEM, "second buffer set");
     auto swap_buf = [](DBuf& a, DBuf& b) {
         std::swap(a.p, b.p);
@@ -1384,6 +1412,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             swap_buf(ctx->hp_pos_s, ctx->hp_pos_s_alt);
             swap_buf(ctx->hp_w_s, ctx->hp_w_s_alt);
             swap_buf(ctx->counters, ctx->counters_alt);
+            swap_buf(ctx->boxes, ctx->boxes_alt);
         }
         hipEvent_t* E = ctx->pev[k & 1];
         if (k >= 2) {
@@ -1460,12 +1489,14 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
         g.scale = 1.0 / (2.0 * radius);
         g.size = S;
+        g.sub_bits = render_sub_bits(ctx, S, false);
         ctx->stream = A;
         uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
         ctx->stream = keep_stream;
         launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                              ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                              ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
+        if (g.sub_bits > 0) launch_gather_boxes(A, ctx->ph_a.as<float4>(), n, ctx->boxes.as<float4>());
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
         // gather side, in pass order
@@ -1489,6 +1520,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ga.out_rgb = nullptr;
         ga.counters = dc;
         ga.used = nullptr;
+        ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
         launch_gather(Cs, ga);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[GEND], Cs));
@@ -1502,6 +1534,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ctx->last_scale = g.scale;
         ctx->last_perm1 = perm1;
         ctx->last_sorted = sorted;
+        ctx->last_sub_bits = g.sub_bits;
     }
     for (int32_t k = std::max(0, count - 2); k < count; k++) read_times(k);
     TRY_HIP(ctx, hipStreamSynchronize(Cs));
@@ -1528,6 +1561,8 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             st.gather_buckets = (int64_t)c.gather_buckets;
             st.gather_candidates = (int64_t)c.gather_candidates;
             st.gather_inradius = (int64_t)c.gather_inradius;
+            st.gather_tested = (int64_t)c.gather_tested;
+            st.gather_blocks = (int64_t)c.gather_blocks;
             st.photon_ms = t[0];
             st.order_ms = t[1];
             st.grid_ms = 0.0;
@@ -1603,6 +1638,12 @@ int pppm_last_grid(void* ctx_raw, int32_t* sorted_idx, int32_t* cell_start, int3
         TRY_HIP(ctx, hipMemcpyAsync(sorted_idx, ctx->last_sorted, sizeof(uint32_t) * (size_t)ctx->last_n, hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->off.p, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sorted_idx && ctx->last_sub_bits > 0) {
+        // the pass kept each bucket in cell-local Morton order for the gather;
+        // the record ids of a bucket in ascending order are its insertion order
+        for (int64_t h = 0; h < hash_size; h++)
+            std::sort(sorted_idx + off[(size_t)h], sorted_idx + off[(size_t)h + 1]);
+    }
     copy_ranges(off, hash_size, cell_start, cell_end);
     if (bbox_min)
         for (int d = 0; d < 3; d++) bbox_min[d] = ctx->grid_bmin[d];
@@ -1688,8 +1729,15 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     for (int d = 0; d < 3; d++) g.bmin[d] = bbox_min[d];
     g.scale = 1.0 / (2.0 * radius);
     g.size = (uint32_t)hash_size;
+    // the lanes kernel as in a pass: Morton order inside buckets, block boxes
+    if (gather_mode == GATHER_LANES)
+        g.sub_bits = std::max(0, std::min(3, (32 - grid_bucket_bits(g.size, false)) / 3));
     uint32_t* sorted = build_grid(ctx, p4, nullptr, nullptr, n_ph, g, -1);
     launch_gather_layout(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s, gather_mode == GATHER_LANES ? nullptr : c4s);
+    if (g.sub_bits > 0) {
+        TRY_HIP(ctx, ctx->boxes.ensure(sizeof(float4) * 2 * (size_t)gather_box_count(n_ph)));
+        launch_gather_boxes(s, p4s, n_ph, ctx->boxes.as<float4>());
+    }
     GatherArgs ga;
     ga.mode = gather_mode;
     ga.ph_a = p4s;
@@ -1715,6 +1763,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     ga.out_rgb = dout;
     ga.counters = dc;
     ga.used = nullptr;
+    ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
     launch_gather(s, ga);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -1735,6 +1784,8 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
         st->gather_buckets = (int64_t)hc.gather_buckets;
         st->gather_candidates = (int64_t)hc.gather_candidates;
         st->gather_inradius = (int64_t)hc.gather_inradius;
+        st->gather_tested = (int64_t)hc.gather_tested;
+        st->gather_blocks = (int64_t)hc.gather_blocks;
     }
     return PPPM_OK;
 }
@@ -2189,12 +2240,14 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
     g.scale = 0.0;
     g.size = (uint32_t)ctx->rp.hash_size;
+    g.sub_bits = render_sub_bits(ctx, g.size, true);
     g.pd = pd;
     const uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, cap, g, keep,
                                         &dc->photons_stored);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
+    if (g.sub_bits > 0) launch_gather_boxes(s, ctx->ph_a.as<float4>(), cap, ctx->boxes.as<float4>());
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
     launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * 64 : 16384);
@@ -2215,6 +2268,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     ga.img_sum = ctx->img_sum.as<float>();
     ga.img_comp = ctx->img_comp.as<float>();
     ga.counters = dc;
+    ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
     launch_gather(s, ga);
 }
 

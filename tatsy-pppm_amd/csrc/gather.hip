@@ -6,17 +6,19 @@
 // |x - p|^2 <= r^2 and n_p . n_x > 0 contributes (1 - |x - p| / (1.1 r)) Phi_p;
 // L = W * sum / (pi r^2 (1 - 2 / 3.3)).
 //
-// Photons are read in bucket (cell) order from a gather layout of three
-// 16-byte words: (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g) (Phi_b, -, -, -), the
-// normal decoded once when the layout is written.  Hit points arrive sorted
+// Photons are read in bucket (cell) order from a gather layout: for the lanes
+// kernel the records as stored, (x, y, z, octahedral normal) (Phi, key); for
+// the others three 16-byte words (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g)
+// (Phi_b, key, -, -) with the normal decoded once.  Hit points arrive sorted
 // along a Morton curve of half cells (edge r), so neighbouring lanes share
 // their eight buckets.  Three kernels compute the same sum:
-//   k_gather_wave (default): each wave walks the union of its 64 lanes'
-//     buckets one bucket at a time with a wave-uniform photon index, so photon
-//     records arrive through scalar loads and are tested by all 64 lanes.
-//   k_gather_tiled: the union's photons are staged through LDS in coalesced
-//     64-photon chunks and tested by all 64 lanes.
-//   k_gather_lanes: one lane per hit point walking its own buckets.
+//   k_gather_lanes (default): one lane per hit point walking its own buckets;
+//     inside a bucket the records follow a cell-local Morton order and blocks
+//     of 32 records whose bounding box misses the ball are skipped.
+//   k_gather_tiled: the union of a wave's buckets is staged through LDS in
+//     coalesced 64-photon chunks and tested by all 64 lanes.
+//   k_gather_wave: the union is walked with a wave-uniform photon index, so
+//     photon records arrive through scalar loads and are tested by all lanes.
 #include "internal.h"
 
 namespace {
@@ -144,22 +146,84 @@ __device__ __forceinline__ void accumulate(Acc& acc, v3f nx, float d2, float inv
 }
 
 __device__ __forceinline__ void flush_counters(const GatherArgs& a, bool valid, unsigned long long nb,
-                                      unsigned long long nc, unsigned long long ni) {
+                                      unsigned long long nc, unsigned long long ni,
+                                      unsigned long long nt, unsigned long long nk) {
     nb = wave_sum(valid ? nb : 0);
     nc = wave_sum(valid ? nc : 0);
     ni = wave_sum(valid ? ni : 0);
+    nt = wave_sum(valid ? nt : 0);
+    nk = wave_sum(valid ? nk : 0);
     if ((threadIdx.x & 63) == 0 && a.counters) {
         atomicAdd(&a.counters->gather_buckets, nb);
         atomicAdd(&a.counters->gather_candidates, nc);
         atomicAdd(&a.counters->gather_inradius, ni);
+        atomicAdd(&a.counters->gather_tested, nt);
+        atomicAdd(&a.counters->gather_blocks, nk);
     }
 }
 
+// The records [j0, j1) of one bucket: four 16-byte records in flight per wait
+// (the range's last record repeated past its end and masked out), flux loads
+// issued before the sums, accumulation in record order.
+__device__ __forceinline__ void lanes_range(const GatherArgs& a, uint32_t j0, uint32_t j1, v3f x, v3f nx, float r2,
+                                            float inv_kr, float& ax, float& ay, float& az, uint32_t& ni) {
+    const float4* __restrict__ pa = a.ph_a;
+    const float4* __restrict__ pb = a.ph_b;
+    for (uint32_t j = j0; j < j1; j += 4) {
+        const uint32_t last = j1 - 1;
+        float4 A[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) A[q] = pa[min(j + q, last)];
+        // the normal word is read with the position (one 16-byte load)
+#pragma unroll
+        for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
+        float d2[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            d2[q] = dist2(x, A[q]);
+            in[q] = j + q < j1 && d2[q] <= r2;
+            if (in[q]) {
+                const v3f np = oct_direction(__float_as_uint(A[q].w));
+                in[q] = np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f;
+            }
+        }
+        float4 F[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (in[q]) F[q] = pb[j + q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (in[q]) {
+                const float wt = cone_weight(d2[q], inv_kr);
+                ax += wt * F[q].x;
+                ay += wt * F[q].y;
+                az += wt * F[q].z;
+                ni++;
+                if (a.used) a.used[j + q] = 1u;
+            }
+        }
+    }
+}
+
+// Squared distance from x to the box (lo, hi), zero inside.
+__device__ __forceinline__ float box_dist2(v3f x, float4 lo, float4 hi) {
+    const float dx = fmaxf(fmaxf(lo.x - x.x, x.x - hi.x), 0.0f);
+    const float dy = fmaxf(fmaxf(lo.y - x.y, x.y - hi.y), 0.0f);
+    const float dz = fmaxf(fmaxf(lo.z - x.z, x.z - hi.z), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One lane per hit point over its own buckets.  With block boxes, a bucket is
+// walked in blocks of GATHER_BLOCK records and a block whose box lies farther
+// than the radius (with a relative margin of 2^-12 on r^2, far above the FP32
+// rounding of either distance) holds no accepted record and is skipped; the
+// accepted records and their order are those of the full walk.
 __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
     const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
-    unsigned long long nb = 0, nc = 0;
+    unsigned long long nb = 0, nc = 0, nt = 0, nk = 0;
     uint32_t ni = 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     if (valid) {
@@ -170,59 +234,33 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
         uint32_t mb[8];
         uint32_t todo = own_buckets(a, hp, mb);
         const float r2 = (float)(gather_radius(a) * gather_radius(a));
+        const float r2_box = r2 * (1.0f + 1.0f / 4096.0f);
         const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
-        const float4* __restrict__ pa = a.ph_a;
-        const float4* __restrict__ pb = a.ph_b;
         // slots in the reference's z, y, x cell order: lanes of neighbouring
         // half cells then walk their shared buckets in step
-#pragma unroll
+#pragma nounroll
         for (int c = 0; c < 8; c++) {
             if (!((todo >> c) & 1u)) continue;
             const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
             nb++;
             nc += b1 - b0;
-            // compact layout: (x, y, z, octahedral normal) (Phi_r, Phi_g, Phi_b, key);
-            // four records in flight per wait, the bucket's last record
-            // repeated past its end and masked out
-            for (uint32_t j = b0; j < b1; j += 4) {
-                const uint32_t last = b1 - 1;
-                float4 A[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) A[q] = pa[min(j + q, last)];
-                // the normal word is read with the position (one 16-byte load)
-#pragma unroll
-                for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
-                float d2[4];
-                bool in[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    d2[q] = dist2(x, A[q]);
-                    in[q] = j + q < b1 && d2[q] <= r2;
-                    if (in[q]) {
-                        const v3f np = oct_direction(__float_as_uint(A[q].w));
-                        in[q] = np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f;
-                    }
-                }
-                float4 F[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (in[q]) F[q] = pb[j + q];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (in[q]) {
-                        const float wt = cone_weight(d2[q], inv_kr);
-                        ax += wt * F[q].x;
-                        ay += wt * F[q].y;
-                        az += wt * F[q].z;
-                        ni++;
-                        if (a.used) a.used[j + q] = 1u;
-                    }
-                }
+            if (!a.boxes) {
+                nt += b1 - b0;
+                lanes_range(a, b0, b1, x, nx, r2, inv_kr, ax, ay, az, ni);
+                continue;
+            }
+            if (b1 <= b0) continue;
+            for (uint32_t k = b0 / GATHER_BLOCK; k <= (b1 - 1) / GATHER_BLOCK; k++) {
+                nk++;
+                if (box_dist2(x, a.boxes[2 * k], a.boxes[2 * k + 1]) > r2_box) continue;
+                const uint32_t j0 = max(b0, k * GATHER_BLOCK), j1 = min(b1, (k + 1) * GATHER_BLOCK);
+                nt += j1 - j0;
+                lanes_range(a, j0, j1, x, nx, r2, inv_kr, ax, ay, az, ni);
             }
         }
         write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
     }
-    flush_counters(a, valid, nb, nc, ni);
+    flush_counters(a, valid, nb, nc, ni, nt, nk);
 }
 
 // Per-lane setup shared by the wave-cooperative kernels.
@@ -301,7 +339,7 @@ __global__ __launch_bounds__(256) void k_gather_wave(GatherArgs a) {
         }
     }
     if (h.valid) write_result(a, (int64_t)__float_as_uint(h.hw.w), xyz(h.hw), acc.x, acc.y, acc.z);
-    flush_counters(a, h.valid, h.nb, h.nc, acc.n);
+    flush_counters(a, h.valid, h.nb, h.nc, acc.n, h.nc, 0);
 }
 
 // One wave (64 lanes) per workgroup; the union's photons staged through LDS.
@@ -370,7 +408,32 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
         }
     }
     if (h.valid) write_result(a, (int64_t)__float_as_uint(h.hw.w), xyz(h.hw), acc.x, acc.y, acc.z);
-    flush_counters(a, h.valid, h.nb, h.nc, acc.n);
+    flush_counters(a, h.valid, h.nb, h.nc, acc.n, h.nc, 0);
+}
+
+// Box of each block of GATHER_BLOCK consecutive records: half-wave min / max
+// reductions (GATHER_BLOCK = 32 lanes per block).
+__global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__ a, int64_t n,
+                                                      float4* __restrict__ boxes) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k = j / GATHER_BLOCK;
+    const int64_t nbox = (n + GATHER_BLOCK - 1) / GATHER_BLOCK;
+    if (k >= nbox) return;  // whole blocks of 32 lanes leave together
+    const float4 p = a[min(j, n - 1)];  // the last record repeated past the end
+    float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+#pragma unroll
+    for (int off = GATHER_BLOCK / 2; off > 0; off >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, off));
+        ly = fminf(ly, __shfl_xor(ly, off));
+        lz = fminf(lz, __shfl_xor(lz, off));
+        hx = fmaxf(hx, __shfl_xor(hx, off));
+        hy = fmaxf(hy, __shfl_xor(hy, off));
+        hz = fmaxf(hz, __shfl_xor(hz, off));
+    }
+    if ((j % GATHER_BLOCK) == 0) {
+        boxes[2 * k] = make_float4(lx, ly, lz, 0.0f);
+        boxes[2 * k + 1] = make_float4(hx, hy, hz, 0.0f);
+    }
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -486,6 +549,12 @@ void launch_gather(hipStream_t s, const GatherArgs& a) {
         hipLaunchKernelGGL(k_gather_tiled, dim3(grid1(a.n_hit, 64)), dim3(64), 0, s, a);
     else
         hipLaunchKernelGGL(k_gather_wave, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
+}
+
+void launch_gather_boxes(hipStream_t s, const float4* a, int64_t n, float4* boxes) {
+    if (n <= 0) return;
+    const int64_t threads = gather_box_count(n) * GATHER_BLOCK;
+    hipLaunchKernelGGL(k_gather_boxes, dim3(grid1(threads, 256)), dim3(256), 0, s, a, n, boxes);
 }
 
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,

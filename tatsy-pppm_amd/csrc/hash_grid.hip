@@ -23,6 +23,22 @@ __device__ inline uint32_t hash3(int32_t ix, int32_t iy, int32_t iz, uint32_t si
     return h % size;
 }
 
+// Sub-cell of x inside its cell along one axis, `bits` bits: the fraction of
+// (x - min) * scale past its truncation.  Any value is valid (it orders records
+// inside a bucket only); this one puts nearby records next to each other.
+__device__ inline uint32_t sub_cell(double x, double mn, double scale, int bits) {
+    const double b = (x - mn) * scale;
+    const double f = fabs(b - trunc(b));
+    const uint32_t k = (uint32_t)(f * (double)(1u << bits));
+    return min(k, (1u << bits) - 1u);
+}
+
+__device__ inline uint32_t spread3(uint32_t v, int bits) {  // bit i -> bit 3 i
+    uint32_t o = 0;
+    for (int i = 0; i < bits; i++) o |= ((v >> i) & 1u) << (3 * i);
+    return o;
+}
+
 __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __restrict__ pos3,
                              const uint32_t* __restrict__ perm, int64_t n, GridParams g,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -43,19 +59,28 @@ __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __res
         x = pos3[3 * j]; y = pos3[3 * j + 1]; z = pos3[3 * j + 2];
     }
     const double sc = grid_scale(g);
-    keys[i] = hash3(cell_of(x, g.bmin[0], sc), cell_of(y, g.bmin[1], sc), cell_of(z, g.bmin[2], sc), g.size);
+    uint32_t key = hash3(cell_of(x, g.bmin[0], sc), cell_of(y, g.bmin[1], sc), cell_of(z, g.bmin[2], sc), g.size);
+    if (g.sub_bits > 0) {
+        const int m = g.sub_bits;
+        key = (key << (3 * m)) | spread3(sub_cell(x, g.bmin[0], sc, m), m) |
+              (spread3(sub_cell(y, g.bmin[1], sc, m), m) << 1) | (spread3(sub_cell(z, g.bmin[2], sc, m), m) << 2);
+    }
+    keys[i] = key;
     vals[i] = (uint32_t)i;
 }
 
-// off[h] = lower_bound(sorted_keys, h) for h in [0, size]; each key boundary
-// fills the empty buckets up to it.  Keys >= size (sentinels of unused slots)
-// may follow the real ones.
-__global__ void k_bucket_offsets(const uint32_t* __restrict__ keys, int64_t n, uint32_t size,
+// off[h] = lower_bound(sorted buckets, h) for h in [0, size], the bucket of a
+// key being key >> shift; each bucket boundary fills the empty buckets up to
+// it.  Buckets >= size (sentinels of unused slots) may follow the real ones.
+__global__ void k_bucket_offsets(const uint32_t* __restrict__ keys, int64_t n, uint32_t size, int shift,
                                  uint32_t* __restrict__ off) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > n) return;
-    int64_t lo = (j == 0) ? 0 : (int64_t)keys[j - 1] + 1;
-    int64_t hi = (j == n) ? (int64_t)size : (int64_t)keys[j];
+    const int64_t prev = (j == 0) ? -1 : (int64_t)(keys[j - 1] >> shift);
+    const int64_t cur = (j == n) ? (int64_t)size : (int64_t)(keys[j] >> shift);
+    if (j > 0 && j < n && cur == prev) return;  // inside a bucket
+    int64_t lo = prev + 1;
+    int64_t hi = cur;
     if (hi > (int64_t)size) hi = size;  // sentinel keys (>= size) close the last bucket
     for (int64_t h = lo; h <= hi; h++) off[h] = (uint32_t)j;
 }
@@ -145,9 +170,9 @@ void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, con
 }
 
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
-                           uint32_t* off) {
+                           uint32_t* off, int shift) {
     hipLaunchKernelGGL(k_bucket_offsets, dim3(grid1(n + 1, 256)), dim3(256), 0, s, sorted_keys, n,
-                       size, off);
+                       size, shift, off);
 }
 
 void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,

@@ -43,6 +43,9 @@ struct GridParams {
     double bmin[3];
     double scale;
     uint32_t size;
+    // bits per axis of a cell-local Morton code below the bucket in the sort
+    // key (0: records of a bucket in insertion order, the reference's layout)
+    int32_t sub_bits = 0;
     const PassDev* pd = nullptr;  // when set, the scale is read from it
 };
 __device__ __forceinline__ double grid_scale(const GridParams& g) { return g.pd ? g.pd->scale : g.scale; }
@@ -51,9 +54,9 @@ __device__ __forceinline__ double grid_scale(const GridParams& g) { return g.pd 
 void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
                        int64_t n, GridParams g, uint32_t* keys, uint32_t* vals,
                        const unsigned long long* count = nullptr, uint32_t sentinel = 0xFFFFFFFFu);
-// off[h] = number of sorted keys < h, for h = 0..size
+// off[h] = number of sorted keys whose bucket (key >> shift) is < h, for h = 0..size
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
-                           uint32_t* off);
+                           uint32_t* off, int shift = 0);
 // box insertion: per-item pair counts, then pairs in z, y, x order
 void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,
                        int64_t n, GridParams g, uint64_t* counts);
@@ -95,6 +98,8 @@ struct Counters {
     unsigned long long irr_stored;   // photons stored on BSSRDF surfaces
     unsigned long long sss_hits;     // eye paths ending in the subsurface estimate
     unsigned long long photon_next;  // next photon index handed to a photon-pass lane
+    unsigned long long gather_tested;  // photon records distance-tested by the gather
+    unsigned long long gather_blocks;  // block boxes tested by the lanes gather
 };
 
 // Persistent photon-pass blocks per CU: at least the blocks one CU holds at
@@ -154,6 +159,9 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 // PPPM_GATHER_AUTO resolves to the kernel measured fastest on MI355X
 #define GATHER_AUTO_CHOICE GATHER_LANES
 
+// photon records per block box of the lanes gather
+#define GATHER_BLOCK 32
+
 struct GatherArgs {
     int32_t mode;           // GATHER_WAVE, GATHER_TILED or GATHER_LANES
     // photons in bucket order.  Lanes: (x, y, z, oct normal) (Phi, key); the
@@ -173,6 +181,9 @@ struct GatherArgs {
     float* out_rgb;   // [pixel id][3] raw estimates (test hook)
     Counters* counters;
     uint32_t* used;   // lanes kernel: used[j] = 1 for accepted photon records j (may be null)
+    // lanes kernel: per block of GATHER_BLOCK records the box (lo, hi) of their
+    // positions; blocks whose box is farther than the radius are skipped
+    const float4* boxes = nullptr;
 };
 __device__ __forceinline__ double gather_radius(const GatherArgs& a) { return a.g.pd ? a.g.pd->radius : a.radius; }
 void launch_gather(hipStream_t s, const GatherArgs& a);
@@ -188,6 +199,10 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
                      double radius, uint32_t* keys, uint32_t* vals);
 // gather layout of n photons: record k = photon perm[sorted[k]] (either may be
 // null); C null selects the compact two-word layout of the lanes kernel
+// boxes[2 k], boxes[2 k + 1] = (lo, hi) of the positions of records
+// [GATHER_BLOCK k, GATHER_BLOCK (k + 1)) of a (n records)
+void launch_gather_boxes(hipStream_t s, const float4* a, int64_t n, float4* boxes);
+inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GATHER_BLOCK; }
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
                           const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C);
 

@@ -39,7 +39,10 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
             const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             const int4 ch = reinterpret_cast<const int4*>(np)[6];
-            float tn[4], tf[4];
+            // one 32-bit key per child: the entry distance (non-negative, so
+            // its bit pattern orders like the value) with the child slot in the
+            // two low mantissa bits; a missed child keys above every distance
+            uint32_t key[4];
             const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
             const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
             const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
@@ -50,26 +53,27 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
                 float z0 = lza[c] * iz - oz, z1 = hza[c] * iz - oz;
                 float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
                 float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h.t));
-                tn[c] = (n <= f * SLAB_FAR_SCALE) ? n : __builtin_inff();
-                tf[c] = f;
+                key[c] = (n <= f * SLAB_FAR_SCALE) ? ((__float_as_uint(n) & 0x7FFFFFFCu) | (uint32_t)c)
+                                                   : (0xFFFFFFFCu | (uint32_t)c);
             }
-            int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
-            float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
-            // sorting network on (key, ref), ascending
-#define CSWAP(ka, ra, kb, rb)                     \
-    if (kb < ka) {                                \
-        float tk = ka; ka = kb; kb = tk;          \
-        int32_t tr = ra; ra = rb; rb = tr;        \
+            // sorting network, ascending, on min / max only (near child first)
+            uint32_t k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
+#define KSORT(a, b)                  \
+    {                                \
+        const uint32_t lo = min(a, b); \
+        b = max(a, b);               \
+        a = lo;                      \
     }
-            CSWAP(k0, r0, k1, r1);
-            CSWAP(k2, r2, k3, r3);
-            CSWAP(k0, r0, k2, r2);
-            CSWAP(k1, r1, k3, r3);
-            CSWAP(k1, r1, k2, r2);
-#undef CSWAP
-            const float INF = __builtin_inff();
-            int32_t push[3] = {r3, r2, r1};
-            bool pv[3] = {k3 != INF, k2 != INF, k1 != INF};
+            KSORT(k0, k1);
+            KSORT(k2, k3);
+            KSORT(k0, k2);
+            KSORT(k1, k3);
+            KSORT(k1, k2);
+#undef KSORT
+            const int32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
+            const uint32_t MISS = 0x7F800000u;
+            int32_t push[3] = {ref[k3 & 3u], ref[k2 & 3u], ref[k1 & 3u]};
+            bool pv[3] = {k3 < MISS, k2 < MISS, k1 < MISS};
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 if (pv[q]) {
@@ -84,8 +88,8 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
                     sp++;
                 }
             }
-            if (k0 != INF) {
-                cur = r0;
+            if (k0 < MISS) {
+                cur = ref[k0 & 3u];
                 continue;
             }
         } else if (cur != EMPTY_REF) {

@@ -59,7 +59,8 @@ class PassStats(C.Structure):
                 ("hitsort_ms", C.c_double), ("proposal_ms", C.c_double), ("train_loss", C.c_double),
                 ("proposal_records", C.c_int64), ("proposal_active", C.c_int32), ("retrained", C.c_int32),
                 ("irradiance_ms", C.c_double), ("dipole_ms", C.c_double), ("irradiance_photons", C.c_int64),
-                ("sss_hits", C.c_int64), ("sss_points", C.c_int64)]
+                ("sss_hits", C.c_int64), ("sss_points", C.c_int64),
+                ("gather_tested", C.c_int64), ("gather_blocks", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -311,6 +311,10 @@ def test_full_size_pass_properties(gpu_ctx):
         img = gpu_ctx.image()
         assert st_l["gather_inradius"] == st["gather_inradius"]
         assert st_l["gather_candidates"] == st["gather_candidates"]
+        # the whole-bucket kernels test every candidate; the lanes kernel skips
+        # the blocks whose box misses the ball, and accepts the same photons
+        assert st["gather_tested"] == st["gather_candidates"] and st["gather_blocks"] == 0
+        assert 0 < st_l["gather_tested"] < 0.6 * st_l["gather_candidates"] and st_l["gather_blocks"] > 0
         assert np.quantile(np.abs(img - img_lanes) / den, 0.999) < 1e-4
     n = st["photons_stored"]
     assert 1.2 * 4e6 < n < 8 * 4e6
