@@ -97,9 +97,13 @@ struct Ctx {
     hipEvent_t ev[13] = {};
     hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
     hipStream_t stream3 = nullptr;  // the gather of pipelined passes (pppm_render_passes)
-    hipEvent_t pev[2][11] = {};     // per buffer set: events of a pipelined pass
+    hipStream_t stream4 = nullptr;  // the eye passes of odd pipelined passes
+    hipEvent_t pev[6][11] = {};     // events of pipelined passes (a ring over passes)
     // the second buffer set of pipelined passes: what a pass's gather reads
     DBuf ph_a_alt, ph_b_alt, ph_c_alt, off_alt, hp_pos_s_alt, hp_w_s_alt, counters_alt, pass_counters, boxes_alt;
+    DBuf ph_a_alt2, ph_b_alt2, ph_c_alt2, off_alt2, hp_pos_s_alt2, hp_w_s_alt2, counters_alt2, boxes_alt2;
+    DBuf hp_pos_alt, hp_w_alt, hp_pos_alt2, hp_w_alt2;  // eye-pass pixel slots per set
+    DBuf hk_alt[4], hhist_alt;                          // hit point sort of the second eye stream
     // block boxes of the lanes gather over ph_a (GATHER_BLOCK records each)
     DBuf boxes;
     // subsurface scattering (BSSRDF materials)
@@ -228,7 +232,8 @@ int render_sub_bits(const Ctx* c, uint32_t size, bool sentinel) {
 }
 
 uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32_t* perm, int64_t n,
-                     const GridParams& g, int keep, const unsigned long long* count = nullptr) {
+                     const GridParams& g, int keep, const unsigned long long* count = nullptr,
+                     uint32_t* off_out = nullptr) {
     uint32_t* b[4];
     int k = 0;
     for (int i = 0; i < 5 && k < 4; i++)
@@ -241,7 +246,7 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
     bool alt = radix_sort_pairs(c->stream, b[0], b[1], b[2], b[3], c->hist.as<uint32_t>(), n, bits);
     uint32_t* keys = alt ? b[2] : b[0];
     uint32_t* vals = alt ? b[3] : b[1];
-    launch_bucket_offsets(c->stream, keys, n, g.size, c->off.as<uint32_t>(), shift);
+    launch_bucket_offsets(c->stream, keys, n, g.size, off_out ? off_out : c->off.as<uint32_t>(), shift);
     return vals;
 }
 
@@ -589,13 +594,18 @@ hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius) {
 // The hit points among the cap slots of (hp, hw), compacted into hp_pos_s /
 // hp_w_s in Morton order of their half cells (edge r); a stable sort, so the
 // order is a function of the slot order alone.
+void sort_hitpoints(hipStream_t s, DBuf* hk, DBuf& hhist, const float4* hp, const float4* hw, int64_t cap,
+                    const GridParams& g, double radius, float4* out_pos, float4* out_w) {
+    uint32_t* k[4];
+    for (int i = 0; i < 4; i++) k[i] = hk[i].as<uint32_t>();
+    launch_hit_keys(s, hp, hw, cap, g, radius, k[0], k[1]);
+    bool alt = radix_sort_pairs(s, k[0], k[1], k[2], k[3], hhist.as<uint32_t>(), cap, 32);
+    launch_permute2(s, hp, hw, nullptr, alt ? k[3] : k[1], cap, out_pos, out_w);
+}
+
 void sort_hitpoints(Ctx* c, hipStream_t s, const float4* hp, const float4* hw, int64_t cap, const GridParams& g,
                     double radius) {
-    uint32_t* k[4];
-    for (int i = 0; i < 4; i++) k[i] = c->hk[i].as<uint32_t>();
-    launch_hit_keys(s, hp, hw, cap, g, radius, k[0], k[1]);
-    bool alt = radix_sort_pairs(s, k[0], k[1], k[2], k[3], c->hhist.as<uint32_t>(), cap, 32);
-    launch_permute2(s, hp, hw, nullptr, alt ? k[3] : k[1], cap, c->hp_pos_s.as<float4>(), c->hp_w_s.as<float4>());
+    sort_hitpoints(s, c->hk, c->hhist, hp, hw, cap, g, radius, c->hp_pos_s.as<float4>(), c->hp_w_s.as<float4>());
 }
 
 void copy_ranges(const std::vector<uint32_t>& off, int32_t S, int32_t* cs, int32_t* ce) {
@@ -677,9 +687,11 @@ int pppm_create(void** out, int32_t device) {
         delete c;
         return PPPM_ERR_HIP;
     }
-    if (hipStreamCreate(&c->stream2) != hipSuccess || hipStreamCreate(&c->stream3) != hipSuccess) {
+    if (hipStreamCreate(&c->stream2) != hipSuccess || hipStreamCreate(&c->stream3) != hipSuccess ||
+        hipStreamCreate(&c->stream4) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         if (c->stream2) (void)hipStreamDestroy(c->stream2);
+        if (c->stream3) (void)hipStreamDestroy(c->stream3);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PPPM_ERR_HIP;
@@ -705,9 +717,11 @@ int pppm_destroy(void* ctx_raw) {
     c->counters.release(); c->scratch.release();
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream3);
+    (void)hipStreamSynchronize(c->stream4);
     for (auto& ev : c->ev) (void)hipEventDestroy(ev);
     for (auto& set : c->pev)
         for (auto& ev : set) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream4);
     (void)hipStreamDestroy(c->stream3);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
@@ -1344,13 +1358,24 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     return PPPM_OK;
 }
 
-// Pipelined passes: three streams -- photons and sorts (stream), eye pass and
-// hit point sort (stream2), gather (stream3) -- so that the gather of pass i
-// runs while the photons of pass i + 1 are traced.  What a gather reads (the
-// gather layout, bucket offsets, sorted hit points, counters) alternates
-// between two buffer sets; everything else is single and ordered by its
-// stream.  The gathers run in pass order, so the image equals that of the
-// same number of pppm_render_pass calls bit for bit.
+// Pipelined passes on four streams:
+//   A (stream):  photon pass, then the photon order, hash grid, gather layout;
+//   B0 / B1 (stream2 / stream4, even / odd passes): eye pass, hit point sort;
+//   C (stream3): the direct emission the eye pass saw, then the gather.
+// The eye side of a pass does not depend on its photons, so it is enqueued two
+// passes ahead, and consecutive eye passes (whose last long specular paths
+// keep few waves busy) overlap on their two streams.  What the C side reads
+// (pixel slots, sorted hit points, gather layout, bucket offsets, block boxes,
+// counters) rotates over three buffer sets; a set is refilled only after the
+// gather three passes back has finished.  C adds every pass's direct emission
+// and gather in pass order, so the image equals that of the same number of
+// pppm_render_pass calls bit for bit.
+namespace {
+struct PassSet {
+    DBuf *ph_a, *ph_b, *ph_c, *off, *boxes, *hp_pos, *hp_w, *hp_pos_s, *hp_w_s, *counters;
+};
+}  // namespace
+
 int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     CHECK_CTX(ctx);
     if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "pppm_render_begin not called");
@@ -1367,33 +1392,51 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         return PPPM_OK;
     }
     (void)hipSetDevice(ctx->device);
-    TRY_HIP(ctx, ctx->ph_a_alt.ensure(ctx->ph_a.n));
-    TRY_HIP(ctx, ctx->ph_b_alt.ensure(ctx->ph_b.n));
-    TRY_HIP(ctx, ctx->ph_c_alt.ensure(ctx->ph_c.n));
-    TRY_HIP(ctx, ctx->off_alt.ensure(ctx->off.n));
-    TRY_HIP(ctx, ctx->hp_pos_s_alt.ensure(ctx->hp_pos_s.n));
-    TRY_HIP(ctx, ctx->hp_w_s_alt.ensure(ctx->hp_w_s.n));
-    TRY_HIP(ctx, ctx->counters_alt.ensure(ctx->counters.n));
-    TRY_HIP(ctx, ctx->boxes_alt.ensure(ctx->boxes.n));
-    TRY_HIP(ctx, ctx->pass_counters.ensure(sizeof(Counters) * (size_t)count));
-    // the alternate set must be sized as the current one for every later pass
-    if (ctx->off_alt.n < ctx->off.n || ctx->ph_a_alt.n < ctx->ph_a.n || ctx->boxes_alt.n < ctx->boxes.n)
-        return ctx->fail(PPPM_ERR_NOMEM, "second buffer set");
-    auto swap_buf = [](DBuf& a, DBuf& b) {
-        std::swap(a.p, b.p);
-        std::swap(a.n, b.n);
-    };
-    hipStream_t A = ctx->stream, B = ctx->stream2, Cs = ctx->stream3;
-    enum { ZERO, P0, P1, O0, ODONE, E0, E1, HDONE, G0, GEND, GDONE };
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
+    PassSet set[3] = {
+        {&ctx->ph_a, &ctx->ph_b, &ctx->ph_c, &ctx->off, &ctx->boxes, &ctx->hp_pos, &ctx->hp_w, &ctx->hp_pos_s,
+         &ctx->hp_w_s, &ctx->counters},
+        {&ctx->ph_a_alt, &ctx->ph_b_alt, &ctx->ph_c_alt, &ctx->off_alt, &ctx->boxes_alt, &ctx->hp_pos_alt,
+         &ctx->hp_w_alt, &ctx->hp_pos_s_alt, &ctx->hp_w_s_alt, &ctx->counters_alt},
+        {&ctx->ph_a_alt2, &ctx->ph_b_alt2, &ctx->ph_c_alt2, &ctx->off_alt2, &ctx->boxes_alt2, &ctx->hp_pos_alt2,
+         &ctx->hp_w_alt2, &ctx->hp_pos_s_alt2, &ctx->hp_w_s_alt2, &ctx->counters_alt2}};
+    for (int q = 1; q < 3; q++) {
+        TRY_HIP(ctx, set[q].ph_a->ensure(ctx->ph_a.n));
+        TRY_HIP(ctx, set[q].ph_b->ensure(ctx->ph_b.n));
+        TRY_HIP(ctx, set[q].ph_c->ensure(ctx->ph_c.n));
+        TRY_HIP(ctx, set[q].off->ensure(ctx->off.n));
+        TRY_HIP(ctx, set[q].boxes->ensure(ctx->boxes.n));
+        TRY_HIP(ctx, set[q].hp_pos->ensure(ctx->hp_pos.n));
+        TRY_HIP(ctx, set[q].hp_w->ensure(ctx->hp_w.n));
+        TRY_HIP(ctx, set[q].hp_pos_s->ensure(ctx->hp_pos_s.n));
+        TRY_HIP(ctx, set[q].hp_w_s->ensure(ctx->hp_w_s.n));
+        TRY_HIP(ctx, set[q].counters->ensure(ctx->counters.n));
+        // every later pass of this render must fit the set
+        if (set[q].off->n < ctx->off.n || set[q].ph_a->n < ctx->ph_a.n || set[q].boxes->n < ctx->boxes.n)
+            return ctx->fail(PPPM_ERR_NOMEM, "pipelined buffer set");
+    }
+    for (int i = 0; i < 4; i++) TRY_HIP(ctx, ctx->hk_alt[i].ensure(ctx->hk[i].n));
+    TRY_HIP(ctx, ctx->hhist_alt.ensure(ctx->hhist.n));
+    TRY_HIP(ctx, ctx->pass_counters.ensure(sizeof(Counters) * (size_t)count));
+    hipStream_t A = ctx->stream, Cs = ctx->stream3;
+    hipStream_t Bq[2] = {ctx->stream2, ctx->stream4};
+    DBuf* hkq[2] = {ctx->hk, ctx->hk_alt};
+    DBuf* hhq[2] = {&ctx->hhist, &ctx->hhist_alt};
+    enum { ZERO, P0, P1, O0, ODONE, E0, E1, HDONE, G0, GEND, GDONE };
+    const int NEV = 6;  // event ring: pass k records into pev[k % NEV]
     const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
     const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
+    const int32_t first = ctx->passes_done + 1;  // pass index of k = 0
     std::vector<Counters> hc_pass((size_t)count);
     std::vector<float> times((size_t)count * 6, 0.0f);
     std::vector<int32_t> sizes((size_t)count, 0);
     std::vector<double> rads((size_t)count, 0.0);
+    for (int32_t k = 0; k < count; k++) rads[(size_t)k] = ctx->radii[(size_t)(first + k) - 1];
+    std::vector<bool> timed((size_t)count, false);
     auto read_times = [&](int32_t k) {
-        hipEvent_t* E = ctx->pev[k & 1];
+        if (timed[(size_t)k]) return;
+        timed[(size_t)k] = true;
+        hipEvent_t* E = ctx->pev[k % NEV];
         (void)hipEventSynchronize(E[GDONE]);
         float* t = &times[(size_t)k * 6];
         (void)hipEventElapsedTime(&t[0], E[P0], E[P1]);
@@ -1403,43 +1446,47 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         (void)hipEventElapsedTime(&t[4], E[G0], E[GEND]);
         (void)hipEventElapsedTime(&t[5], E[ZERO], E[GDONE]);
     };
-    for (int32_t k = 0; k < count; k++) {
-        if (k > 0) {
-            swap_buf(ctx->ph_a, ctx->ph_a_alt);
-            swap_buf(ctx->ph_b, ctx->ph_b_alt);
-            swap_buf(ctx->ph_c, ctx->ph_c_alt);
-            swap_buf(ctx->off, ctx->off_alt);
-            swap_buf(ctx->hp_pos_s, ctx->hp_pos_s_alt);
-            swap_buf(ctx->hp_w_s, ctx->hp_w_s_alt);
-            swap_buf(ctx->counters, ctx->counters_alt);
-            swap_buf(ctx->boxes, ctx->boxes_alt);
-        }
-        hipEvent_t* E = ctx->pev[k & 1];
-        if (k >= 2) {
-            // this buffer set was last read by the gather of pass k - 2
-            read_times(k - 2);
-            TRY_HIP(ctx, hipStreamWaitEvent(A, E[GDONE], 0));
-            TRY_HIP(ctx, hipStreamWaitEvent(B, E[GDONE], 0));
-        }
-        const int32_t pass = ctx->passes_done + 1;
-        const double radius = ctx->radii[(size_t)pass - 1];
-        rads[(size_t)k] = radius;
-        Counters* dc = ctx->counters.as<Counters>();
-        launch_zero_u32(A, dc, sizeof(Counters) / 4);
-        TRY_HIP(ctx, hipEventRecord(E[ZERO], A));
-        // eye side
-        TRY_HIP(ctx, hipStreamWaitEvent(B, E[ZERO], 0));
-        TRY_HIP(ctx, hipEventRecord(E[E0], B));
-        EyePassArgs ea = eye_args(ctx, (uint32_t)pass);
+    // eye side of pass k on B[k % 2]: counters zeroed, eye pass into the pixel
+    // slots of set k % 3, hit point sort into the set's sorted buffers
+    auto enqueue_eye = [&](int32_t k) -> hipError_t {
+        const PassSet& S = set[k % 3];
+        hipStream_t B = Bq[k & 1];
+        hipEvent_t* E = ctx->pev[k % NEV];
+        hipError_t e;
+        if (k >= 3 && (e = hipStreamWaitEvent(B, ctx->pev[(k - 3) % NEV][GDONE], 0)) != hipSuccess) return e;
+        launch_zero_u32(B, S.counters->p, sizeof(Counters) / 4);
+        if ((e = hipEventRecord(E[ZERO], B)) != hipSuccess) return e;
+        if ((e = hipEventRecord(E[E0], B)) != hipSuccess) return e;
+        EyePassArgs ea = eye_args(ctx, (uint32_t)(first + k));
+        ea.hp_pos = S.hp_pos->as<float4>();
+        ea.hp_w = S.hp_w->as<float4>();
+        ea.counters = S.counters->as<Counters>();
+        ea.img_sum = nullptr;  // the direct emission stays in the slots for C
+        ea.img_comp = nullptr;
         launch_eye_pass(B, ea, grid_limit);
-        TRY_HIP(ctx, hipEventRecord(E[E1], B));
+        if ((e = hipEventRecord(E[E1], B)) != hipSuccess) return e;
         GridParams gh;
         for (int d = 0; d < 3; d++) gh.bmin[d] = ctx->grid_bmin[d];
-        gh.scale = 1.0 / (2.0 * radius);
+        gh.scale = 1.0 / (2.0 * rads[(size_t)k]);
         gh.size = 1;
-        sort_hitpoints(ctx, B, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, gh, radius);
-        TRY_HIP(ctx, hipEventRecord(E[HDONE], B));
-        // photon side
+        sort_hitpoints(B, hkq[k & 1], *hhq[k & 1], S.hp_pos->as<float4>(), S.hp_w->as<float4>(), npix, gh,
+                       rads[(size_t)k], S.hp_pos_s->as<float4>(), S.hp_w_s->as<float4>());
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return hipEventRecord(E[HDONE], B);
+    };
+    TRY_HIP(ctx, enqueue_eye(0));
+    if (count > 1) TRY_HIP(ctx, enqueue_eye(1));
+    for (int32_t k = 0; k < count; k++) {
+        // the event set of pass k + 2 was last used by pass k - 4
+        if (k >= 3) read_times(k - 3);
+        if (k + 2 < count) TRY_HIP(ctx, enqueue_eye(k + 2));
+        const PassSet& S = set[k % 3];
+        hipEvent_t* E = ctx->pev[k % NEV];
+        const int32_t pass = first + k;
+        const double radius = rads[(size_t)k];
+        Counters* dc = S.counters->as<Counters>();
+        // photon side: the counters are zeroed on the eye stream
+        TRY_HIP(ctx, hipStreamWaitEvent(A, E[ZERO], 0));
         TRY_HIP(ctx, hipEventRecord(E[P0], A));
         PhotonPassArgs pa;
         pa.sc = ctx->scene_dev();
@@ -1462,10 +1509,10 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
-        // the stored count sizes the sorts; meanwhile the previous gather runs
+        // the stored count sizes the sorts; meanwhile the eye passes and the
+        // previous gather run
         Counters& hc = hc_pass[(size_t)k];
         TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, A));
-        TRY_HIP(ctx, hipEventSynchronize(E[P1]));
         TRY_HIP(ctx, hipStreamSynchronize(A));
         const int64_t n = (int64_t)hc.photons_stored;
         if (n > ctx->cap) {
@@ -1483,34 +1530,36 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
                                           ctx->hist.as<uint32_t>(), n, obits);
         const int keep = alt ? 3 : 1;
         uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
-        const uint32_t S = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
-        sizes[(size_t)k] = (int32_t)S;
+        const uint32_t Sz = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
+        sizes[(size_t)k] = (int32_t)Sz;
         GridParams g;
         for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
         g.scale = 1.0 / (2.0 * radius);
-        g.size = S;
-        g.sub_bits = render_sub_bits(ctx, S, false);
+        g.size = Sz;
+        g.sub_bits = render_sub_bits(ctx, Sz, false);
         ctx->stream = A;
-        uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
+        uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr,
+                                      S.off->as<uint32_t>());
         ctx->stream = keep_stream;
         launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
-                             ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
-                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
-        if (g.sub_bits > 0) launch_gather_boxes(A, ctx->ph_a.as<float4>(), n, ctx->boxes.as<float4>());
+                             S.ph_a->as<float4>(), S.ph_b->as<float4>(),
+                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>());
+        if (g.sub_bits > 0) launch_gather_boxes(A, S.ph_a->as<float4>(), n, S.boxes->as<float4>());
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
-        // gather side, in pass order
+        // C: direct emission and gather of this pass, in pass order
         TRY_HIP(ctx, hipStreamWaitEvent(Cs, E[ODONE], 0));
         TRY_HIP(ctx, hipStreamWaitEvent(Cs, E[HDONE], 0));
         TRY_HIP(ctx, hipEventRecord(E[G0], Cs));
+        launch_direct_add(Cs, S.hp_w->as<float4>(), npix, ctx->img_dsum.as<float>(), ctx->img_dcomp.as<float>());
         GatherArgs ga;
         ga.mode = ctx->rp.gather_mode;
-        ga.ph_a = ctx->ph_a.as<float4>();
-        ga.ph_b = ctx->ph_b.as<float4>();
-        ga.ph_c = ctx->ph_c.as<float4>();
-        ga.off = ctx->off.as<uint32_t>();
-        ga.hp_pos = ctx->hp_pos_s.as<float4>();
-        ga.hp_w = ctx->hp_w_s.as<float4>();
+        ga.ph_a = S.ph_a->as<float4>();
+        ga.ph_b = S.ph_b->as<float4>();
+        ga.ph_c = S.ph_c->as<float4>();
+        ga.off = S.off->as<uint32_t>();
+        ga.hp_pos = S.hp_pos_s->as<float4>();
+        ga.hp_w = S.hp_w_s->as<float4>();
         ga.hp_count_src = dc;
         ga.n_hit = npix;
         ga.g = g;
@@ -1520,7 +1569,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ga.out_rgb = nullptr;
         ga.counters = dc;
         ga.used = nullptr;
-        ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
+        ga.boxes = g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr;
         launch_gather(Cs, ga);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[GEND], Cs));
@@ -1530,15 +1579,34 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ctx->passes_done = pass;
         ctx->last_n = n;
         ctx->last_n_irr = 0;
-        ctx->last_S = (int32_t)S;
+        ctx->last_S = (int32_t)Sz;
         ctx->last_scale = g.scale;
         ctx->last_perm1 = perm1;
         ctx->last_sorted = sorted;
         ctx->last_sub_bits = g.sub_bits;
     }
-    for (int32_t k = std::max(0, count - 2); k < count; k++) read_times(k);
+    for (int32_t k = std::max(0, count - 3); k < count; k++) read_times(k);
     TRY_HIP(ctx, hipStreamSynchronize(Cs));
-    TRY_HIP(ctx, hipStreamSynchronize(B));
+    TRY_HIP(ctx, hipStreamSynchronize(Bq[0]));
+    TRY_HIP(ctx, hipStreamSynchronize(Bq[1]));
+    // the last pass's set becomes the current one (what pppm_last_* read)
+    const int last = (count - 1) % 3;
+    if (last != 0) {
+        auto swap_buf = [](DBuf* a, DBuf* b) {
+            std::swap(a->p, b->p);
+            std::swap(a->n, b->n);
+        };
+        swap_buf(set[0].ph_a, set[last].ph_a);
+        swap_buf(set[0].ph_b, set[last].ph_b);
+        swap_buf(set[0].ph_c, set[last].ph_c);
+        swap_buf(set[0].off, set[last].off);
+        swap_buf(set[0].boxes, set[last].boxes);
+        swap_buf(set[0].hp_pos, set[last].hp_pos);
+        swap_buf(set[0].hp_w, set[last].hp_w);
+        swap_buf(set[0].hp_pos_s, set[last].hp_pos_s);
+        swap_buf(set[0].hp_w_s, set[last].hp_w_s);
+        swap_buf(set[0].counters, set[last].counters);
+    }
     std::vector<Counters> dcs((size_t)count);
     TRY_HIP(ctx, hipMemcpy(dcs.data(), ctx->pass_counters.p, sizeof(Counters) * (size_t)count, hipMemcpyDeviceToHost));
     for (int32_t k = 0; k < count; k++)
@@ -1550,7 +1618,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             const Counters& c = dcs[(size_t)k];
             const float* t = &times[(size_t)k * 6];
             std::memset(&st, 0, sizeof(st));
-            st.pass_index = ctx->passes_done - count + k + 1;
+            st.pass_index = first + k;
             st.hash_size = sizes[(size_t)k];
             st.radius = rads[(size_t)k];
             st.photons_emitted = ctx->rp.photons;

@@ -140,8 +140,8 @@ struct EyePassArgs {
     int32_t max_bounces;
     float ray_eps;
     float4* hp_pos;  // xyz, oct normal bits
-    float4* hp_w;    // weight rgb, pixel id bits
-    float* img_sum;
+    float4* hp_w;    // weight rgb, pixel id bits (no hit point: the direct emission, id all ones)
+    float* img_sum;   // direct emission, Kahan-added (null: left in the pixel slots)
     float* img_comp;
     Counters* counters;
     const PassDev* pd = nullptr;  // when set, the pass index is read from it
@@ -152,6 +152,9 @@ struct EyePassArgs {
     int64_t sss_cap;
 };
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
+// img += the direct emission stored in the pixel slots without hit point
+// (an eye pass run with img_sum = null)
+void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp);
 
 #define GATHER_TILED PPPM_GATHER_TILED
 #define GATHER_LANES PPPM_GATHER_LANES

@@ -287,10 +287,13 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
             d = nd;
         }
         // the hit point of pixel p lives in slot p (pixel id all ones: none), so
-        // the hit point order entering the Morton sort is the pixel order
+        // the hit point order entering the Morton sort is the pixel order; a
+        // slot without hit point carries the emission the path saw directly
+        // (a path ends at a hit point or at an emitter, never both)
+        if (__float_as_uint(hit_w.w) == NO_HITPOINT) hit_w = make_float4(direct.x, direct.y, direct.z, hit_w.w);
         a.hp_pos[p] = hit_pos;
         a.hp_w[p] = hit_w;
-        if (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f) {
+        if (a.img_sum && (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f)) {
             kahan_add(a.img_sum, a.img_comp, 3 * p + 0, direct.x);
             kahan_add(a.img_sum, a.img_comp, 3 * p + 1, direct.y);
             kahan_add(a.img_sum, a.img_comp, 3 * p + 2, direct.z);
@@ -303,6 +306,21 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         atomicAdd(&a.counters->eye_segments, segs);
         atomicAdd(&a.counters->hitpoints, nhit);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
+    }
+}
+
+// The direct emission of an eye pass run with img_sum = null, added from the
+// pixel slots later and in pass order: the same Kahan additions, in the same
+// order per pixel, as the eye kernel makes when it adds them itself.
+__global__ void k_direct_add(const float4* __restrict__ hp_w, int64_t np, float* img_sum, float* img_comp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const float4 w = hp_w[p];
+    if (__float_as_uint(w.w) != NO_HITPOINT) return;
+    if (w.x != 0.0f || w.y != 0.0f || w.z != 0.0f) {
+        kahan_add(img_sum, img_comp, 3 * p + 0, w.x);
+        kahan_add(img_sum, img_comp, 3 * p + 1, w.y);
+        kahan_add(img_sum, img_comp, 3 * p + 2, w.z);
     }
 }
 
@@ -379,6 +397,11 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
     unsigned g = grid1(np, 256);
     if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
     hipLaunchKernelGGL(k_eye, dim3(g), dim3(256), 0, s, a);
+}
+
+void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_direct_add, dim3(grid1(np, 256)), dim3(256), 0, s, hp_w, np, img_sum, img_comp);
 }
 
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,
