@@ -246,7 +246,9 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
     bool alt = radix_sort_pairs(c->stream, b[0], b[1], b[2], b[3], c->hist.as<uint32_t>(), n, bits);
     uint32_t* keys = alt ? b[2] : b[0];
     uint32_t* vals = alt ? b[3] : b[1];
-    launch_bucket_offsets(c->stream, keys, n, g.size, off_out ? off_out : c->off.as<uint32_t>(), shift);
+    // the radix histogram is free again: scratch of the offsets' suffix minimum
+    launch_bucket_offsets(c->stream, keys, n, g.size, off_out ? off_out : c->off.as<uint32_t>(), shift,
+                          c->hist.as<uint32_t>(), (int64_t)(c->hist.n / sizeof(uint32_t)));
     return vals;
 }
 

@@ -85,6 +85,96 @@ __global__ void k_bucket_offsets(const uint32_t* __restrict__ keys, int64_t n, u
     for (int64_t h = lo; h <= hi; h++) off[h] = (uint32_t)j;
 }
 
+// Faster form for sparse grids: off = n everywhere, then off[b] = first record
+// of each non-empty bucket b (and off[size] = the first sentinel slot), then a
+// suffix minimum over off[0..size] fills every empty bucket with the start of
+// the next non-empty one.
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t m, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[i] = v;
+}
+
+__global__ void k_mark_bucket_starts(const uint32_t* __restrict__ keys, int64_t n, uint32_t size, int shift,
+                                     uint32_t* __restrict__ off) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t b = keys[j] >> shift;
+    if (b >= size) {
+        // sentinel slots (unused records) follow the real ones: the first
+        // closes the grid
+        if (j == 0 || (keys[j - 1] >> shift) < size) off[size] = (uint32_t)j;
+        return;
+    }
+    if (j == 0 || (keys[j - 1] >> shift) != b) off[b] = (uint32_t)j;
+}
+
+#define SMIN_ITEMS 16
+#define SMIN_TILE (256 * SMIN_ITEMS)
+
+// Suffix minimum inside tiles of SMIN_TILE entries; tile minima to tmin.
+__global__ __launch_bounds__(256) void k_suffix_min_tiles(uint32_t* __restrict__ a, int64_t m,
+                                                          uint32_t* __restrict__ tmin) {
+    __shared__ uint32_t s[256];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * SMIN_TILE + (int64_t)tid * SMIN_ITEMS;
+    uint32_t v[SMIN_ITEMS];
+    uint32_t run = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = SMIN_ITEMS - 1; k >= 0; k--) {
+        const uint32_t x = base + k < m ? a[base + k] : 0xFFFFFFFFu;
+        run = min(run, x);
+        v[k] = run;
+    }
+    // suffix minimum of the thread minima, right to left
+    s[tid] = run;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = tid + off < 256 ? s[tid + off] : 0xFFFFFFFFu;
+        __syncthreads();
+        s[tid] = min(s[tid], t);
+        __syncthreads();
+    }
+    const uint32_t right = tid + 1 < 256 ? s[tid + 1] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < SMIN_ITEMS; k++)
+        if (base + k < m) a[base + k] = min(v[k], right);
+    if (tid == 0) tmin[blockIdx.x] = s[0];
+}
+
+// One workgroup: suffix minimum of the tile minima, in place.
+__global__ __launch_bounds__(256) void k_suffix_min_top(uint32_t* __restrict__ tmin, int64_t nt) {
+    __shared__ uint32_t s[256];
+    __shared__ uint32_t carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) carry = 0xFFFFFFFFu;
+    __syncthreads();
+    for (int64_t c = ((nt + 255) / 256 - 1) * 256; c >= 0; c -= 256) {
+        const int64_t i = c + tid;
+        s[tid] = i < nt ? tmin[i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const uint32_t t = tid + off < 256 ? s[tid + off] : 0xFFFFFFFFu;
+            __syncthreads();
+            s[tid] = min(s[tid], t);
+            __syncthreads();
+        }
+        const uint32_t cv = carry;
+        if (i < nt) tmin[i] = min(s[tid], cv);
+        __syncthreads();
+        if (tid == 0) carry = min(s[0], cv);
+        __syncthreads();
+    }
+}
+
+// a[i] = min(a[i], suffix minimum of the tiles to the right of i's tile)
+__global__ void k_suffix_min_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t* __restrict__ tmin,
+                                   int64_t nt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t t = i / SMIN_TILE;
+    if (t + 1 < nt) a[i] = min(a[i], tmin[t + 1]);
+}
+
 __device__ inline void box_cells(const double* bmin_items, const double* bmax_items, int64_t i,
                                  const GridParams& g, int32_t lo[3], int32_t hi[3]) {
     for (int d = 0; d < 3; d++) {
@@ -170,9 +260,22 @@ void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, con
 }
 
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
-                           uint32_t* off, int shift) {
-    hipLaunchKernelGGL(k_bucket_offsets, dim3(grid1(n + 1, 256)), dim3(256), 0, s, sorted_keys, n,
-                       size, shift, off);
+                           uint32_t* off, int shift, uint32_t* scratch, int64_t scratch_words) {
+    const int64_t m = (int64_t)size + 1;
+    const int64_t nt = (m + SMIN_TILE - 1) / SMIN_TILE;
+    if (!scratch || scratch_words < nt) {
+        // one thread per bucket boundary fills the empty buckets before it
+        hipLaunchKernelGGL(k_bucket_offsets, dim3(grid1(n + 1, 256)), dim3(256), 0, s, sorted_keys, n,
+                           size, shift, off);
+        return;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid1(m, 256)), dim3(256), 0, s, off, m, (uint32_t)n);
+    if (n > 0)
+        hipLaunchKernelGGL(k_mark_bucket_starts, dim3(grid1(n, 256)), dim3(256), 0, s, sorted_keys, n, size, shift,
+                           off);
+    hipLaunchKernelGGL(k_suffix_min_tiles, dim3((unsigned)nt), dim3(256), 0, s, off, m, scratch);
+    hipLaunchKernelGGL(k_suffix_min_top, dim3(1), dim3(256), 0, s, scratch, nt);
+    hipLaunchKernelGGL(k_suffix_min_apply, dim3(grid1(m, 256)), dim3(256), 0, s, off, m, scratch, nt);
 }
 
 void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,

@@ -55,8 +55,9 @@ void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, con
                        int64_t n, GridParams g, uint32_t* keys, uint32_t* vals,
                        const unsigned long long* count = nullptr, uint32_t sentinel = 0xFFFFFFFFu);
 // off[h] = number of sorted keys whose bucket (key >> shift) is < h, for h = 0..size
+// (scratch: at least (size + 1) / 4096 + 1 free words, or null)
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
-                           uint32_t* off, int shift = 0);
+                           uint32_t* off, int shift = 0, uint32_t* scratch = nullptr, int64_t scratch_words = 0);
 // box insertion: per-item pair counts, then pairs in z, y, x order
 void launch_box_counts(hipStream_t s, const double* bmin_items, const double* bmax_items,
                        int64_t n, GridParams g, uint64_t* counts);

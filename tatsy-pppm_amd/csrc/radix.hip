@@ -7,7 +7,9 @@
 //      inside a wave, keys with equal digits are matched with 8 ballots, waves
 //      are ordered through a per-round LDS table, so the output order of equal
 //      digits is the input order (the property the hash grid needs: a bucket
-//      lists its items in insertion order).
+//      lists its items in insertion order).  The ranked tile is laid out in
+//      LDS in digit order and written from there, so the stores of a wave fall
+//      in runs of consecutive positions.
 #include "internal.h"
 
 namespace {
@@ -67,10 +69,13 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_scatter(
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals, int64_t n, int shift,
     int64_t nblocks) {
     constexpr int WAVES = RADIX_BLOCK / 64;
-    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_base[256];    // global position of this tile's first key of each digit
+    __shared__ uint32_t s_start[256];   // position in the tile of its first key of each digit
     __shared__ uint32_t s_run[256];
     __shared__ uint32_t s_cnt[WAVES][256];
     __shared__ uint32_t s_scan[256];
+    __shared__ uint32_t s_key[RADIX_TILE];
+    __shared__ uint32_t s_val[RADIX_TILE];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -88,16 +93,22 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_scatter(
     s_base[tid] = (s_scan[tid] - tot) + hist[(int64_t)tid * nblocks + blockIdx.x];
     s_run[tid] = 0;
 
+    // 1. stable rank of every key among the tile's keys of its digit: rounds of
+    //    256 consecutive keys; inside a wave equal digits are matched with 8
+    //    ballots, waves are ordered through a per-round LDS table
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
+    uint32_t key[RADIX_ITEMS], val[RADIX_ITEMS], dig[RADIX_ITEMS], idx[RADIX_ITEMS];
+#pragma unroll
     for (int r = 0; r < RADIX_ITEMS; r++) {
         for (int w = 0; w < WAVES; w++) s_cnt[w][tid] = 0;
         __syncthreads();
         const int64_t i = base + (int64_t)r * RADIX_BLOCK + tid;
         const bool valid = i < n;
-        uint32_t key = valid ? keys_in[i] : 0u;
-        uint32_t val = valid ? vals_in[i] : 0u;
-        uint32_t d = (key >> shift) & 0xFFu;
+        key[r] = valid ? keys_in[i] : 0u;
+        val[r] = valid ? vals_in[i] : 0u;
+        const uint32_t d = (key[r] >> shift) & 0xFFu;
+        dig[r] = valid ? d : 0xFFFFFFFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -105,7 +116,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_scatter(
             uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t rank = __popcll(peers & lt_mask);
         if (valid && (63 - __clzll(peers)) == lane) s_cnt[wave][d] = __popcll(peers);
         __syncthreads();
         {
@@ -119,12 +130,44 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_scatter(
             s_run[tid] = run;
         }
         __syncthreads();
-        if (valid) {
-            uint32_t pos = s_base[d] + s_cnt[wave][d] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
-        }
+        idx[r] = valid ? s_cnt[wave][d] + rank : 0u;
+        __syncthreads();  // s_cnt is cleared by the next round
+    }
+    // 2. the tile in digit order in LDS: s_start = exclusive scan of the
+    //    tile's digit counts (s_run)
+    const uint32_t cnt = s_run[tid];
+    s_scan[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t t = tid >= off ? s_scan[tid - off] : 0u;
         __syncthreads();
+        s_scan[tid] += t;
+        __syncthreads();
+    }
+    s_start[tid] = s_scan[tid] - cnt;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RADIX_ITEMS; r++) {
+        if (dig[r] != 0xFFFFFFFFu) {
+            const uint32_t t = s_start[dig[r]] + idx[r];
+            s_key[t] = key[r];
+            s_val[t] = val[r];
+        }
+    }
+    __syncthreads();
+    // 3. write out in tile order: keys of one digit go to consecutive
+    //    positions, so a wave's stores fall in runs
+    const int64_t m = min((int64_t)RADIX_TILE, n - base);
+#pragma unroll
+    for (int r = 0; r < RADIX_ITEMS; r++) {
+        const int t = r * RADIX_BLOCK + tid;
+        if (t < m) {
+            const uint32_t k = s_key[t];
+            const uint32_t d = (k >> shift) & 0xFFu;
+            const uint32_t pos = s_base[d] + (uint32_t)t - s_start[d];
+            keys_out[pos] = k;
+            vals_out[pos] = s_val[t];
+        }
     }
 }
 
