@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
     const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
-    unsigned long long nb = 0, nc = 0, nt = 0, nk = 0;
+    uint32_t nb = 0, nc = 0, nt = 0, nk = 0;
     uint32_t ni = 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     if (valid) {
@@ -244,16 +244,17 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
             const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
             nb++;
             nc += b1 - b0;
-            if (!a.boxes) {
-                nt += b1 - b0;
-                lanes_range(a, b0, b1, x, nx, r2, inv_kr, ax, ay, az, ni);
-                continue;
-            }
             if (b1 <= b0) continue;
-            for (uint32_t k = b0 / GATHER_BLOCK; k <= (b1 - 1) / GATHER_BLOCK; k++) {
-                nk++;
-                if (box_dist2(x, a.boxes[2 * k], a.boxes[2 * k + 1]) > r2_box) continue;
-                const uint32_t j0 = max(b0, k * GATHER_BLOCK), j1 = min(b1, (k + 1) * GATHER_BLOCK);
+            // segments: the whole bucket, or its blocks whose box meets the ball
+            const bool boxed = a.boxes != nullptr;
+            const uint32_t k0 = boxed ? b0 / GATHER_BLOCK : 0, k1 = boxed ? (b1 - 1) / GATHER_BLOCK : 0;
+            for (uint32_t k = k0; k <= k1; k++) {
+                if (boxed) {
+                    nk++;
+                    if (box_dist2(x, a.boxes[2 * k], a.boxes[2 * k + 1]) > r2_box) continue;
+                }
+                const uint32_t j0 = boxed ? max(b0, k * GATHER_BLOCK) : b0;
+                const uint32_t j1 = boxed ? min(b1, (k + 1) * GATHER_BLOCK) : b1;
                 nt += j1 - j0;
                 lanes_range(a, j0, j1, x, nx, r2, inv_kr, ax, ay, az, ni);
             }
