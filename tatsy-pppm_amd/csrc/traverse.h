@@ -4,6 +4,10 @@
 // LDS (laid out [slot][lane] so the 64 lanes of a wave hit 64 different banks)
 // and spills deeper entries to a per-lane private array.  Nodes are fetched as
 // seven 16-byte loads from one 128-byte line; children are visited near-first.
+// The loop is the while-while form with postponed leaves of Aila and Laine
+// (2009): a lane that reaches its first leaf keeps it waiting and goes on
+// through inner nodes until every lane of the wave has a leaf, so that node
+// steps and triangle tests each run with most lanes of the wave busy.
 #pragma once
 
 #include "pppm_device.h"
@@ -33,9 +37,17 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
     const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
     int32_t ovf[STACK_OVF];
     int sp = 0;
-    int32_t cur = sc.root;
+    // cur: an inner node (>= 0), a leaf (< 0) or EMPTY_REF (the stack is
+    // empty); leaf: a leaf waiting for its triangle tests (< 0) or none
+#define POP() (sp == 0 ? (int32_t)EMPTY_REF : (sp--, sp < STACK_LDS ? stk[sp * BLOCK] : ovf[sp - STACK_LDS]))
+    int32_t cur = sc.root, leaf = EMPTY_REF;
+    if (cur < 0) {
+        leaf = cur;
+        cur = EMPTY_REF;
+    }
     while (true) {
-        if (cur >= 0) {
+        // inner nodes, until every lane of the wave has a leaf waiting
+        while ((uint32_t)cur < (uint32_t)EMPTY_REF) {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
             const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             const int4 ch = reinterpret_cast<const int4*>(np)[6];
@@ -58,18 +70,18 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             }
             // sorting network, ascending, on min / max only (near child first)
             uint32_t k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
-#define KSORT(a, b)                  \
-    {                                \
-        const uint32_t lo = min(a, b); \
-        b = max(a, b);               \
-        a = lo;                      \
-    }
+    #define KSORT(a, b)                  \
+        {                                \
+            const uint32_t lo = min(a, b); \
+            b = max(a, b);               \
+            a = lo;                      \
+        }
             KSORT(k0, k1);
             KSORT(k2, k3);
             KSORT(k0, k2);
             KSORT(k1, k3);
             KSORT(k1, k2);
-#undef KSORT
+    #undef KSORT
             const int32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
             const uint32_t MISS = 0x7F800000u;
             int32_t push[3] = {ref[k3 & 3u], ref[k2 & 3u], ref[k1 & 3u]};
@@ -88,12 +100,16 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
                     sp++;
                 }
             }
-            if (k0 < MISS) {
-                cur = ref[k0 & 3u];
-                continue;
+            cur = k0 < MISS ? ref[k0 & 3u] : POP();
+            if (cur < 0 && leaf >= 0) {  // the first leaf waits; traversal goes on
+                leaf = cur;
+                cur = POP();
             }
-        } else if (cur != EMPTY_REF) {
-            const uint32_t v = (uint32_t)(~cur);
+            if (__ballot(leaf >= 0) == 0) break;
+        }
+        // the waiting leaf, then a leaf popped behind it
+        while (leaf < 0) {
+            const uint32_t v = (uint32_t)(~leaf);
             const int32_t first = (int32_t)(v >> 3);
             const int32_t cnt = (int32_t)(v & 7u) + 1;
             for (int k = 0; k < cnt; k++) {
@@ -106,11 +122,12 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
                     h.v = w;
                 }
             }
+            leaf = cur;
+            if (cur < 0) cur = POP();
         }
-        if (sp == 0) break;
-        sp--;
-        cur = sp < STACK_LDS ? stk[sp * BLOCK] : ovf[sp - STACK_LDS];
+        if (cur == EMPTY_REF) break;
     }
+#undef POP
     return h;
 }
 
