@@ -145,6 +145,10 @@ typedef struct {
     /* gather work beyond the reference convention (gather_candidates) */
     int64_t gather_tested;     /* photon records the kernel distance-tested */
     int64_t gather_blocks;     /* 32-record block boxes tested (lanes kernel) */
+    /* traversal work (zero unless pppm_count_traversal is on) */
+    int64_t photon_nodes;      /* inner BVH nodes visited by the photon pass */
+    int64_t photon_tris;       /* triangles tested by the photon pass */
+    int64_t eye_nodes, eye_tris;
 } pppm_pass_stats;
 
 /* ---- context ---- */
@@ -200,6 +204,11 @@ int pppm_render_pass(void* ctx, pppm_pass_stats* stats);
  * pass.  With the learned proposal or BSSRDF materials the passes run one by
  * one. */
 int pppm_render_passes(void* ctx, int32_t count, pppm_pass_stats* stats);
+/* Measurement hook, no reference counterpart: with enable = 1 the following
+ * passes run kernel variants that also count BVH nodes visited and triangles
+ * tested per pass (stats photon_nodes ... eye_tris); the images are the same.
+ * For measuring traversal bytes; the counting costs a little time. */
+int pppm_count_traversal(void* ctx, int32_t enable);
 /* FP64 image [height][width][3], row 0 at the top, averaged over the passes
  * done: (sum - compensation) / passes. */
 int pppm_image_read(void* ctx, double* rgb, int64_t n_values);

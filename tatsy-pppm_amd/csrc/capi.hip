@@ -94,6 +94,7 @@ struct Ctx {
     uint32_t* last_perm1 = nullptr;
     uint32_t* last_sorted = nullptr;
     int32_t last_sub_bits = 0;  // the last pass ordered each bucket by a cell-local Morton code
+    int32_t count_traversal = 0;  // passes run the node / triangle counting kernels
     hipEvent_t ev[13] = {};
     hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
     hipStream_t stream3 = nullptr;  // the gather of pipelined passes (pppm_render_passes)
@@ -655,6 +656,7 @@ EyePassArgs eye_args(Ctx* ctx, uint32_t pass) {
     ea.sss_pos = ctx->sss_present ? ctx->sss_pos.as<float4>() : nullptr;
     ea.sss_w = ctx->sss_present ? ctx->sss_w.as<float4>() : nullptr;
     ea.sss_cap = ctx->sss_present ? npix : 0;
+    ea.count_traversal = ctx->count_traversal;
     return ea;
 }
 
@@ -1154,6 +1156,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     pa.irr_pos = ctx->sss_present ? ctx->irr_pos.as<float4>() : nullptr;
     pa.irr_flux = ctx->sss_present ? ctx->irr_flux.as<float4>() : nullptr;
     pa.irr_cap = ctx->sss_present ? ctx->cap : 0;
+    pa.count_traversal = ctx->count_traversal;
     launch_photon_pass(s, pa, photon_grid);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -1331,6 +1334,10 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         st->gather_inradius = (int64_t)hc.gather_inradius;
         st->gather_tested = (int64_t)hc.gather_tested;
         st->gather_blocks = (int64_t)hc.gather_blocks;
+        st->photon_nodes = (int64_t)hc.photon_nodes;
+        st->photon_tris = (int64_t)hc.photon_tris;
+        st->eye_nodes = (int64_t)hc.eye_nodes;
+        st->eye_tris = (int64_t)hc.eye_tris;
         st->photon_ms = t[0];
         st->order_ms = t[1];
         st->grid_ms = t[2];
@@ -1508,6 +1515,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         pa.irr_pos = nullptr;
         pa.irr_flux = nullptr;
         pa.irr_cap = 0;
+        pa.count_traversal = ctx->count_traversal;
         launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
@@ -1633,6 +1641,10 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             st.gather_inradius = (int64_t)c.gather_inradius;
             st.gather_tested = (int64_t)c.gather_tested;
             st.gather_blocks = (int64_t)c.gather_blocks;
+            st.photon_nodes = (int64_t)c.photon_nodes;
+            st.photon_tris = (int64_t)c.photon_tris;
+            st.eye_nodes = (int64_t)c.eye_nodes;
+            st.eye_tris = (int64_t)c.eye_tris;
             st.photon_ms = t[0];
             st.order_ms = t[1];
             st.grid_ms = 0.0;
@@ -1642,6 +1654,12 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             st.total_ms = t[5];
         }
     }
+    return PPPM_OK;
+}
+
+int pppm_count_traversal(void* ctx_raw, int32_t enable) {
+    CHECK_CTX(ctx);
+    ctx->count_traversal = enable ? 1 : 0;
     return PPPM_OK;
 }
 

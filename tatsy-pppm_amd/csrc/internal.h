@@ -101,6 +101,8 @@ struct Counters {
     unsigned long long photon_next;  // next photon index handed to a photon-pass lane
     unsigned long long gather_tested;  // photon records distance-tested by the gather
     unsigned long long gather_blocks;  // block boxes tested by the lanes gather
+    // traversal work, counted by the counting kernels only
+    unsigned long long photon_nodes, photon_tris, eye_nodes, eye_tris;
 };
 
 // Persistent photon-pass blocks per CU: at least the blocks one CU holds at
@@ -125,6 +127,7 @@ struct PhotonPassArgs {
     float4* irr_pos;  // xyz, oct normal bits
     float4* irr_flux; // rgb, origin key bits
     int64_t irr_cap;
+    int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 
@@ -151,6 +154,7 @@ struct EyePassArgs {
     float4* sss_pos;
     float4* sss_w;
     int64_t sss_cap;
+    int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
 };
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 // img += the direct emission stored in the pixel slots without hit point

@@ -116,6 +116,7 @@ __device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, 
 // do not idle behind the longest path of their wave.  A photon's random
 // numbers are keyed by its index, so its path and records do not depend on
 // the lane that traces it.
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
@@ -129,6 +130,7 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
     uint32_t i = 0;
     int bounce = 0;
     bool live = false;
+    TravCount tc;
     while (true) {
         const uint64_t need = __ballot(!live);
         if (need != 0) {
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
             live = false;
             continue;
         }
-        Hit h = trace_closest<256>(sc, o, dir, 0.0f, TRACE_TMAX, stk, &ovf);
+        Hit h = trace_closest<256, COUNT>(sc, o, dir, 0.0f, TRACE_TMAX, stk, &ovf, &tc);
         segs++;
         if (h.tri < 0) {
             live = false;
@@ -209,14 +211,24 @@ __global__ __launch_bounds__(256) void k_photon(PhotonPassArgs a) {
         atomicAdd(&a.counters->photon_segments, segs);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
     }
+    if (COUNT) {
+        const unsigned long long nn = wave_sum((unsigned long long)tc.nodes);
+        const unsigned long long nt = wave_sum((unsigned long long)tc.tris);
+        if (lane == 0) {
+            atomicAdd(&a.counters->photon_nodes, nn);
+            atomicAdd(&a.counters->photon_tris, nt);
+        }
+    }
 }
 
 
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
     unsigned long long segs = 0, nhit = 0;
     bool ovf = false;
+    TravCount tc;
     const SceneDev& sc = a.sc;
     const int W = a.cam.width, H = a.cam.height;
     const int64_t np = (int64_t)W * H;
@@ -239,7 +251,7 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
         for (int bounce = 0;; bounce++) {
             if (bounce >= a.max_bounces) break;
-            Hit h = trace_closest<256>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf);
+            Hit h = trace_closest<256, COUNT>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf, &tc);
             segs++;
             if (h.tri < 0) break;
             const int dd = EYE_DIM_BOUNCE + 3 * bounce;
@@ -306,6 +318,14 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         atomicAdd(&a.counters->eye_segments, segs);
         atomicAdd(&a.counters->hitpoints, nhit);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
+    }
+    if (COUNT) {
+        const unsigned long long nn = wave_sum((unsigned long long)tc.nodes);
+        const unsigned long long nt = wave_sum((unsigned long long)tc.tris);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.counters->eye_nodes, nn);
+            atomicAdd(&a.counters->eye_tris, nt);
+        }
     }
 }
 
@@ -388,7 +408,10 @@ void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit) 
     if (a.n_photons <= 0) return;
     unsigned g = grid1(a.n_photons, 256);
     if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
-    hipLaunchKernelGGL(k_photon, dim3(g), dim3(256), 0, s, a);
+    if (a.count_traversal)
+        hipLaunchKernelGGL(k_photon<true>, dim3(g), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_photon<false>, dim3(g), dim3(256), 0, s, a);
 }
 
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
@@ -396,7 +419,10 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
     if (np <= 0) return;
     unsigned g = grid1(np, 256);
     if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
-    hipLaunchKernelGGL(k_eye, dim3(g), dim3(256), 0, s, a);
+    if (a.count_traversal)
+        hipLaunchKernelGGL(k_eye<true>, dim3(g), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_eye<false>, dim3(g), dim3(256), 0, s, a);
 }
 
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp) {

@@ -20,9 +20,15 @@
 // 1 + 2^-22 so FP32 rounding cannot cull a box the triangle test would hit.
 #define SLAB_FAR_SCALE 1.00000024f
 
-template <int BLOCK>
+// Per-lane traversal work, counted by the COUNT instantiation only.
+struct TravCount {
+    uint32_t nodes = 0;  // inner nodes visited (one 128-byte fetch each)
+    uint32_t tris = 0;   // triangles tested (48 bytes each)
+};
+
+template <int BLOCK, bool COUNT = false>
 __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin, float tmax,
-                                    int32_t* __restrict__ stk, bool* overflow) {
+                                    int32_t* __restrict__ stk, bool* overflow, TravCount* tc = nullptr) {
     Hit h;
     h.t = tmax;
     h.tri = -1;
@@ -48,6 +54,7 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
     while (true) {
         // inner nodes, until every lane of the wave has a leaf waiting
         while ((uint32_t)cur < (uint32_t)EMPTY_REF) {
+            if (COUNT) tc->nodes++;
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
             const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             const int4 ch = reinterpret_cast<const int4*>(np)[6];
@@ -112,6 +119,7 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             const uint32_t v = (uint32_t)(~leaf);
             const int32_t first = (int32_t)(v >> 3);
             const int32_t cnt = (int32_t)(v & 7u) + 1;
+            if (COUNT) tc->tris += (uint32_t)cnt;
             for (int k = 0; k < cnt; k++) {
                 const float4* tp = sc.tris + 3 * (int64_t)(first + k);
                 float t, u, w;

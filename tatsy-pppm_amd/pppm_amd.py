@@ -60,7 +60,9 @@ class PassStats(C.Structure):
                 ("proposal_records", C.c_int64), ("proposal_active", C.c_int32), ("retrained", C.c_int32),
                 ("irradiance_ms", C.c_double), ("dipole_ms", C.c_double), ("irradiance_photons", C.c_int64),
                 ("sss_hits", C.c_int64), ("sss_points", C.c_int64),
-                ("gather_tested", C.c_int64), ("gather_blocks", C.c_int64)]
+                ("gather_tested", C.c_int64), ("gather_blocks", C.c_int64),
+                ("photon_nodes", C.c_int64), ("photon_tris", C.c_int64), ("eye_nodes", C.c_int64),
+                ("eye_tris", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -101,6 +103,7 @@ _DECLS = {
     "pppm_render_pass": (C.c_int, [_P, C.POINTER(PassStats)]),
     "pppm_render_passes": (C.c_int, [_P, C.c_int32, _P]),
     "pppm_image_read": (C.c_int, [_P, _P, C.c_int64]),
+    "pppm_count_traversal": (C.c_int, [_P, C.c_int32]),
     "pppm_last_photons": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pppm_last_grid": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, C.POINTER(C.c_double)]),
     "pppm_last_hitpoints": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
@@ -321,6 +324,9 @@ class Context:
         st = PassStats()
         self._chk(self.L.pppm_render_pass(self.h, C.byref(st)))
         return st.as_dict()
+
+    def count_traversal(self, enable=True):
+        self._chk(self.L.pppm_count_traversal(self.h, 1 if enable else 0))
 
     def render_passes(self, count):
         sts = (PassStats * max(count, 1))()

@@ -366,3 +366,27 @@ def test_pipelined_passes_equal_single_passes(gpu_ctx, mode):
                   "gather_inradius", "hash_size"):
             assert x[k] == y[k], k
         assert y["gather_ms"] > 0 and y["photon_ms"] > 0
+
+
+def test_traversal_counting_kernels_change_nothing_but_the_counts(gpu_ctx, c1_small):
+    # pppm_count_traversal swaps in kernel variants that count BVH nodes and
+    # triangles; images and photon records are those of the plain kernels
+    s, _ = c1_small
+    say("traversal counting kernels")
+    st0 = _render_one_pass(gpu_ctx, s, 40000, 0.0866, passes=2)
+    a, pa = gpu_ctx.image(), gpu_ctx.last_photons()
+    gpu_ctx.count_traversal(True)
+    try:
+        st1 = _render_one_pass(gpu_ctx, s, 40000, 0.0866, passes=2)
+    finally:
+        gpu_ctx.count_traversal(False)
+    b, pb = gpu_ctx.image(), gpu_ctx.last_photons()
+    assert np.array_equal(a, b)
+    for x, y in zip(pa, pb):
+        assert np.array_equal(x, y)
+    for s0, s1 in zip(st0, st1):
+        assert s0["photon_nodes"] == 0 and s0["eye_nodes"] == 0
+        # every ray visits the root (an inner node for this scene)
+        assert s1["photon_nodes"] >= s1["photon_segments"] > 0
+        assert s1["eye_nodes"] >= s1["eye_segments"] > 0
+        assert 0 < s1["photon_tris"] and 0 < s1["eye_tris"]
