@@ -2,7 +2,8 @@
 //
 // Replaces QBVHAccel::construct (sources/qbvh_accel.cc:139-203), whose serial
 // object-median build sorts at every level.  Here:
-//   1. 30-bit Morton codes of triangle centroids, stable radix sort;
+//   1. 30-bit Morton codes of triangle centroids (big triangles after all
+//      others), stable radix sort;
 //   2. binary radix tree over the sorted codes (Karras 2012), every subtree a
 //      contiguous range of the sorted triangles;
 //   3. bottom-up boxes (one thread per leaf, the second arrival at a node
@@ -56,8 +57,17 @@ __global__ void k_centroid_bounds(const float* __restrict__ tri, int64_t n,
     if (tid < 6) partial[blockIdx.x * 6 + tid] = s[tid][0];
 }
 
+// A triangle whose box has more than BIG_AREA times the half area of the
+// centroid bounds is "big" (the walls of a room, an area light): big
+// triangles sort after all others (key bit 30), so the root of the radix tree
+// splits them off, and they become one-triangle leaves.  Grouped with small
+// triangles by centroid, a wall would widen every box above it and be tested
+// by most rays.
+#define BIG_AREA 2.0e-3f
+
 __global__ void k_morton(const float* __restrict__ tri, int64_t n, const float* __restrict__ partial,
-                         int nparts, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                         int nparts, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                         uint32_t* __restrict__ n_big) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
@@ -68,14 +78,20 @@ __global__ void k_morton(const float* __restrict__ tri, int64_t n, const float* 
         }
     const float* t = tri + 9 * i;
     uint32_t q[3];
+    float e[3];
     for (int d = 0; d < 3; d++) {
         float c = (t[d] + t[3 + d] + t[6 + d]) * (1.0f / 3.0f);
         float ext = hi[d] - lo[d];
         float u = ext > 0.0f ? (c - lo[d]) / ext : 0.5f;
         q[d] = (uint32_t)fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+        e[d] = fmaxf(fmaxf(t[d], t[3 + d]), t[6 + d]) - fminf(fminf(t[d], t[3 + d]), t[6 + d]);
     }
-    keys[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    const float scene_area = ex * ey + ey * ez + ez * ex;
+    const bool big = e[0] * e[1] + e[1] * e[2] + e[2] * e[0] > BIG_AREA * scene_area;
+    keys[i] = (big ? (1u << 30) : 0u) | (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
     vals[i] = (uint32_t)i;
+    if (big) atomicAdd(n_big, 1u);
 }
 
 // common-prefix length of sorted codes i and j; ties broken by index
@@ -192,7 +208,7 @@ __device__ inline float half_area(const Box& b) {
 
 __global__ void k_collapse(const QItem* __restrict__ qin, const uint32_t* __restrict__ qin_count,
                            QItem* __restrict__ qout, uint32_t* __restrict__ qout_count,
-                           uint32_t* __restrict__ node_count, int64_t n,
+                           uint32_t* __restrict__ node_count, int64_t n, int64_t big_first,
                            const int32_t* __restrict__ child, const int32_t* __restrict__ range,
                            const float* __restrict__ boxes, Node4* __restrict__ nodes) {
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,12 +221,18 @@ __global__ void k_collapse(const QItem* __restrict__ qin, const uint32_t* __rest
     auto size_of = [&](int32_t b) -> int32_t {
         return b >= n - 1 ? 1 : range[2 * b + 1] - range[2 * b] + 1;
     };
+    // leaf size limit: one triangle below the split of the big triangles
+    // (every binary node but the root lies entirely on one side of it)
+    auto leaf_max = [&](int32_t b) -> int32_t {
+        const int64_t first = b >= n - 1 ? (int64_t)(b - (n - 1)) : (int64_t)range[2 * b];
+        return first >= big_first ? 1 : LEAF_MAX;
+    };
     while (cnt < 4) {
         int best = -1;
         float best_a = -1.0f;
         for (int c = 0; c < cnt; c++) {
             int32_t b = cand[c];
-            if (b < n - 1 && size_of(b) > LEAF_MAX) {
+            if (b < n - 1 && size_of(b) > leaf_max(b)) {
                 float a = half_area(load_box(boxes, b));
                 if (a > best_a) { best_a = a; best = c; }
             }
@@ -229,7 +251,7 @@ __global__ void k_collapse(const QItem* __restrict__ qin, const uint32_t* __rest
             nd.loy[c] = bx.lo[1]; nd.hiy[c] = bx.hi[1];
             nd.loz[c] = bx.lo[2]; nd.hiz[c] = bx.hi[2];
             int32_t sz = size_of(b);
-            if (b >= n - 1 || sz <= LEAF_MAX) {
+            if (b >= n - 1 || sz <= leaf_max(b)) {
                 int32_t first = b >= n - 1 ? (int32_t)(b - (n - 1)) : range[2 * b];
                 nd.child[c] = leaf_ref(first, sz);
             } else {
@@ -328,9 +350,14 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, BvhBuild* out
         HIPCHK(hipMalloc(&out->nodes, sizeof(Node4) * ni));
 
         hipLaunchKernelGGL(k_centroid_bounds, dim3(nparts), dim3(256), 0, s, tri9, n, partial);
+        launch_zero_u32(s, counts + 3, 1);
         hipLaunchKernelGGL(k_morton, dim3(grid1(n, 256)), dim3(256), 0, s, tri9, n, partial, nparts,
-                           keys, vals);
-        bool alt = radix_sort_pairs(s, keys, vals, keys2, vals2, hist, n, 30);
+                           keys, vals, counts + 3);
+        bool alt = radix_sort_pairs(s, keys, vals, keys2, vals2, hist, n, 31);
+        uint32_t n_big = 0;
+        HIPCHK(hipMemcpyAsync(&n_big, counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t big_first = n - (int64_t)n_big;
         uint32_t* codes = alt ? keys2 : keys;
         uint32_t* order = alt ? vals2 : vals;
         if (n > 1)
@@ -355,7 +382,7 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, BvhBuild* out
         while (nin > 0) {
             launch_zero_u32(s, co, 1);
             hipLaunchKernelGGL(k_collapse, dim3(grid1(nin, 256)), dim3(256), 0, s, qi, ci, qo, co,
-                               counts + 2, n, child, range, boxes, out->nodes);
+                               counts + 2, n, big_first, child, range, boxes, out->nodes);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(&nin, co, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
