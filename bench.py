@@ -46,9 +46,9 @@ def traversal_bytes(rays, nodes_per_ray, tris_per_ray):
     return rays * (32 + 16 + 128 * nodes_per_ray + 36 * tris_per_ray)
 
 
-def read_traffic():
-    """HBM bytes per gather launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "gather_pmc_caustic_box_1024.txt")
+def read_traffic(name="gather_pmc_caustic_box_1024.txt"):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -477,6 +477,7 @@ def main():
             "eye_nodes_per_ray": eye_nodes_per_ray,
             "eye_tris_per_ray": eye_tris_per_ray,
             "algorithmic_bytes_per_launch": trav_bytes,
+            "traffic": read_traffic("photon_pmc_caustic_box_1024.txt"),
             "avg_launch_ms": photon_kernel_ms,
             "achieved": trav_gbps,
             "unit": "GB/s",
