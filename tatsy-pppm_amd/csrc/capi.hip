@@ -147,6 +147,7 @@ struct Ctx {
         s.lights = lights.as<LightDev>();
         s.n_lights = (int32_t)h_lights.size();
         s.root = root;
+        s.n_nodes = (int32_t)binfo.nodes;
         s.n_tris = n_tris;
         return s;
     }
@@ -867,6 +868,9 @@ int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
     // the traversal stack holds at most 3 entries per level below the root
     if (3 * (int64_t)b.depth + 1 > STACK_MAX)
         return ctx->fail(PPPM_ERR_INVALID, "BVH deeper than the traversal stack allows");
+    // node fetches address the node array with 32-bit byte offsets
+    if ((int64_t)b.n_nodes * (int64_t)sizeof(Node4) > (int64_t)INT32_MAX)
+        return ctx->fail(PPPM_ERR_INVALID, "BVH larger than 2 GiB of nodes");
     ctx->binfo.nodes = b.n_nodes;
     ctx->binfo.leaves = b.n_leaves;
     ctx->binfo.depth = b.depth;
@@ -1067,7 +1071,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     const int64_t cap = ctx->cap;
     TRY_HIP(ctx, ctx->ph_pos.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_flux.ensure(sizeof(float4) * (size_t)cap));
-    TRY_HIP(ctx, ctx->ph_a.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_a.ensure(sizeof(float4) * (size_t)(cap + GATHER_PAD)));
     TRY_HIP(ctx, ctx->ph_b.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_c.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->boxes.ensure(sizeof(float4) * 2 * (size_t)gather_box_count(cap)));
@@ -1111,7 +1115,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     const int32_t pass = ctx->passes_done + 1;
     const double radius = ctx->radii[(size_t)pass - 1];
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
-    const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    const int eye_grid = ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048;
     const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
     Counters* dc = ctx->counters.as<Counters>();
     ctx->last_valid = false;
@@ -1126,7 +1130,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     TRY_HIP(ctx, hipEventRecord(ctx->ev[11], s2));
     {
         EyePassArgs ea = eye_args(ctx, (uint32_t)pass);
-        launch_eye_pass(s2, ea, grid_limit);
+        launch_eye_pass(s2, ea, eye_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(ctx->ev[4], s2));
         GridParams gh;
@@ -1433,7 +1437,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     DBuf* hhq[2] = {&ctx->hhist, &ctx->hhist_alt};
     enum { ZERO, P0, P1, O0, ODONE, E0, E1, HDONE, G0, GEND, GDONE };
     const int NEV = 6;  // event ring: pass k records into pev[k % NEV]
-    const int grid_limit = ctx->cu > 0 ? ctx->cu * 64 : 16384;
+    const int eye_grid = ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048;
     const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
     const int32_t first = ctx->passes_done + 1;  // pass index of k = 0
     std::vector<Counters> hc_pass((size_t)count);
@@ -1472,7 +1476,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ea.counters = S.counters->as<Counters>();
         ea.img_sum = nullptr;  // the direct emission stays in the slots for C
         ea.img_comp = nullptr;
-        launch_eye_pass(B, ea, grid_limit);
+        launch_eye_pass(B, ea, eye_grid);
         if ((e = hipEventRecord(E[E1], B)) != hipSuccess) return e;
         GridParams gh;
         for (int d = 0; d < 3; d++) gh.bmin[d] = ctx->grid_bmin[d];
@@ -1783,6 +1787,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     hipStream_t s = ctx->stream;
     const int64_t np = n_ph > 0 ? n_ph : 1, nh = n_hit > 0 ? n_hit : 1;
     // staging: photons [3 x 3 floats] + hit points [3 x 3 floats] + outputs
+    // (the lanes kernel's reads past the sorted positions p4s land in f4s)
     size_t bytes = sizeof(float) * 9 * (size_t)(np + nh) + sizeof(float4) * 5 * (size_t)np +
                    sizeof(float4) * 2 * (size_t)nh + sizeof(float) * 3 * (size_t)nh;
     TRY_HIP(ctx, ctx->scratch.ensure(bytes));
@@ -2338,7 +2343,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     if (g.sub_bits > 0) launch_gather_boxes(s, ctx->ph_a.as<float4>(), cap, ctx->boxes.as<float4>());
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
-    launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * 64 : 16384);
+    launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048);
     sort_hitpoints(ctx, ctx->stream, ctx->hp_pos.as<float4>(), ctx->hp_w.as<float4>(), npix, g, 0.0);
     GatherArgs ga;
     std::memset(&ga, 0, sizeof(ga));

@@ -60,6 +60,19 @@ __device__ __forceinline__ v3f oct_direction(uint32_t e) {
     return mk(qx, qy, qz);
 }
 
+// oct_direction(e) . n > 0 in fewer instructions, with the same values: the
+// fold's sign factor is a copysign (a component is never -0: int16 0
+// converts to +0), and the dot product is contracted in one fixed order.
+__device__ __forceinline__ bool normal_agrees(uint32_t e, v3f n) {
+    float qx = (float)(int16_t)(e & 0xFFFF) * (1.0f / 32767.0f);
+    float qy = (float)(int16_t)(e >> 16) * (1.0f / 32767.0f);
+    const float qz = 1.0f - fabsf(qx) - fabsf(qy);
+    const float fx = copysignf(1.0f - fabsf(qy), qx), fy = copysignf(1.0f - fabsf(qx), qy);
+    qx = qz < 0.0f ? fx : qx;
+    qy = qz < 0.0f ? fy : qy;
+    return fmaf(qz, n.z, fmaf(qx, n.x, qy * n.y)) > 0.0f;
+}
+
 // Smallest bucket id among the slots of mb set in m (all ones if none), and
 // the slot set with that bucket cleared: value operations only, so the eight
 // slots stay in registers.
@@ -170,28 +183,29 @@ __device__ __forceinline__ void lanes_range(const GatherArgs& a, uint32_t j0, ui
     const float4* __restrict__ pa = a.ph_a;
     const float4* __restrict__ pb = a.ph_b;
     for (uint32_t j = j0; j < j1; j += 4) {
-        const uint32_t last = j1 - 1;
+        // up to three records past the range are read and masked out: the
+        // layout array holds GATHER_PAD records beyond its last one
+        const float4* __restrict__ pj = pa + j;
         float4 A[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) A[q] = pa[min(j + q, last)];
+        for (int q = 0; q < 4; q++) A[q] = pj[q];
         // the normal word is read with the position (one 16-byte load)
 #pragma unroll
         for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
         float d2[4];
         bool in[4];
+        // the normal test runs on every slot, without a branch: some lane of
+        // the wave nearly always has a slot inside the radius
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             d2[q] = dist2(x, A[q]);
-            in[q] = j + q < j1 && d2[q] <= r2;
-            if (in[q]) {
-                const v3f np = oct_direction(__float_as_uint(A[q].w));
-                in[q] = np.x * nx.x + np.y * nx.y + np.z * nx.z > 0.0f;
-            }
+            in[q] = j + q < j1 && d2[q] <= r2 && normal_agrees(__float_as_uint(A[q].w), nx);
         }
         float4 F[4];
+        const float4* __restrict__ pbj = pb + j;
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            if (in[q]) F[q] = pb[j + q];
+            if (in[q]) F[q] = pbj[q];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (in[q]) {

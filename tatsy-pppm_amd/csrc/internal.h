@@ -99,6 +99,7 @@ struct Counters {
     unsigned long long irr_stored;   // photons stored on BSSRDF surfaces
     unsigned long long sss_hits;     // eye paths ending in the subsurface estimate
     unsigned long long photon_next;  // next photon index handed to a photon-pass lane
+    unsigned long long eye_next;     // next pixel handed to an eye-pass lane
     unsigned long long gather_tested;  // photon records distance-tested by the gather
     unsigned long long gather_blocks;  // block boxes tested by the lanes gather
     // traversal work, counted by the counting kernels only
@@ -108,6 +109,8 @@ struct Counters {
 // Persistent photon-pass blocks per CU: at least the blocks one CU holds at
 // once (the kernel regenerates paths, so extra blocks find no work).
 #define PHOTON_BLOCKS_PER_CU 8
+// The same for the eye pass, which regenerates pixel paths.
+#define EYE_BLOCKS_PER_CU 8
 
 struct PhotonPassArgs {
     SceneDev sc;
@@ -156,7 +159,8 @@ struct EyePassArgs {
     int64_t sss_cap;
     int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
 };
-void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
+// grid: the persistent grid (blocks; at most one block per 256 pixels is used)
+void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid);
 // img += the direct emission stored in the pixel slots without hit point
 // (an eye pass run with img_sum = null)
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp);
@@ -211,6 +215,9 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 // [GATHER_BLOCK k, GATHER_BLOCK (k + 1)) of a (n records)
 void launch_gather_boxes(hipStream_t s, const float4* a, int64_t n, float4* boxes);
 inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GATHER_BLOCK; }
+// Records a lanes-layout position array holds past its capacity: the lanes
+// kernel reads positions four at a time and masks out those past a bucket.
+#define GATHER_PAD 4
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
                           const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C);
 

@@ -169,6 +169,7 @@ struct SceneDev {
     const LightDev* lights;
     int32_t n_lights;
     int32_t root;            // root ref (inner index 0, a leaf ref, or EMPTY_REF)
+    int32_t n_nodes;         // inner nodes (their bytes fit a 32-bit buffer offset)
     int64_t n_tris;
 };
 

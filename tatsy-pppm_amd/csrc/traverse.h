@@ -3,7 +3,10 @@
 // One ray per lane.  The traversal stack keeps its top STACK_LDS entries in
 // LDS (laid out [slot][lane] so the 64 lanes of a wave hit 64 different banks)
 // and spills deeper entries to a per-lane private array.  Nodes are fetched as
-// seven 16-byte loads from one 128-byte line; children are visited near-first.
+// seven 16-byte buffer loads from one 128-byte line: per axis the near plane
+// (lo for a non-negative direction component, else hi) and the far plane, so
+// the slab test needs no min / max to order them; children are visited
+// near-first.
 // The loop is the while-while form with postponed leaves of Aila and Laine
 // (2009): a lane that reaches its first leaf keeps it waiting and goes on
 // through inner nodes until every lane of the wave has a leaf, so that node
@@ -26,52 +29,118 @@ struct TravCount {
     uint32_t tris = 0;   // triangles tested (48 bytes each)
 };
 
-template <int BLOCK, bool COUNT = false>
-__device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin, float tmax,
-                                    int32_t* __restrict__ stk, bool* overflow, TravCount* tc = nullptr) {
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+// Word 3 of a raw buffer resource on CDNA: 32-bit data format, no swizzle.
+#define BUFFER_RSRC_WORD3 0x00020000
+
+// Pop the traversal stack; EMPTY_REF when it is empty.  The LDS slot is read
+// at a clamped index so that only the rare private-array case branches.
+template <int BLOCK>
+__device__ inline int32_t stack_pop(int& sp, const int32_t* __restrict__ stk, const int32_t* __restrict__ ovf) {
+    const int s = sp - 1;
+    int32_t v = stk[min(max(s, 0), STACK_LDS - 1) * BLOCK];
+    // keeps the two reads apart: merged, they become one flat load of a
+    // selected LDS-or-private address, slower than an LDS read
+    asm volatile("" : "+v"(v));
+    if (s >= STACK_LDS) v = ovf[s - STACK_LDS];
+    sp = max(s, 0);
+    return s < 0 ? (int32_t)EMPTY_REF : v;
+}
+
+// A ray set up for traversal: the watertight test's shear and the slab
+// test's scaled inverse direction.
+struct TraceRay {
+    RayPre pre;  // pre.o is the origin
+    float ix, iy, iz, ox, oy, oz;
+    float tmin;
+    uint32_t near_off;  // byte offsets of the near x / y / z planes in a node, 8 bits each
+};
+
+// The resumable part of a traversal.  Between rounds cur is the next inner
+// node or EMPTY_REF once the ray is finished (then h is its closest hit); a
+// leaf root is kept in cur until the first round tests it.
+struct TraceState {
+    int32_t cur;
+    int32_t sp;
     Hit h;
-    h.t = tmax;
-    h.tri = -1;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    if (sc.root == EMPTY_REF) return h;
-    const RayPre pre = ray_pre(o, d);
+};
+
+__device__ inline void trace_start(const SceneDev& sc, v3f o, v3f d, float tmin, float tmax, TraceRay& r,
+                                   TraceState& s) {
+    r.pre = ray_pre(o, d);
     // inverse direction as the reference computes it (qbvh_accel.cc:219-221)
-    const float ix = d.x == 0.0f ? 1.0e20f : 1.0f / d.x;
-    const float iy = d.y == 0.0f ? 1.0e20f : 1.0f / d.y;
-    const float iz = d.z == 0.0f ? 1.0e20f : 1.0f / d.z;
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-    int32_t ovf[STACK_OVF];
-    int sp = 0;
+    r.ix = d.x == 0.0f ? 1.0e20f : 1.0f / d.x;
+    r.iy = d.y == 0.0f ? 1.0e20f : 1.0f / d.y;
+    r.iz = d.z == 0.0f ? 1.0e20f : 1.0f / d.z;
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    r.tmin = tmin;
+    // Node4: lo x at byte 0, hi x at 16, lo y 32, hi y 48, lo z 64, hi z 80
+    r.near_off = (r.ix >= 0.0f ? 0u : 16u) | ((r.iy >= 0.0f ? 32u : 48u) << 8) | ((r.iz >= 0.0f ? 64u : 80u) << 16);
+    s.cur = sc.root;
+    s.sp = 0;
+    s.h.t = tmax;
+    s.h.tri = -1;
+    s.h.u = 0.0f;
+    s.h.v = 0.0f;
+}
+
+// Runs rounds of the while-while loop (inner nodes until every busy lane has
+// a leaf waiting, then the waiting leaves) over the lanes whose ray is not
+// finished, and returns once at most `keep` lanes of the wave are unfinished
+// (keep = 0: every ray finished).  Lanes with s.cur == EMPTY_REF on entry take
+// no part.  The stack lives in stk (LDS, stride BLOCK) and ovf (private).
+template <int BLOCK, bool COUNT = false>
+__device__ inline void trace_resume(const SceneDev& sc, const TraceRay& r, TraceState& s,
+                                    int32_t* __restrict__ stk, int32_t* __restrict__ ovf, bool* overflow,
+                                    TravCount* tc, int keep) {
+    const float ix = r.ix, iy = r.iy, iz = r.iz, ox = r.ox, oy = r.oy, oz = r.oz, tmin = r.tmin;
+    int32_t cur = s.cur;
+    int sp = s.sp;
+    Hit h = s.h;
     // cur: an inner node (>= 0), a leaf (< 0) or EMPTY_REF (the stack is
     // empty); leaf: a leaf waiting for its triangle tests (< 0) or none
-#define POP() (sp == 0 ? (int32_t)EMPTY_REF : (sp--, sp < STACK_LDS ? stk[sp * BLOCK] : ovf[sp - STACK_LDS]))
-    int32_t cur = sc.root, leaf = EMPTY_REF;
-    if (cur < 0) {
-        leaf = cur;
-        cur = EMPTY_REF;
-    }
+#define POP() stack_pop<BLOCK>(sp, stk, ovf)
+    // the node array as a buffer: 32-bit per-lane byte offsets, and a fetch
+    // past its end reads zeros instead of faulting
+    const __amdgpu_buffer_rsrc_t nbuf = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)sc.nodes, (short)0, sc.n_nodes * (int32_t)sizeof(Node4), BUFFER_RSRC_WORD3);
+    const uint32_t nxo = r.near_off & 0xFFu, nyo = (r.near_off >> 8) & 0xFFu, nzo = r.near_off >> 16;
     while (true) {
+        int32_t leaf = EMPTY_REF;
+        if (cur < 0) {  // a leaf root
+            leaf = cur;
+            cur = POP();
+        }
         // inner nodes, until every lane of the wave has a leaf waiting
         while ((uint32_t)cur < (uint32_t)EMPTY_REF) {
             if (COUNT) tc->nodes++;
-            const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 ch = reinterpret_cast<const int4*>(np)[6];
+            const uint32_t base = (uint32_t)cur << 7;
+            const uint32_t ax = base | nxo, ay = base | nyo, az = base | nzo;
+            const v4u nx4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)ax, 0, 0);
+            const v4u fx4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)(ax ^ 16u), 0, 0);
+            const v4u ny4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)ay, 0, 0);
+            const v4u fy4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)(ay ^ 16u), 0, 0);
+            const v4u nz4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)az, 0, 0);
+            const v4u fz4 = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)(az ^ 16u), 0, 0);
+            const v4u ch = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)base, 96, 0);
             // one 32-bit key per child: the entry distance (non-negative, so
             // its bit pattern orders like the value) with the child slot in the
-            // two low mantissa bits; a missed child keys above every distance
+            // two low mantissa bits; a missed child keys above every distance.
+            // With the planes ordered by the direction's signs these are the
+            // min / max of the two slab distances per axis, bit for bit
+            // (rounding is monotonic), and an empty slot (every plane +inf)
+            // still misses.
             uint32_t key[4];
-            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                float x0 = lxa[c] * ix - ox, x1 = hxa[c] * ix - ox;
-                float y0 = lya[c] * iy - oy, y1 = hya[c] * iy - oy;
-                float z0 = lza[c] * iz - oz, z1 = hza[c] * iz - oz;
-                float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-                float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h.t));
+                const float x0 = __uint_as_float(nx4[c]) * ix - ox, x1 = __uint_as_float(fx4[c]) * ix - ox;
+                const float y0 = __uint_as_float(ny4[c]) * iy - oy, y1 = __uint_as_float(fy4[c]) * iy - oy;
+                const float z0 = __uint_as_float(nz4[c]) * iz - oz, z1 = __uint_as_float(fz4[c]) * iz - oz;
+                const float n = fmaxf(fmaxf(x0, y0), fmaxf(z0, tmin));
+                const float f = fminf(fminf(x1, y1), fminf(z1, h.t));
                 key[c] = (n <= f * SLAB_FAR_SCALE) ? ((__float_as_uint(n) & 0x7FFFFFFCu) | (uint32_t)c)
                                                    : (0xFFFFFFFCu | (uint32_t)c);
             }
@@ -89,22 +158,32 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             KSORT(k1, k3);
             KSORT(k1, k2);
     #undef KSORT
-            const int32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
+            const int32_t ref[4] = {(int32_t)ch[0], (int32_t)ch[1], (int32_t)ch[2], (int32_t)ch[3]};
             const uint32_t MISS = 0x7F800000u;
             int32_t push[3] = {ref[k3 & 3u], ref[k2 & 3u], ref[k1 & 3u]};
             bool pv[3] = {k3 < MISS, k2 < MISS, k1 < MISS};
+            if (sp <= STACK_LDS - 3) {
+                // room for three in LDS: every candidate is written to the
+                // slot above the top, which only a valid one claims
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                if (pv[q]) {
-                    if (sp < STACK_LDS) {
-                        stk[sp * BLOCK] = push[q];
-                    } else if (sp < STACK_MAX) {
-                        ovf[sp - STACK_LDS] = push[q];
-                    } else {
-                        *overflow = true;
-                        sp--;
+                for (int q = 0; q < 3; q++) {
+                    stk[sp * BLOCK] = push[q];
+                    sp += pv[q] ? 1 : 0;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    if (pv[q]) {
+                        if (sp < STACK_LDS) {
+                            stk[sp * BLOCK] = push[q];
+                        } else if (sp < STACK_MAX) {
+                            ovf[sp - STACK_LDS] = push[q];
+                        } else {
+                            *overflow = true;
+                            sp--;
+                        }
+                        sp++;
                     }
-                    sp++;
                 }
             }
             cur = k0 < MISS ? ref[k0 & 3u] : POP();
@@ -123,7 +202,7 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             for (int k = 0; k < cnt; k++) {
                 const float4* tp = sc.tris + 3 * (int64_t)(first + k);
                 float t, u, w;
-                if (tri_watertight(pre, tp[0], tp[1], tp[2], tmin, h.t, &t, &u, &w)) {
+                if (tri_watertight(r.pre, tp[0], tp[1], tp[2], tmin, h.t, &t, &u, &w)) {
                     h.t = t;
                     h.tri = first + k;
                     h.u = u;
@@ -133,10 +212,24 @@ __device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin
             leaf = cur;
             if (cur < 0) cur = POP();
         }
-        if (cur == EMPTY_REF) break;
+        if (__popcll(__ballot(cur != (int32_t)EMPTY_REF)) <= keep) break;
     }
 #undef POP
-    return h;
+    s.cur = cur;
+    s.sp = sp;
+    s.h = h;
+}
+
+// Closest hit of one ray per lane, every lane's traversal run to its end.
+template <int BLOCK, bool COUNT = false>
+__device__ inline Hit trace_closest(const SceneDev& sc, v3f o, v3f d, float tmin, float tmax,
+                                    int32_t* __restrict__ stk, bool* overflow, TravCount* tc = nullptr) {
+    TraceRay r;
+    TraceState s;
+    trace_start(sc, o, d, tmin, tmax, r, s);
+    int32_t ovf[STACK_OVF];
+    trace_resume<BLOCK, COUNT>(sc, r, s, stk, ovf, overflow, tc, 0);
+    return s.h;
 }
 
 // Geometric normal of a leaf-order triangle (triangle.cc:45-47).
