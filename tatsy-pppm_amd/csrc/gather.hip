@@ -176,10 +176,11 @@ __device__ __forceinline__ void flush_counters(const GatherArgs& a, bool valid, 
 }
 
 // The records [j0, j1) of one bucket: four 16-byte records in flight per wait
-// (the range's last record repeated past its end and masked out), flux loads
-// issued before the sums, accumulation in record order.
+// (up to three records past the range read and masked out), flux loads issued
+// before the sums, accumulation in record order.
 __device__ __forceinline__ void lanes_range(const GatherArgs& a, uint32_t j0, uint32_t j1, v3f x, v3f nx, float r2,
-                                            float inv_kr, float& ax, float& ay, float& az, uint32_t& ni) {
+                                            float inv_kr, bool test_normals, float& ax, float& ay, float& az,
+                                            uint32_t& ni) {
     const float4* __restrict__ pa = a.ph_a;
     const float4* __restrict__ pb = a.ph_b;
     for (uint32_t j = j0; j < j1; j += 4) {
@@ -194,12 +195,11 @@ __device__ __forceinline__ void lanes_range(const GatherArgs& a, uint32_t j0, ui
         for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
         float d2[4];
         bool in[4];
-        // the normal test runs on every slot, without a branch: some lane of
-        // the wave nearly always has a slot inside the radius
+        // the normal test is skipped where the block's normals agree as one
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             d2[q] = dist2(x, A[q]);
-            in[q] = j + q < j1 && d2[q] <= r2 && normal_agrees(__float_as_uint(A[q].w), nx);
+            in[q] = j + q < j1 && d2[q] <= r2 && (!test_normals || normal_agrees(__float_as_uint(A[q].w), nx));
         }
         float4 F[4];
         const float4* __restrict__ pbj = pb + j;
@@ -263,14 +263,23 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
             const bool boxed = a.boxes != nullptr;
             const uint32_t k0 = boxed ? b0 / GATHER_BLOCK : 0, k1 = boxed ? (b1 - 1) / GATHER_BLOCK : 0;
             for (uint32_t k = k0; k <= k1; k++) {
+                bool test_normals = true;
                 if (boxed) {
                     nk++;
-                    if (box_dist2(x, a.boxes[2 * k], a.boxes[2 * k + 1]) > r2_box) continue;
+                    const float4 lo = a.boxes[2 * k];
+                    if (box_dist2(x, lo, a.boxes[2 * k + 1]) > r2_box) continue;
+                    // one normal for the whole block: one test decides for
+                    // every record of it
+                    const uint32_t eb = __float_as_uint(lo.w);
+                    if (eb != MIXED_NORMALS) {
+                        if (!normal_agrees(eb, nx)) continue;
+                        test_normals = false;
+                    }
                 }
                 const uint32_t j0 = boxed ? max(b0, k * GATHER_BLOCK) : b0;
                 const uint32_t j1 = boxed ? min(b1, (k + 1) * GATHER_BLOCK) : b1;
                 nt += j1 - j0;
-                lanes_range(a, j0, j1, x, nx, r2, inv_kr, ax, ay, az, ni);
+                lanes_range(a, j0, j1, x, nx, r2, inv_kr, test_normals, ax, ay, az, ni);
             }
         }
         write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
@@ -436,6 +445,7 @@ __global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__
     if (k >= nbox) return;  // whole blocks of 32 lanes leave together
     const float4 p = a[min(j, n - 1)];  // the last record repeated past the end
     float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+    uint32_t emin = __float_as_uint(p.w), emax = emin;
 #pragma unroll
     for (int off = GATHER_BLOCK / 2; off > 0; off >>= 1) {
         lx = fminf(lx, __shfl_xor(lx, off));
@@ -444,9 +454,12 @@ __global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__
         hx = fmaxf(hx, __shfl_xor(hx, off));
         hy = fmaxf(hy, __shfl_xor(hy, off));
         hz = fmaxf(hz, __shfl_xor(hz, off));
+        emin = min(emin, (uint32_t)__shfl_xor(emin, off));
+        emax = max(emax, (uint32_t)__shfl_xor(emax, off));
     }
     if ((j % GATHER_BLOCK) == 0) {
-        boxes[2 * k] = make_float4(lx, ly, lz, 0.0f);
+        const uint32_t e = emin == emax ? emin : MIXED_NORMALS;
+        boxes[2 * k] = make_float4(lx, ly, lz, __uint_as_float(e));
         boxes[2 * k + 1] = make_float4(hx, hy, hz, 0.0f);
     }
 }

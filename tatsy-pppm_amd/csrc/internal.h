@@ -212,8 +212,11 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 // gather layout of n photons: record k = photon perm[sorted[k]] (either may be
 // null); C null selects the compact two-word layout of the lanes kernel
 // boxes[2 k], boxes[2 k + 1] = (lo, hi) of the positions of records
-// [GATHER_BLOCK k, GATHER_BLOCK (k + 1)) of a (n records)
+// [GATHER_BLOCK k, GATHER_BLOCK (k + 1)) of a (n records); lo.w holds the
+// records' octahedral normal code when they all share it, else MIXED_NORMALS
+// (0x8000 halves: a code oct_encode never makes)
 void launch_gather_boxes(hipStream_t s, const float4* a, int64_t n, float4* boxes);
+#define MIXED_NORMALS 0x80008000u
 inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GATHER_BLOCK; }
 // Records a lanes-layout position array holds past its capacity: the lanes
 // kernel reads positions four at a time and masks out those past a bucket.
