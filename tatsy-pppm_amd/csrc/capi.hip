@@ -566,7 +566,7 @@ hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius) {
     uint32_t* u0 = c->u[0].as<uint32_t>();
     uint32_t* u1 = c->u[1].as<uint32_t>();
     launch_origin_keys(s, c->irr_flux.as<float4>(), n_irr, u0, u1);
-    const int obits = bits_for(((uint64_t)(c->rp.photons - 1) << 6) | 63u);
+    const int obits = bits_for((uint64_t)(c->rp.photons - 1));
     const bool alt = radix_sort_pairs(s, u0, u1, c->u[2].as<uint32_t>(), c->u[3].as<uint32_t>(),
                                       c->hist.as<uint32_t>(), n_irr, obits);
     const int keep = alt ? 3 : 1;
@@ -1196,7 +1196,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     uint32_t* u2 = ctx->u[2].as<uint32_t>();
     uint32_t* u3 = ctx->u[3].as<uint32_t>();
     launch_origin_keys(s, ctx->ph_flux.as<float4>(), n, u0, u1);
-    int obits = bits_for(((uint64_t)(ctx->rp.photons - 1) << 6) | 63u);
+    int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
     bool alt = radix_sort_pairs(s, u0, u1, u2, u3, ctx->hist.as<uint32_t>(), n, obits);
     int keep = alt ? 3 : 1;
     uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
@@ -1538,7 +1538,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         uint32_t* u0 = ctx->u[0].as<uint32_t>();
         uint32_t* u1 = ctx->u[1].as<uint32_t>();
         launch_origin_keys(A, ctx->ph_flux.as<float4>(), n, u0, u1);
-        const int obits = bits_for(((uint64_t)(ctx->rp.photons - 1) << 6) | 63u);
+        const int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
         hipStream_t keep_stream = ctx->stream;
         const bool alt = radix_sort_pairs(A, u0, u1, ctx->u[2].as<uint32_t>(), ctx->u[3].as<uint32_t>(),
                                           ctx->hist.as<uint32_t>(), n, obits);
@@ -2322,7 +2322,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     launch_capacity_check(s, pd, dc, cap);
     uint32_t* u0 = ctx->u[0].as<uint32_t>();
     uint32_t* u1 = ctx->u[1].as<uint32_t>();
-    const int obits = bits_for(((uint64_t)(ctx->rp.photons - 1) << 6) | 63u);
+    const int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
     const uint32_t osent = obits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << obits) - 1);
     launch_origin_keys(s, ctx->ph_flux.as<float4>(), cap, u0, u1, &dc->photons_stored, osent);
     const bool alt = radix_sort_pairs(s, u0, u1, ctx->u[2].as<uint32_t>(), ctx->u[3].as<uint32_t>(),

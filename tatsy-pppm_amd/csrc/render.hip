@@ -432,12 +432,16 @@ __global__ void k_direct_add(const float4* __restrict__ hp_w, int64_t np, float*
     }
 }
 
+// Sort key of a photon record: its photon index.  A photon's records are
+// appended by the lane that traces it, in bounce order, at increasing slots,
+// so the stable sort by index alone orders them by (index, bounce): the
+// bounce's six bits need no radix pass.
 __global__ void k_origin_keys(const float4* __restrict__ ph_flux, int64_t n, uint32_t* keys,
                               uint32_t* vals, const unsigned long long* count, uint32_t sentinel) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool real = !count || i < (int64_t)min((unsigned long long)n, *count);
-    keys[i] = real ? __float_as_uint(ph_flux[i].w) : sentinel;
+    keys[i] = real ? __float_as_uint(ph_flux[i].w) >> 6 : sentinel;
     vals[i] = (uint32_t)i;
 }
 
