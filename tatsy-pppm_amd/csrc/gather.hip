@@ -242,11 +242,19 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     if (valid) {
         const float4 hp = a.hp_pos[i];
-        const float4 hw = a.hp_w[i];
         const v3f x = xyz(hp);
         const v3f nx = oct_decode(__float_as_uint(hp.w));
-        uint32_t mb[8];
-        uint32_t todo = own_buckets(a, hp, mb);
+        // the slots to walk; a slot's bucket id is hashed again when it is
+        // walked, and the hit point's weight read at the end, which keeps the
+        // registers within eight waves per SIMD
+        uint32_t todo;
+        int32_t clo[3];
+        {
+            uint32_t mb[8];
+            todo = own_buckets(a, hp, mb);
+            int32_t chi[3];
+            hit_cells(a, hp, clo, chi);
+        }
         const float r2 = (float)(gather_radius(a) * gather_radius(a));
         const float r2_box = r2 * (1.0f + 1.0f / 4096.0f);
         const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
@@ -255,7 +263,8 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
 #pragma nounroll
         for (int c = 0; c < 8; c++) {
             if (!((todo >> c) & 1u)) continue;
-            const uint32_t b0 = a.off[mb[c]], b1 = a.off[mb[c] + 1];
+            const uint32_t bc = grid_hash(clo[0] + (c & 1), clo[1] + ((c >> 1) & 1), clo[2] + (c >> 2), a.g.size);
+            const uint32_t b0 = a.off[bc], b1 = a.off[bc + 1];
             nb++;
             nc += b1 - b0;
             if (b1 <= b0) continue;
@@ -282,6 +291,7 @@ __global__ __launch_bounds__(256) void k_gather_lanes(GatherArgs a) {
                 lanes_range(a, j0, j1, x, nx, r2, inv_kr, test_normals, ax, ay, az, ni);
             }
         }
+        const float4 hw = a.hp_w[i];
         write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
     }
     flush_counters(a, valid, nb, nc, ni, nt, nk);
