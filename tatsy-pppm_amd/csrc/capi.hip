@@ -1008,6 +1008,9 @@ int pppm_camera_set(void* ctx_raw, const pppm_camera* cam) {
     CHECK_CTX(ctx);
     if (!cam || cam->width <= 0 || cam->height <= 0 || !(cam->fov_y_deg > 0.0 && cam->fov_y_deg < 180.0))
         return ctx->fail(PPPM_ERR_INVALID, "bad camera");
+    // pixel ids travel as 32-bit words, all ones meaning none
+    if ((int64_t)cam->width * (int64_t)cam->height > (int64_t)1 << 31)
+        return ctx->fail(PPPM_ERR_INVALID, "at most 2^31 pixels");
     ctx->cam = *cam;
     ctx->has_cam = true;
     ctx->rendering = false;

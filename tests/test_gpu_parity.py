@@ -7,6 +7,8 @@ floating point (FP32 GPU against the FP64 oracle), written next to each check.
 import math
 import sys
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -282,6 +284,18 @@ def test_render_is_bitwise_deterministic(gpu_ctx, c1_small):
     _render_one_pass(gpu_ctx, s, 30000, 0.0866, passes=2)
     b = gpu_ctx.image()
     assert np.array_equal(a, b)
+
+
+def test_camera_larger_than_32_bit_pixel_ids_is_rejected(gpu_ctx, c1_small):
+    s, _ = c1_small
+    say("camera size limit")
+    cam = P.Camera()
+    C.memmove(C.addressof(cam), C.addressof(s.camera), C.sizeof(P.Camera))
+    cam.width, cam.height = 65536, 65536
+    with pytest.raises(P.PPPMError) as e:
+        gpu_ctx.set_camera(cam)
+    assert e.value.code == P.ERR_INVALID
+    gpu_ctx.set_camera(s.camera)  # the context stays usable
 
 
 def test_capacity_error_is_reported(gpu_ctx, c1_small):
