@@ -1215,8 +1215,8 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
-                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
-    if (g.sub_bits > 0) launch_gather_boxes(s, ctx->ph_a.as<float4>(), n, ctx->boxes.as<float4>());
+                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
+                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     // gather, once the hit points are sorted on the second stream
@@ -1560,8 +1560,8 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ctx->stream = keep_stream;
         launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                              S.ph_a->as<float4>(), S.ph_b->as<float4>(),
-                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>());
-        if (g.sub_bits > 0) launch_gather_boxes(A, S.ph_a->as<float4>(), n, S.boxes->as<float4>());
+                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>(),
+                             g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
         // C: direct emission and gather of this pass, in pass order
@@ -2342,8 +2342,8 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
                                         &dc->photons_stored);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
-                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>());
-    if (g.sub_bits > 0) launch_gather_boxes(s, ctx->ph_a.as<float4>(), cap, ctx->boxes.as<float4>());
+                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
+                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
     launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048);

@@ -445,15 +445,11 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
     flush_counters(a, h.valid, h.nb, h.nc, acc.n, h.nc, 0);
 }
 
-// Box of each block of GATHER_BLOCK consecutive records: half-wave min / max
-// reductions (GATHER_BLOCK = 32 lanes per block).
-__global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__ a, int64_t n,
-                                                      float4* __restrict__ boxes) {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// The box and shared normal code of the block of GATHER_BLOCK records that
+// record j belongs to, from record p on each of the block's 32 consecutive
+// lanes: half-wave min / max reductions; the block's first lane writes them.
+__device__ __forceinline__ void block_box(float4 p, int64_t j, float4* __restrict__ boxes) {
     const int64_t k = j / GATHER_BLOCK;
-    const int64_t nbox = (n + GATHER_BLOCK - 1) / GATHER_BLOCK;
-    if (k >= nbox) return;  // whole blocks of 32 lanes leave together
-    const float4 p = a[min(j, n - 1)];  // the last record repeated past the end
     float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
     uint32_t emin = __float_as_uint(p.w), emax = emin;
 #pragma unroll
@@ -472,6 +468,16 @@ __global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__
         boxes[2 * k] = make_float4(lx, ly, lz, __uint_as_float(e));
         boxes[2 * k + 1] = make_float4(hx, hy, hz, 0.0f);
     }
+}
+
+// Box of each block of GATHER_BLOCK consecutive records (GATHER_BLOCK = 32
+// lanes per block).
+__global__ __launch_bounds__(256) void k_gather_boxes(const float4* __restrict__ a, int64_t n,
+                                                      float4* __restrict__ boxes) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nbox = (n + GATHER_BLOCK - 1) / GATHER_BLOCK;
+    if (j / GATHER_BLOCK >= nbox) return;  // whole blocks of 32 lanes leave together
+    block_box(a[min(j, n - 1)], j, boxes);  // the last record repeated past the end
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -501,6 +507,29 @@ __global__ void k_hit_keys(const float4* __restrict__ hp_pos, const float4* __re
     for (int d = 0; d < 3; d++) q[d] = (uint32_t)fmin(fmax(c[d], 0.0), 1023.0);
     keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
     vals[i] = (uint32_t)i;
+}
+
+// Compact gather layout with the block boxes of the lanes kernel in the same
+// pass: record k = photon perm[sorted[k]]; past n, lanes take record n - 1 for
+// the box of the last block and write nothing else.
+__global__ __launch_bounds__(256) void k_gather_layout_boxes(const float4* __restrict__ pos,
+                                                             const float4* __restrict__ flux,
+                                                             const uint32_t* __restrict__ perm,
+                                                             const uint32_t* __restrict__ sorted, int64_t n,
+                                                             float4* __restrict__ A, float4* __restrict__ B,
+                                                             float4* __restrict__ boxes) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nbox = (n + GATHER_BLOCK - 1) / GATHER_BLOCK;
+    if (k / GATHER_BLOCK >= nbox) return;  // whole blocks of 32 lanes leave together
+    const int64_t kk = min(k, n - 1);
+    const uint32_t j = sorted ? sorted[kk] : (uint32_t)kk;
+    const uint32_t src = perm ? perm[j] : j;
+    const float4 p = pos[src];
+    if (k < n) {
+        A[k] = p;
+        B[k] = flux[src];
+    }
+    block_box(p, k, boxes);
 }
 
 // gather layout of the photons in bucket order: record k = photon perm[sorted[k]]
@@ -603,8 +632,14 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 }
 
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
-                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C) {
+                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C, float4* boxes) {
     if (n <= 0) return;
+    if (boxes && !C) {
+        const int64_t threads = gather_box_count(n) * GATHER_BLOCK;
+        hipLaunchKernelGGL(k_gather_layout_boxes, dim3(grid1(threads, 256)), dim3(256), 0, s, pos, flux, perm,
+                           sorted, n, A, B, boxes);
+        return;
+    }
     hipLaunchKernelGGL(k_gather_layout, dim3(grid1(n, 256)), dim3(256), 0, s, pos, flux, perm, sorted, n, A, B,
                        C);
 }

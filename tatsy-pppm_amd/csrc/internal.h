@@ -210,7 +210,8 @@ void launch_scatter_used(hipStream_t s, const uint32_t* used, const uint32_t* so
 void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, int64_t cap, GridParams g,
                      double radius, uint32_t* keys, uint32_t* vals);
 // gather layout of n photons: record k = photon perm[sorted[k]] (either may be
-// null); C null selects the compact two-word layout of the lanes kernel
+// null); C null selects the compact two-word layout of the lanes kernel, which
+// then also writes the block boxes (as launch_gather_boxes) when boxes is set
 // boxes[2 k], boxes[2 k + 1] = (lo, hi) of the positions of records
 // [GATHER_BLOCK k, GATHER_BLOCK (k + 1)) of a (n records); lo.w holds the
 // records' octahedral normal code when they all share it, else MIXED_NORMALS
@@ -222,7 +223,8 @@ inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GAT
 // kernel reads positions four at a time and masks out those past a bucket.
 #define GATHER_PAD 4
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
-                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C);
+                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C,
+                          float4* boxes = nullptr);
 
 // origin keys of the stored photons, and the permutation applied to the records;
 // with a device count, slots i >= min(*count, n) get key `sentinel`
