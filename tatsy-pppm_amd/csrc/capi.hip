@@ -86,6 +86,9 @@ struct Ctx {
         counters, scratch;
     // hit points in Morton order of their half cells (tiled gather)
     DBuf hp_pos_s, hp_w_s, hk[4], hhist;
+    // the photon records' origin order without a sort: rank of each record
+    // among its photon's records, records per photon, their exclusive scan
+    DBuf rec_rank, rec_count, rec_start, rec_tiles;
     // last pass
     bool last_valid = false;
     int64_t last_n = 0;
@@ -235,7 +238,7 @@ int render_sub_bits(const Ctx* c, uint32_t size, bool sentinel) {
 
 uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32_t* perm, int64_t n,
                      const GridParams& g, int keep, const unsigned long long* count = nullptr,
-                     uint32_t* off_out = nullptr) {
+                     uint32_t* off_out = nullptr, bool keys_ready = false) {
     uint32_t* b[4];
     int k = 0;
     for (int i = 0; i < 5 && k < 4; i++)
@@ -244,7 +247,7 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
     const int shift = 3 * g.sub_bits;
     const int bits = bucket_bits + shift;
     const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
-    launch_point_keys(c->stream, pos4, pos3, perm, n, g, b[0], b[1], count, sentinel);
+    if (!keys_ready) launch_point_keys(c->stream, pos4, pos3, perm, n, g, b[0], b[1], count, sentinel);
     bool alt = radix_sort_pairs(c->stream, b[0], b[1], b[2], b[3], c->hist.as<uint32_t>(), n, bits);
     uint32_t* keys = alt ? b[2] : b[0];
     uint32_t* vals = alt ? b[3] : b[1];
@@ -252,6 +255,21 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
     launch_bucket_offsets(c->stream, keys, n, g.size, off_out ? off_out : c->off.as<uint32_t>(), shift,
                           c->hist.as<uint32_t>(), (int64_t)(c->hist.n / sizeof(uint32_t)));
     return vals;
+}
+
+// The photon records of a pass in origin order (stable by photon index) with
+// their grid keys, from the ranks and counts the photon pass wrote: perm1 in
+// u[4], keys and values in u[0], u[1] for build_grid(..., keep = 4,
+// keys_ready).  Returns perm1.
+uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count) {
+    const int bits = grid_bucket_bits(g.size, count != nullptr) + 3 * g.sub_bits;
+    const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
+    launch_origin_starts(s, c->rec_count.as<uint32_t>(), c->rp.photons, c->rec_tiles.as<uint32_t>(),
+                         c->rec_start.as<uint32_t>());
+    launch_origin_point_keys(s, c->ph_pos.as<float4>(), c->ph_flux.as<float4>(), c->rec_rank.as<uint8_t>(),
+                             c->rec_start.as<uint32_t>(), n, count, g, sentinel, c->u[0].as<uint32_t>(),
+                             c->u[1].as<uint32_t>(), c->u[4].as<uint32_t>());
+    return c->u[4].as<uint32_t>();
 }
 
 hipError_t ensure_sort_buffers(Ctx* c, int64_t n, uint32_t S) {
@@ -720,6 +738,7 @@ int pppm_destroy(void* ctx_raw) {
     c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
     c->img_sum.release(); c->img_comp.release(); c->img_dsum.release(); c->img_dcomp.release();
     c->counters.release(); c->scratch.release();
+    c->rec_rank.release(); c->rec_count.release(); c->rec_start.release(); c->rec_tiles.release();
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream4);
@@ -1074,6 +1093,10 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     const int64_t cap = ctx->cap;
     TRY_HIP(ctx, ctx->ph_pos.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_flux.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->rec_rank.ensure((size_t)cap));
+    TRY_HIP(ctx, ctx->rec_count.ensure(sizeof(uint32_t) * (size_t)p->photons));
+    TRY_HIP(ctx, ctx->rec_start.ensure(sizeof(uint32_t) * (size_t)p->photons));
+    TRY_HIP(ctx, ctx->rec_tiles.ensure(sizeof(uint32_t) * (size_t)origin_tiles(p->photons)));
     TRY_HIP(ctx, ctx->ph_a.ensure(sizeof(float4) * (size_t)(cap + GATHER_PAD)));
     TRY_HIP(ctx, ctx->ph_b.ensure(sizeof(float4) * (size_t)cap));
     TRY_HIP(ctx, ctx->ph_c.ensure(sizeof(float4) * (size_t)cap));
@@ -1164,6 +1187,8 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     pa.irr_flux = ctx->sss_present ? ctx->irr_flux.as<float4>() : nullptr;
     pa.irr_cap = ctx->sss_present ? ctx->cap : 0;
     pa.count_traversal = ctx->count_traversal;
+    pa.rec_rank = ctx->rec_rank.as<uint8_t>();
+    pa.rec_count = ctx->rec_count.as<uint32_t>();
     launch_photon_pass(s, pa, photon_grid);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -1193,18 +1218,8 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         TRY_HIP(ctx, subsurface_irradiance(ctx, n_irr, radius));
     }
     TRY_HIP(ctx, hipEventRecord(ctx->ev[9], s));
-    // deterministic photon order: stable sort by (photon index, bounce)
-    uint32_t* u0 = ctx->u[0].as<uint32_t>();
-    uint32_t* u1 = ctx->u[1].as<uint32_t>();
-    uint32_t* u2 = ctx->u[2].as<uint32_t>();
-    uint32_t* u3 = ctx->u[3].as<uint32_t>();
-    launch_origin_keys(s, ctx->ph_flux.as<float4>(), n, u0, u1);
-    int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
-    bool alt = radix_sort_pairs(s, u0, u1, u2, u3, ctx->hist.as<uint32_t>(), n, obits);
-    int keep = alt ? 3 : 1;
-    uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
-    TRY_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    // hash grid over the ordered photons
+    // hash grid over the photons in their deterministic order, (photon index,
+    // bounce): the order and the grid keys in one pass, then the grid sort
     uint32_t S = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
     GridParams g;
     for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
@@ -1212,7 +1227,10 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     g.size = S;
     g.sub_bits = render_sub_bits(ctx, S, false);
     TRY_HIP(ctx, ctx->off.ensure(sizeof(uint32_t) * ((size_t)S + 1)));
-    uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep);
+    const int keep = 4;
+    uint32_t* perm1 = origin_order(ctx, s, n, g, nullptr);
+    TRY_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr, nullptr, true);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
@@ -1523,6 +1541,8 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         pa.irr_flux = nullptr;
         pa.irr_cap = 0;
         pa.count_traversal = ctx->count_traversal;
+        pa.rec_rank = ctx->rec_rank.as<uint8_t>();
+        pa.rec_count = ctx->rec_count.as<uint32_t>();
         launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
@@ -1538,15 +1558,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
                                                     " records, the pass stored " + std::to_string(n));
         }
         TRY_HIP(ctx, hipEventRecord(E[O0], A));
-        uint32_t* u0 = ctx->u[0].as<uint32_t>();
-        uint32_t* u1 = ctx->u[1].as<uint32_t>();
-        launch_origin_keys(A, ctx->ph_flux.as<float4>(), n, u0, u1);
-        const int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
         hipStream_t keep_stream = ctx->stream;
-        const bool alt = radix_sort_pairs(A, u0, u1, ctx->u[2].as<uint32_t>(), ctx->u[3].as<uint32_t>(),
-                                          ctx->hist.as<uint32_t>(), n, obits);
-        const int keep = alt ? 3 : 1;
-        uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
         const uint32_t Sz = ctx->rp.hash_size > 0 ? (uint32_t)ctx->rp.hash_size : (uint32_t)(n > 0 ? n : 1);
         sizes[(size_t)k] = (int32_t)Sz;
         GridParams g;
@@ -1554,9 +1566,11 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         g.scale = 1.0 / (2.0 * radius);
         g.size = Sz;
         g.sub_bits = render_sub_bits(ctx, Sz, false);
+        const int keep = 4;
+        uint32_t* perm1 = origin_order(ctx, A, n, g, nullptr);
         ctx->stream = A;
         uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr,
-                                      S.off->as<uint32_t>());
+                                      S.off->as<uint32_t>(), true);
         ctx->stream = keep_stream;
         launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
                              S.ph_a->as<float4>(), S.ph_b->as<float4>(),
@@ -2321,25 +2335,20 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     pa.irr_pos = nullptr;
     pa.irr_flux = nullptr;
     pa.irr_cap = 0;
+    pa.rec_rank = ctx->rec_rank.as<uint8_t>();
+    pa.rec_count = ctx->rec_count.as<uint32_t>();
     launch_photon_pass(s, pa, photon_grid);
     launch_capacity_check(s, pd, dc, cap);
-    uint32_t* u0 = ctx->u[0].as<uint32_t>();
-    uint32_t* u1 = ctx->u[1].as<uint32_t>();
-    const int obits = bits_for((uint64_t)(ctx->rp.photons - 1));
-    const uint32_t osent = obits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << obits) - 1);
-    launch_origin_keys(s, ctx->ph_flux.as<float4>(), cap, u0, u1, &dc->photons_stored, osent);
-    const bool alt = radix_sort_pairs(s, u0, u1, ctx->u[2].as<uint32_t>(), ctx->u[3].as<uint32_t>(),
-                                      ctx->hist.as<uint32_t>(), cap, obits);
-    const int keep = alt ? 3 : 1;
-    const uint32_t* perm1 = ctx->u[keep].as<uint32_t>();
     GridParams g;
     for (int d = 0; d < 3; d++) g.bmin[d] = ctx->grid_bmin[d];
     g.scale = 0.0;
     g.size = (uint32_t)ctx->rp.hash_size;
     g.sub_bits = render_sub_bits(ctx, g.size, true);
     g.pd = pd;
+    const int keep = 4;
+    const uint32_t* perm1 = origin_order(ctx, s, cap, g, &dc->photons_stored);
     const uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, cap, g, keep,
-                                        &dc->photons_stored);
+                                        &dc->photons_stored, nullptr, true);
     launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),

@@ -39,6 +39,19 @@ __device__ inline uint32_t spread3(uint32_t v, int bits) {  // bit i -> bit 3 i
     return o;
 }
 
+// Grid sort key of a point: its bucket, then (sub_bits > 0) its cell-local
+// Morton sub-cell below it.
+__device__ inline uint32_t point_key(double x, double y, double z, const GridParams& g) {
+    const double sc = grid_scale(g);
+    uint32_t key = hash3(cell_of(x, g.bmin[0], sc), cell_of(y, g.bmin[1], sc), cell_of(z, g.bmin[2], sc), g.size);
+    if (g.sub_bits > 0) {
+        const int m = g.sub_bits;
+        key = (key << (3 * m)) | spread3(sub_cell(x, g.bmin[0], sc, m), m) |
+              (spread3(sub_cell(y, g.bmin[1], sc, m), m) << 1) | (spread3(sub_cell(z, g.bmin[2], sc, m), m) << 2);
+    }
+    return key;
+}
+
 __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __restrict__ pos3,
                              const uint32_t* __restrict__ perm, int64_t n, GridParams g,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -58,15 +71,116 @@ __global__ void k_point_keys(const float4* __restrict__ pos4, const float* __res
     } else {
         x = pos3[3 * j]; y = pos3[3 * j + 1]; z = pos3[3 * j + 2];
     }
-    const double sc = grid_scale(g);
-    uint32_t key = hash3(cell_of(x, g.bmin[0], sc), cell_of(y, g.bmin[1], sc), cell_of(z, g.bmin[2], sc), g.size);
-    if (g.sub_bits > 0) {
-        const int m = g.sub_bits;
-        key = (key << (3 * m)) | spread3(sub_cell(x, g.bmin[0], sc, m), m) |
-              (spread3(sub_cell(y, g.bmin[1], sc, m), m) << 1) | (spread3(sub_cell(z, g.bmin[2], sc, m), m) << 2);
-    }
-    keys[i] = key;
+    keys[i] = point_key(x, y, z, g);
     vals[i] = (uint32_t)i;
+}
+
+// Exclusive scan of the per-photon record counts, in tiles of ORIGIN_TILE:
+// the sum of every tile first, then each tile adds the sums of the tiles
+// before it (a few thousand words, read from L2) and scans itself.
+#define ORIGIN_TILE 4096
+__global__ __launch_bounds__(256) void k_origin_tile_sums(const uint32_t* __restrict__ count, int64_t m,
+                                                          uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t red[256];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * ORIGIN_TILE;
+    uint32_t sum = 0;
+    for (int r = 0; r < ORIGIN_TILE / 256; r++) {
+        const int64_t i = base + (int64_t)r * 256 + tid;
+        if (i < m) sum += count[i];
+    }
+    red[tid] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) tile_sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_origin_starts(const uint32_t* __restrict__ count, int64_t m,
+                                                       const uint32_t* __restrict__ tile_sums,
+                                                       uint32_t* __restrict__ start) {
+    __shared__ uint32_t red[256];
+    __shared__ uint32_t s_scan[256];
+    const int tid = threadIdx.x;
+    constexpr int PER = ORIGIN_TILE / 256;  // consecutive counts per thread
+    // the records of every earlier tile
+    uint32_t before = 0;
+    for (int64_t t = tid; t < (int64_t)blockIdx.x; t += 256) before += tile_sums[t];
+    red[tid] = before;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    const uint32_t tile_base = red[0];
+    // this thread's run of PER counts, then a block scan of the run sums
+    const int64_t base = (int64_t)blockIdx.x * ORIGIN_TILE + (int64_t)tid * PER;
+    uint32_t c[PER];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        c[k] = base + k < m ? count[base + k] : 0u;
+        run += c[k];
+    }
+    s_scan[tid] = run;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = tid >= off ? s_scan[tid - off] : 0u;
+        __syncthreads();
+        s_scan[tid] += t;
+        __syncthreads();
+    }
+    uint32_t acc = tile_base + s_scan[tid] - run;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (base + k < m) start[base + k] = acc;
+        acc += c[k];
+    }
+}
+
+// Every position a sentinel at its own place: with a device count, a pass
+// whose records overflowed the capacity leaves positions that no record
+// reaches (the pass is reported as failed, but what follows must stay in
+// bounds).
+__global__ void k_origin_fill(int64_t n, uint32_t sentinel, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals, uint32_t* __restrict__ perm) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    keys[p] = sentinel;
+    vals[p] = (uint32_t)p;
+    perm[p] = (uint32_t)p;
+}
+
+// Photon records in origin order without a sort, and their grid keys: record
+// slot j of photon i (key word i << 6 | bounce) holding rank r among that
+// photon's stored records goes to position p = start[i] + r, which is its
+// place in the stable sort by photon index, since a photon's records are
+// appended in bounce order.  keys[p] = grid key, vals[p] = p, perm[p] = j.
+// With a device count, slots j >= min(*count, n) keep their place with key
+// `sentinel`; a position past n (records the capacity dropped: the pass is
+// reported as failed) is not written, and the positions those records would
+// have left are sentinels from k_origin_fill.
+__global__ __launch_bounds__(256) void k_origin_point_keys(
+    const float4* __restrict__ pos, const float4* __restrict__ flux, const uint8_t* __restrict__ rank,
+    const uint32_t* __restrict__ start, int64_t n, const unsigned long long* __restrict__ count, GridParams g,
+    uint32_t sentinel, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ perm) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    if (count && j >= (int64_t)min((unsigned long long)n, *count)) {
+        keys[j] = sentinel;
+        vals[j] = (uint32_t)j;
+        perm[j] = (uint32_t)j;
+        return;
+    }
+    const uint32_t i = __float_as_uint(flux[j].w) >> 6;
+    const int64_t p = (int64_t)start[i] + rank[j];
+    if (p >= n) return;
+    const float4 q = pos[j];
+    keys[p] = point_key(q.x, q.y, q.z, g);
+    vals[p] = (uint32_t)p;
+    perm[p] = (uint32_t)j;
 }
 
 // off[h] = lower_bound(sorted buckets, h) for h in [0, size], the bucket of a
@@ -257,6 +371,26 @@ void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, con
     if (n <= 0) return;
     hipLaunchKernelGGL(k_point_keys, dim3(grid1(n, 256)), dim3(256), 0, s, pos4, pos3, perm, n, g,
                        keys, vals, count, sentinel);
+}
+
+int64_t origin_tiles(int64_t photons) { return (photons + ORIGIN_TILE - 1) / ORIGIN_TILE; }
+
+void launch_origin_starts(hipStream_t s, const uint32_t* count, int64_t photons, uint32_t* tile_sums,
+                          uint32_t* start) {
+    if (photons <= 0) return;
+    const int64_t nt = origin_tiles(photons);
+    hipLaunchKernelGGL(k_origin_tile_sums, dim3((unsigned)nt), dim3(256), 0, s, count, photons, tile_sums);
+    hipLaunchKernelGGL(k_origin_starts, dim3((unsigned)nt), dim3(256), 0, s, count, photons, tile_sums, start);
+}
+
+void launch_origin_point_keys(hipStream_t s, const float4* pos, const float4* flux, const uint8_t* rank,
+                              const uint32_t* start, int64_t n, const unsigned long long* count, GridParams g,
+                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm) {
+    if (n <= 0) return;
+    if (count)
+        hipLaunchKernelGGL(k_origin_fill, dim3(grid1(n, 256)), dim3(256), 0, s, n, sentinel, keys, vals, perm);
+    hipLaunchKernelGGL(k_origin_point_keys, dim3(grid1(n, 256)), dim3(256), 0, s, pos, flux, rank, start, n, count,
+                       g, sentinel, keys, vals, perm);
 }
 
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,

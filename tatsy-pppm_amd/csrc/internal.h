@@ -54,6 +54,18 @@ __device__ __forceinline__ double grid_scale(const GridParams& g) { return g.pd 
 void launch_point_keys(hipStream_t s, const float4* pos4, const float* pos3, const uint32_t* perm,
                        int64_t n, GridParams g, uint32_t* keys, uint32_t* vals,
                        const unsigned long long* count = nullptr, uint32_t sentinel = 0xFFFFFFFFu);
+// Origin order without a sort (see k_origin_point_keys): start[i] = number of
+// records of photons 0..i-1, from the per-photon counts the photon pass wrote
+// (tile_sums: origin_tiles(photons) words of scratch) ...
+int64_t origin_tiles(int64_t photons);
+void launch_origin_starts(hipStream_t s, const uint32_t* count, int64_t photons, uint32_t* tile_sums,
+                          uint32_t* start);
+// ... then every record slot j goes to p = start[photon] + rank[j] with its
+// grid key: keys[p], vals[p] = p, perm[p] = j (with a device count, unused
+// slots and positions keep their place with key `sentinel`)
+void launch_origin_point_keys(hipStream_t s, const float4* pos, const float4* flux, const uint8_t* rank,
+                              const uint32_t* start, int64_t n, const unsigned long long* count, GridParams g,
+                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm);
 // off[h] = number of sorted keys whose bucket (key >> shift) is < h, for h = 0..size
 // (scratch: at least (size + 1) / 4096 + 1 free words, or null)
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
@@ -131,6 +143,10 @@ struct PhotonPassArgs {
     float4* irr_flux; // rgb, origin key bits
     int64_t irr_cap;
     int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
+    // when set: rec_rank[slot] = the record's rank among its photon's stored
+    // records, rec_count[photon index] = that photon's stored records
+    uint8_t* rec_rank = nullptr;
+    uint32_t* rec_count = nullptr;
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 

@@ -143,6 +143,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     v3f o = mk(0.0f, 0.0f, 0.0f), dir = o, flux = o;
     uint32_t i = 0;
     int bounce = 0;
+    uint32_t stored = 0;     // records the current photon has stored
     bool live = false;       // the lane holds a photon path
     bool exhausted = false;  // the pass-wide counter has run out (wave-uniform)
     TraceRay ray;
@@ -184,7 +185,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                             a.ph_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
                             a.ph_flux[slot] = make_float4(flux.x, flux.y, flux.z,
                                                           __uint_as_float((i << 6) | (uint32_t)bounce));
+                            if (a.rec_rank) a.rec_rank[slot] = (uint8_t)stored;
                         }
+                        stored++;
                         if (r0 < m.rr_mean) {
                             dir = on_hemisphere(on, r1, r2);
                             o = spawn(hp, ng, dir, a.ray_eps);
@@ -206,6 +209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             }
             if (live && (bounce >= a.max_bounces || vmax3(flux) <= 0.0f)) live = false;
             if (live) trace_start(sc, o, dir, 0.0f, TRACE_TMAX, ray, ts);
+            else if (a.rec_count) a.rec_count[i] = stored;  // the path has ended
         }
         if (!exhausted) {
             const uint64_t need = __ballot(!live);
@@ -221,8 +225,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                         rng.init(a.seed, KIND_PHOTON, pass, i, a.scene_id);
                         emit_photon(a, sc, rng, o, dir, flux);
                         bounce = 0;
+                        stored = 0;
                         live = a.max_bounces > 0 && vmax3(flux) > 0.0f;
                         if (live) trace_start(sc, o, dir, 0.0f, TRACE_TMAX, ray, ts);
+                        else if (a.rec_count) a.rec_count[i] = 0u;
                     }
                 }
                 exhausted = base + (unsigned long long)__popcll(need) >= (unsigned long long)a.n_photons;
