@@ -111,7 +111,6 @@ struct Counters {
     unsigned long long irr_stored;   // photons stored on BSSRDF surfaces
     unsigned long long sss_hits;     // eye paths ending in the subsurface estimate
     unsigned long long photon_next;  // next photon index handed to a photon-pass lane
-    unsigned long long eye_next;     // next pixel handed to an eye-pass lane
     unsigned long long gather_tested;  // photon records distance-tested by the gather
     unsigned long long gather_blocks;  // block boxes tested by the lanes gather
     // traversal work, counted by the counting kernels only
@@ -121,8 +120,9 @@ struct Counters {
 // Persistent photon-pass blocks per CU: at least the blocks one CU holds at
 // once (the kernel regenerates paths, so extra blocks find no work).
 #define PHOTON_BLOCKS_PER_CU 8
-// The same for the eye pass, which regenerates pixel paths.
-#define EYE_BLOCKS_PER_CU 8
+// Eye-pass blocks per CU at most: one lane per pixel, so a 1024^2 image takes
+// 4096 blocks, which the hardware hands out as earlier blocks finish.
+#define EYE_BLOCKS_PER_CU 64
 
 struct PhotonPassArgs {
     SceneDev sc;
@@ -175,8 +175,8 @@ struct EyePassArgs {
     int64_t sss_cap;
     int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
 };
-// grid: the persistent grid (blocks; at most one block per 256 pixels is used)
-void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid);
+// grid_limit: at most this many blocks (a grid-stride loop covers the rest)
+void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 // img += the direct emission stored in the pixel slots without hit point
 // (an eye pass run with img_sum = null)
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp);

@@ -90,10 +90,6 @@ __device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
 #ifndef PHOTON_TRACE_KEEP
 #define PHOTON_TRACE_KEEP 16
 #endif
-// The same for an eye wave.
-#ifndef EYE_TRACE_KEEP
-#define EYE_TRACE_KEEP 16
-#endif
 
 // Emission of photon i: light by power, uniform point, cosine direction (or
 // the learned proposal's mixture); returns the ray origin, direction, flux.
@@ -257,158 +253,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 }
 
 
-// The end of pixel p's eye path: its slot and, when the kernel adds it
-// itself, the direct emission the path saw.  The hit point of pixel p lives
-// in slot p (pixel id all ones: none), so the hit point order entering the
-// Morton sort is the pixel order; a slot without hit point carries the
-// emission the path saw directly (a path ends at a hit point or at an
-// emitter, never both).
-__device__ inline void eye_finish(const EyePassArgs& a, uint32_t p, float4 hit_pos, float4 hit_w, v3f direct) {
-    if (__float_as_uint(hit_w.w) == NO_HITPOINT) hit_w = make_float4(direct.x, direct.y, direct.z, hit_w.w);
-    a.hp_pos[p] = hit_pos;
-    a.hp_w[p] = hit_w;
-    if (a.img_sum && (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f)) {
-        kahan_add(a.img_sum, a.img_comp, 3 * (int64_t)p + 0, direct.x);
-        kahan_add(a.img_sum, a.img_comp, 3 * (int64_t)p + 1, direct.y);
-        kahan_add(a.img_sum, a.img_comp, 3 * (int64_t)p + 2, direct.z);
-    }
-}
-
-// One lane per pixel path, regenerated like the photon pass: the wave traces
-// until at most EYE_TRACE_KEEP of its lanes still have a ray in flight, the
-// lanes whose ray is finished shade it, and lanes whose path has ended take
-// the next pixel from a pass-wide counter.  A pixel's random numbers are keyed
-// by its index and its outputs go to its own slot, so neither depends on the
-// lane that traces it.
+// One lane per pixel path (a grid-stride loop past the grid limit).  Not
+// persistent on purpose: the eye pass runs beside the gather of an earlier
+// pass, and its blocks, which end at different times, hand their registers
+// back to the gather as they end; a persistent eye kernel with path
+// regeneration was faster alone (2.8 against 3.2 ms) but held its registers
+// for the whole pass and slowed the pipelined passes by 1 %.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
-    int32_t ovf_stack[STACK_OVF];
     unsigned long long segs = 0, nhit = 0;
     bool ovf = false;
     TravCount tc;
     const SceneDev& sc = a.sc;
     const int W = a.cam.width, H = a.cam.height;
-    const uint32_t np = (uint32_t)W * (uint32_t)H;
-    const uint32_t pass = a.pd ? a.pd->pass : a.pass;
-    const int lane = threadIdx.x & 63;
+    const int64_t np = (int64_t)W * H;
     const v3f C = mk(a.cam_basis[0], a.cam_basis[1], a.cam_basis[2]);
     const v3f D = mk(a.cam_basis[3], a.cam_basis[4], a.cam_basis[5]);
     const v3f U = mk(a.cam_basis[6], a.cam_basis[7], a.cam_basis[8]);
     const v3f V = mk(a.cam_basis[9], a.cam_basis[10], a.cam_basis[11]);
-    Rng rng;
-    v3f o = C, d = C, thr = C;
-    uint32_t p = 0;
-    int bounce = 0;
-    bool live = false;       // the lane holds a pixel path
-    bool exhausted = false;  // every pixel has been handed out (wave-uniform)
-    TraceRay ray;
-    TraceState ts;
-    ts.cur = EMPTY_REF;
-    ts.sp = 0;
-    const float4 no_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 no_hit = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
-    const v3f dark = mk(0.0f, 0.0f, 0.0f);
-    while (true) {
-        if (live && ts.cur == (int32_t)EMPTY_REF) {
-            // the segment's ray is finished: shade its hit
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < np;
+         p += (int64_t)gridDim.x * 256) {
+        const int x = (int)(p % W), row = (int)(p / W);
+        Rng rng;
+        rng.init(a.seed, KIND_EYE, a.pd ? a.pd->pass : a.pass, (uint32_t)p, a.scene_id);
+        const float px = (float)x + rng.at(0);
+        const float py = (float)(H - 1 - row) + rng.at(1);
+        v3f d = normalize(D + U * (2.0f * px / (float)W - 1.0f) + V * (2.0f * py / (float)H - 1.0f));
+        v3f o = C;
+        v3f thr = mk(1.0f, 1.0f, 1.0f);
+        v3f direct = mk(0.0f, 0.0f, 0.0f);
+        float4 hit_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
+        for (int bounce = 0;; bounce++) {
+            if (bounce >= a.max_bounces) break;
+            Hit h = trace_closest<256, COUNT>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf, &tc);
             segs++;
-            const Hit h = ts.h;
-            bool cont = false;
-            if (h.tri < 0) {
-                eye_finish(a, p, no_pos, no_hit, dark);
+            if (h.tri < 0) break;
+            const int dd = EYE_DIM_BOUNCE + 3 * bounce;
+            const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
+            (void)r2;
+            const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
+            const v3f hp = o + d * h.t;
+            const v3f ng = tri_normal(sc, h.tri);
+            if (m.type & PPPM_BSDF_EMITTER) {
+                if (dot(d, ng) < 0.0f) direct = thr * mk(m.le[0], m.le[1], m.le[2]);
+                break;
+            }
+            float roulette = m.rr_max;
+            if (bounce >= 3) {
+                if (r0 > roulette) break;
             } else {
-                const int dd = EYE_DIM_BOUNCE + 3 * bounce;
-                const float r0 = rng.at(dd), r1 = rng.at(dd + 1);
-                const Mat m = sc.mats[sc.tri_mat[tri_orig_id(sc, h.tri)]];
-                const v3f hp = o + d * h.t;
-                const v3f ng = tri_normal(sc, h.tri);
-                float roulette = bounce >= 3 ? m.rr_max : 1.0f;
-                if (m.type & PPPM_BSDF_EMITTER) {
-                    eye_finish(a, p, no_pos, no_hit, dot(d, ng) < 0.0f ? thr * mk(m.le[0], m.le[1], m.le[2]) : dark);
-                } else if (bounce >= 3 && r0 > roulette) {
-                    eye_finish(a, p, no_pos, no_hit, dark);
-                } else {
-                    const v3f on = dot(ng, d) < 0.0f ? ng : -ng;
-                    const v3f rho = mk(m.rho[0], m.rho[1], m.rho[2]);
-                    if (m.type & PPPM_BSDF_LAMBERTIAN) {
-                        const v3f w = thr * rho * (PPPM_INV_PI / roulette);
-                        nhit++;
-                        eye_finish(a, p, make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on))),
-                                   make_float4(w.x, w.y, w.z, __uint_as_float(p)), dark);
-                    } else {
-                        float pdf = 1.0f;
-                        v3f nd;
-                        if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(d, on);
-                        else nd = refraction_sample(d, ng, m.ior, r1, &pdf);
-                        if ((m.type & PPPM_BSDF_BSSRDF) && a.sss_pos && (dot(nd, ng) < 0.0f) == (dot(d, ng) < 0.0f)) {
-                            // transmitted into the translucent body: the path ends in the
-                            // dipole estimate at hp, weighted F_t / (1 - P) (ppm_probabilistic.cc:272-296)
-                            const v3f w = thr * (1.0f / (pdf * roulette));
-                            unsigned long long slot = wave_append(&a.counters->sss_hits);
-                            if (slot < (unsigned long long)a.sss_cap) {
-                                a.sss_pos[slot] = make_float4(hp.x, hp.y, hp.z, __int_as_float(m.sss));
-                                a.sss_w[slot] = make_float4(w.x, w.y, w.z, __uint_as_float(p));
-                            }
-                            eye_finish(a, p, no_pos, no_hit, dark);
-                        } else {
-                            thr = thr * rho * (1.0f / (pdf * roulette));
-                            o = spawn(hp, ng, nd, a.ray_eps);
-                            d = nd;
-                            bounce++;
-                            cont = true;
-                        }
-                    }
+                roulette = 1.0f;
+            }
+            const v3f on = dot(ng, d) < 0.0f ? ng : -ng;
+            const v3f rho = mk(m.rho[0], m.rho[1], m.rho[2]);
+            if (m.type & PPPM_BSDF_LAMBERTIAN) {
+                const v3f w = thr * rho * (PPPM_INV_PI / roulette);
+                hit_pos = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                hit_w = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
+                nhit++;
+                break;
+            }
+            float pdf = 1.0f;
+            v3f nd;
+            if (m.type & PPPM_BSDF_SPECULAR) nd = reflect(d, on);
+            else nd = refraction_sample(d, ng, m.ior, r1, &pdf);
+            if ((m.type & PPPM_BSDF_BSSRDF) && a.sss_pos && (dot(nd, ng) < 0.0f) == (dot(d, ng) < 0.0f)) {
+                // transmitted into the translucent body: the path ends in the
+                // dipole estimate at hp, weighted F_t / (1 - P) (ppm_probabilistic.cc:272-296)
+                const v3f w = thr * (1.0f / (pdf * roulette));
+                unsigned long long slot = wave_append(&a.counters->sss_hits);
+                if (slot < (unsigned long long)a.sss_cap) {
+                    a.sss_pos[slot] = make_float4(hp.x, hp.y, hp.z, __int_as_float(m.sss));
+                    a.sss_w[slot] = make_float4(w.x, w.y, w.z, __uint_as_float((uint32_t)p));
                 }
+                break;
             }
-            if (cont && bounce >= a.max_bounces) {
-                eye_finish(a, p, no_pos, no_hit, dark);
-                cont = false;
-            }
-            live = cont;
-            if (live) trace_start(sc, o, d, 0.0f, TRACE_TMAX, ray, ts);
+            thr = thr * rho * (1.0f / (pdf * roulette));
+            o = spawn(hp, ng, nd, a.ray_eps);
+            d = nd;
         }
-        if (!exhausted) {
-            const uint64_t need = __ballot(!live);
-            if (need != 0) {
-                const int leader = __ffsll((unsigned long long)need) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&a.counters->eye_next, (unsigned long long)__popcll(need));
-                base = __shfl(base, leader);
-                if (!live) {
-                    const unsigned long long k = base + (unsigned long long)__popcll(need & lanemask_lt());
-                    if (k < (unsigned long long)np) {
-                        p = (uint32_t)k;
-                        const int x = (int)(p % (uint32_t)W), row = (int)(p / (uint32_t)W);
-                        rng.init(a.seed, KIND_EYE, pass, p, a.scene_id);
-                        const float px = (float)x + rng.at(0);
-                        const float py = (float)(H - 1 - row) + rng.at(1);
-                        d = normalize(D + U * (2.0f * px / (float)W - 1.0f) + V * (2.0f * py / (float)H - 1.0f));
-                        o = C;
-                        thr = mk(1.0f, 1.0f, 1.0f);
-                        bounce = 0;
-                        if (a.max_bounces > 0) {
-                            live = true;
-                            trace_start(sc, o, d, 0.0f, TRACE_TMAX, ray, ts);
-                        } else {
-                            eye_finish(a, p, no_pos, no_hit, dark);
-                        }
-                    }
-                }
-                exhausted = base + (unsigned long long)__popcll(need) >= (unsigned long long)np;
-            }
+        // the hit point of pixel p lives in slot p (pixel id all ones: none), so
+        // the hit point order entering the Morton sort is the pixel order; a
+        // slot without hit point carries the emission the path saw directly
+        // (a path ends at a hit point or at an emitter, never both)
+        if (__float_as_uint(hit_w.w) == NO_HITPOINT) hit_w = make_float4(direct.x, direct.y, direct.z, hit_w.w);
+        a.hp_pos[p] = hit_pos;
+        a.hp_w[p] = hit_w;
+        if (a.img_sum && (direct.x != 0.0f || direct.y != 0.0f || direct.z != 0.0f)) {
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 0, direct.x);
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 1, direct.y);
+            kahan_add(a.img_sum, a.img_comp, 3 * p + 2, direct.z);
         }
-        if (__ballot(live) == 0) {
-            if (exhausted) break;
-            continue;
-        }
-        trace_resume<256, COUNT>(sc, ray, ts, stk, ovf_stack, &ovf, &tc, exhausted ? 0 : EYE_TRACE_KEEP);
     }
     segs = wave_sum(segs);
     nhit = wave_sum(nhit);
     uint64_t any_ovf = __ballot(ovf);
-    if (lane == 0) {
+    if ((threadIdx.x & 63) == 0) {
         atomicAdd(&a.counters->eye_segments, segs);
         atomicAdd(&a.counters->hitpoints, nhit);
         if (any_ovf) atomicAdd(&a.counters->stack_overflow, 1ull);
@@ -416,7 +359,7 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
     if (COUNT) {
         const unsigned long long nn = wave_sum((unsigned long long)tc.nodes);
         const unsigned long long nt = wave_sum((unsigned long long)tc.tris);
-        if (lane == 0) {
+        if ((threadIdx.x & 63) == 0) {
             atomicAdd(&a.counters->eye_nodes, nn);
             atomicAdd(&a.counters->eye_tris, nt);
         }
@@ -512,11 +455,11 @@ void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit) 
         hipLaunchKernelGGL(k_photon<false>, dim3(g), dim3(256), 0, s, a);
 }
 
-void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid) {
+void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
     int64_t np = (int64_t)a.cam.width * a.cam.height;
     if (np <= 0) return;
     unsigned g = grid1(np, 256);
-    if (grid > 0 && g > (unsigned)grid) g = (unsigned)grid;
+    if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
     if (a.count_traversal)
         hipLaunchKernelGGL(k_eye<true>, dim3(g), dim3(256), 0, s, a);
     else
