@@ -258,9 +258,9 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
 }
 
 // The photon records of a pass in origin order (stable by photon index) with
-// their grid keys, from the ranks and counts the photon pass wrote: perm1 in
-// u[4], keys and values in u[0], u[1] for build_grid(..., keep = 4,
-// keys_ready).  Returns perm1.
+// their grid keys, from the ranks and counts the photon pass wrote: perm1
+// (origin position -> record slot) in u[4], keys and values (record slots) in
+// u[0], u[1] for build_grid(..., keep = 4, keys_ready).  Returns perm1.
 uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count) {
     const int bits = grid_bucket_bits(g.size, count != nullptr) + 3 * g.sub_bits;
     const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
@@ -1231,7 +1231,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     uint32_t* perm1 = origin_order(ctx, s, n, g, nullptr);
     TRY_HIP(ctx, hipEventRecord(ctx->ev[2], s));
     uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr, nullptr, true);
-    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
+    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
                          g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);
@@ -1572,7 +1572,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr,
                                       S.off->as<uint32_t>(), true);
         ctx->stream = keep_stream;
-        launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, n,
+        launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, n,
                              S.ph_a->as<float4>(), S.ph_b->as<float4>(),
                              ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>(),
                              g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr);
@@ -1743,10 +1743,28 @@ int pppm_last_grid(void* ctx_raw, int32_t* sorted_idx, int32_t* cell_start, int3
     if (hash_size != ctx->last_S) return ctx->fail(PPPM_ERR_INVALID, "hash_size must equal the pass's");
     (void)hipSetDevice(ctx->device);
     std::vector<uint32_t> off((size_t)hash_size + 1);
-    if (sorted_idx && ctx->last_n > 0)
-        TRY_HIP(ctx, hipMemcpyAsync(sorted_idx, ctx->last_sorted, sizeof(uint32_t) * (size_t)ctx->last_n, hipMemcpyDeviceToHost, ctx->stream));
+    const size_t n = (size_t)std::max<int64_t>(ctx->last_n, 0);
+    std::vector<uint32_t> slot(sorted_idx ? n : 0), perm(sorted_idx ? n : 0);
+    if (sorted_idx && n > 0) {
+        TRY_HIP(ctx, hipMemcpyAsync(slot.data(), ctx->last_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        TRY_HIP(ctx, hipMemcpyAsync(perm.data(), ctx->last_perm1, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    }
     TRY_HIP(ctx, hipMemcpyAsync(off.data(), ctx->off.p, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sorted_idx && n > 0) {
+        // the grid sort carried record slots; a record's id is its origin
+        // position, the inverse of the origin order slot = perm[position]
+        std::vector<uint32_t> origin(n, UINT32_MAX);
+        for (size_t p = 0; p < n; p++) {
+            if (perm[p] >= n || origin[perm[p]] != UINT32_MAX)
+                return ctx->fail(PPPM_ERR_STATE, "origin order is not a permutation");
+            origin[perm[p]] = (uint32_t)p;
+        }
+        for (size_t k = 0; k < n; k++) {
+            if (slot[k] >= n) return ctx->fail(PPPM_ERR_STATE, "grid value out of range");
+            sorted_idx[k] = (int32_t)origin[slot[k]];
+        }
+    }
     if (sorted_idx && ctx->last_sub_bits > 0) {
         // the pass kept each bucket in cell-local Morton order for the gather;
         // the record ids of a bucket in ascending order are its insertion order
@@ -2349,7 +2367,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     const uint32_t* perm1 = origin_order(ctx, s, cap, g, &dc->photons_stored);
     const uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, cap, g, keep,
                                         &dc->photons_stored, nullptr, true);
-    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), perm1, sorted, cap,
+    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
                          g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);

@@ -586,7 +586,7 @@ __global__ __launch_bounds__(256) void k_mark_used(GatherArgs a, const uint32_t*
     }
 }
 
-// used_order[sorted[j]] = used[j]: bucket order -> origin order
+// used_order[sorted[j]] = used[j]: bucket order -> the order of `sorted`'s values
 __global__ __launch_bounds__(256) void k_scatter_used(const uint32_t* used, const uint32_t* sorted, int64_t n,
                                                       uint32_t* used_order) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;

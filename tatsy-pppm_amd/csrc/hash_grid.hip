@@ -157,7 +157,9 @@ __global__ void k_origin_fill(int64_t n, uint32_t sentinel, uint32_t* __restrict
 // slot j of photon i (key word i << 6 | bounce) holding rank r among that
 // photon's stored records goes to position p = start[i] + r, which is its
 // place in the stable sort by photon index, since a photon's records are
-// appended in bounce order.  keys[p] = grid key, vals[p] = p, perm[p] = j.
+// appended in bounce order.  keys[p] = grid key, vals[p] = perm[p] = j: the
+// grid sort then carries record slots, and the gather layout reads each
+// record through one index instead of two.
 // With a device count, slots j >= min(*count, n) keep their place with key
 // `sentinel`; a position past n (records the capacity dropped: the pass is
 // reported as failed) is not written, and the positions those records would
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void k_origin_point_keys(
     if (p >= n) return;
     const float4 q = pos[j];
     keys[p] = point_key(q.x, q.y, q.z, g);
-    vals[p] = (uint32_t)p;
+    vals[p] = (uint32_t)j;
     perm[p] = (uint32_t)j;
 }
 

@@ -325,7 +325,7 @@ struct ContribArgs {
     const float4* ph_flux;
     const uint32_t* perm;    // origin order
     int64_t n;
-    const uint32_t* used;    // per origin position: a hit point's gather accepted it
+    const uint32_t* used;    // per record slot: a hit point's gather accepted it
     float* seg_c;            // [n]
     uint64_t* scan;          // [n]
     PropDev prop;

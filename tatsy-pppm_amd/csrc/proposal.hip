@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_contrib(ContribArgs a) {
         for (int64_t j = k; j < a.n; j++) {
             const float4 f = a.ph_flux[a.perm[j]];
             if ((__float_as_uint(f.w) >> 6) != ph) break;
-            if (a.used[j]) c += ((f.x + f.y) + f.z) * (1.0f / 3.0f);
+            if (a.used[a.perm[j]]) c += ((f.x + f.y) + f.z) * (1.0f / 3.0f);
         }
     }
     a.seg_c[k] = c;
