@@ -369,8 +369,12 @@ def test_pipelined_passes_equal_single_passes(gpu_ctx, mode):
             sts = gpu_ctx.render_passes(2) + gpu_ctx.render_passes(4) + gpu_ctx.render_passes(1)
         else:
             sts = [gpu_ctx.render_pass() for _ in range(7)]
-        outs.append((gpu_ctx.image(), gpu_ctx.last_photons(), sts, gpu_ctx.last_hitpoints()))
+        grid = gpu_ctx.last_grid(sts[-1]["photons_stored"], sts[-1]["hash_size"])
+        outs.append((gpu_ctx.image(), gpu_ctx.last_photons(), sts, gpu_ctx.last_hitpoints(), grid))
     assert np.array_equal(outs[0][0], outs[1][0])
+    # the last pass's hash grid (record ids in origin order, bucket ranges)
+    for a, b in zip(outs[0][4], outs[1][4]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
     for a, b in zip(outs[0][1], outs[1][1]):
         assert np.array_equal(a, b)
     for a, b in zip(outs[0][3], outs[1][3]):
