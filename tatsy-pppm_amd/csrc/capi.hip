@@ -77,12 +77,13 @@ struct Ctx {
     std::vector<double> radii;
     int64_t cap = 0;
     double grid_bmin[3] = {0, 0, 0};
-    // ph_pos / ph_flux: photons as the photon pass appended them; ph_a/b/c: the
-    // gather layout in bucket order
+    // ph_rec: photons as the photon pass appended them, slot j's position at
+    // ph_rec[2 j] and its flux at ph_rec[2 j + 1] (one record, one 32-byte read);
+    // ph_a/b/c: the gather layout in bucket order
     // img_sum / img_comp: the gather (and the dipole sum); img_dsum / img_dcomp:
     // emission seen directly by eye paths (its own Kahan pair, so the eye pass
     // never writes the gather's accumulator)
-    DBuf ph_pos, ph_flux, ph_a, ph_b, ph_c, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp, img_dsum, img_dcomp,
+    DBuf ph_rec, ph_a, ph_b, ph_c, u[5], hist, off, hp_pos, hp_w, img_sum, img_comp, img_dsum, img_dcomp,
         counters, scratch;
     // hit points in Morton order of their half cells (tiled gather)
     DBuf hp_pos_s, hp_w_s, hk[4], hhist;
@@ -266,9 +267,9 @@ uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, co
     const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
     launch_origin_starts(s, c->rec_count.as<uint32_t>(), c->rp.photons, c->rec_tiles.as<uint32_t>(),
                          c->rec_start.as<uint32_t>());
-    launch_origin_point_keys(s, c->ph_pos.as<float4>(), c->ph_flux.as<float4>(), c->rec_rank.as<uint8_t>(),
+    launch_origin_point_keys(s, c->ph_rec.as<float4>(), c->ph_rec.as<float4>() + 1, c->rec_rank.as<uint8_t>(),
                              c->rec_start.as<uint32_t>(), n, count, g, sentinel, c->u[0].as<uint32_t>(),
-                             c->u[1].as<uint32_t>(), c->u[4].as<uint32_t>());
+                             c->u[1].as<uint32_t>(), c->u[4].as<uint32_t>(), 2);
     return c->u[4].as<uint32_t>();
 }
 
@@ -733,7 +734,7 @@ int pppm_destroy(void* ctx_raw) {
     (void)hipStreamSynchronize(c->stream);
     c->tri9.release(); c->tri_mat.release(); c->mats.release(); c->lights.release();
     (void)hipFree(c->nodes); (void)hipFree(c->tris4);
-    c->ph_pos.release(); c->ph_flux.release(); c->ph_a.release(); c->ph_b.release(); c->ph_c.release();
+    c->ph_rec.release(); c->ph_a.release(); c->ph_b.release(); c->ph_c.release();
     for (auto& b : c->u) b.release();
     c->hist.release(); c->off.release(); c->hp_pos.release(); c->hp_w.release();
     c->img_sum.release(); c->img_comp.release(); c->img_dsum.release(); c->img_dcomp.release();
@@ -1091,8 +1092,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     TRY_HIP(ctx, hipMemcpy(ctx->lights.p, ld.data(), sizeof(LightDev) * (size_t)nl, hipMemcpyHostToDevice));
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
     const int64_t cap = ctx->cap;
-    TRY_HIP(ctx, ctx->ph_pos.ensure(sizeof(float4) * (size_t)cap));
-    TRY_HIP(ctx, ctx->ph_flux.ensure(sizeof(float4) * (size_t)cap));
+    TRY_HIP(ctx, ctx->ph_rec.ensure(sizeof(float4) * 2 * (size_t)cap));
     TRY_HIP(ctx, ctx->rec_rank.ensure((size_t)cap));
     TRY_HIP(ctx, ctx->rec_count.ensure(sizeof(uint32_t) * (size_t)p->photons));
     TRY_HIP(ctx, ctx->rec_start.ensure(sizeof(uint32_t) * (size_t)p->photons));
@@ -1176,8 +1176,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     pa.scene_id = ctx->rp.scene_id;
     pa.max_bounces = ctx->rp.max_bounces;
     pa.ray_eps = (float)ctx->rp.ray_eps;
-    pa.ph_pos = ctx->ph_pos.as<float4>();
-    pa.ph_flux = ctx->ph_flux.as<float4>();
+    pa.ph_pos = ctx->ph_rec.as<float4>();
+    pa.ph_flux = ctx->ph_rec.as<float4>() + 1;
+    pa.rec_stride = 2;
     pa.counters = dc;
     const bool prop_active = ctx->prop.active;
     pa.prop.cdf = prop_active ? ctx->prop_cdf.as<float>() : nullptr;
@@ -1230,11 +1231,11 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     const int keep = 4;
     uint32_t* perm1 = origin_order(ctx, s, n, g, nullptr);
     TRY_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr, nullptr, true);
-    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, n,
+    uint32_t* sorted = build_grid(ctx, nullptr, nullptr, perm1, n, g, keep, nullptr, nullptr, true);
+    launch_gather_layout(s, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
-                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);
+                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr, 2);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     // gather, once the hit points are sorted on the second stream
@@ -1274,8 +1275,9 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
             launch_mark_used(s, ga, sorted, ctx->prop_flags.as<uint32_t>());
         }
         ContribArgs ca;
-        ca.ph_pos = ctx->ph_pos.as<float4>();
-        ca.ph_flux = ctx->ph_flux.as<float4>();
+        ca.ph_pos = ctx->ph_rec.as<float4>();
+        ca.ph_flux = ctx->ph_rec.as<float4>() + 1;
+        ca.rec_stride = 2;
         ca.perm = perm1;
         ca.n = n;
         ca.used = ctx->prop_flags.as<uint32_t>();
@@ -1531,8 +1533,9 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         pa.scene_id = ctx->rp.scene_id;
         pa.max_bounces = ctx->rp.max_bounces;
         pa.ray_eps = (float)ctx->rp.ray_eps;
-        pa.ph_pos = ctx->ph_pos.as<float4>();
-        pa.ph_flux = ctx->ph_flux.as<float4>();
+        pa.ph_pos = ctx->ph_rec.as<float4>();
+        pa.ph_flux = ctx->ph_rec.as<float4>() + 1;
+        pa.rec_stride = 2;
         pa.counters = dc;
         pa.prop.cdf = nullptr;
         pa.prop.lambda = 1.0f;
@@ -1569,13 +1572,13 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         const int keep = 4;
         uint32_t* perm1 = origin_order(ctx, A, n, g, nullptr);
         ctx->stream = A;
-        uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, n, g, keep, nullptr,
+        uint32_t* sorted = build_grid(ctx, nullptr, nullptr, perm1, n, g, keep, nullptr,
                                       S.off->as<uint32_t>(), true);
         ctx->stream = keep_stream;
-        launch_gather_layout(A, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, n,
+        launch_gather_layout(A, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, n,
                              S.ph_a->as<float4>(), S.ph_b->as<float4>(),
                              ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>(),
-                             g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr);
+                             g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr, 2);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
         // C: direct emission and gather of this pass, in pass order
@@ -1717,8 +1720,8 @@ int pppm_last_photons(void* ctx_raw, float* pos, float* flux, float* normal, uin
     TRY_HIP(ctx, ctx->scratch.ensure(sizeof(float4) * 2 * (size_t)m));
     float4* a = ctx->scratch.as<float4>();
     float4* b = a + m;
-    launch_permute2(ctx->stream, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), ctx->last_perm1,
-                    nullptr, m, a, b);
+    launch_permute2(ctx->stream, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, ctx->last_perm1,
+                    nullptr, m, a, b, 2);
     std::vector<float4> ha((size_t)m), hb((size_t)m);
     TRY_HIP(ctx, hipMemcpyAsync(ha.data(), a, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost, ctx->stream));
     TRY_HIP(ctx, hipMemcpyAsync(hb.data(), b, sizeof(float4) * (size_t)m, hipMemcpyDeviceToHost, ctx->stream));
@@ -2343,8 +2346,9 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     pa.scene_id = ctx->rp.scene_id;
     pa.max_bounces = ctx->rp.max_bounces;
     pa.ray_eps = (float)ctx->rp.ray_eps;
-    pa.ph_pos = ctx->ph_pos.as<float4>();
-    pa.ph_flux = ctx->ph_flux.as<float4>();
+    pa.ph_pos = ctx->ph_rec.as<float4>();
+    pa.ph_flux = ctx->ph_rec.as<float4>() + 1;
+    pa.rec_stride = 2;
     pa.counters = dc;
     pa.pd = pd;
     pa.prop.cdf = nullptr;
@@ -2365,12 +2369,12 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     g.pd = pd;
     const int keep = 4;
     const uint32_t* perm1 = origin_order(ctx, s, cap, g, &dc->photons_stored);
-    const uint32_t* sorted = build_grid(ctx, ctx->ph_pos.as<float4>(), nullptr, perm1, cap, g, keep,
+    const uint32_t* sorted = build_grid(ctx, nullptr, nullptr, perm1, cap, g, keep,
                                         &dc->photons_stored, nullptr, true);
-    launch_gather_layout(s, ctx->ph_pos.as<float4>(), ctx->ph_flux.as<float4>(), nullptr, sorted, cap,
+    launch_gather_layout(s, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
                          ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
-                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr);
+                         g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr, 2);
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
     launch_eye_pass(s, ea, ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048);

@@ -517,17 +517,17 @@ __global__ __launch_bounds__(256) void k_gather_layout_boxes(const float4* __res
                                                              const uint32_t* __restrict__ perm,
                                                              const uint32_t* __restrict__ sorted, int64_t n,
                                                              float4* __restrict__ A, float4* __restrict__ B,
-                                                             float4* __restrict__ boxes) {
+                                                             float4* __restrict__ boxes, int stride) {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t nbox = (n + GATHER_BLOCK - 1) / GATHER_BLOCK;
     if (k / GATHER_BLOCK >= nbox) return;  // whole blocks of 32 lanes leave together
     const int64_t kk = min(k, n - 1);
     const uint32_t j = sorted ? sorted[kk] : (uint32_t)kk;
     const uint32_t src = perm ? perm[j] : j;
-    const float4 p = pos[src];
+    const float4 p = pos[(int64_t)src * stride];
     if (k < n) {
         A[k] = p;
-        B[k] = flux[src];
+        B[k] = flux[(int64_t)src * stride];
     }
     block_box(p, k, boxes);
 }
@@ -536,13 +536,13 @@ __global__ __launch_bounds__(256) void k_gather_layout_boxes(const float4* __res
 __global__ void k_gather_layout(const float4* __restrict__ pos, const float4* __restrict__ flux,
                                 const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sorted,
                                 int64_t n, float4* __restrict__ A, float4* __restrict__ B,
-                                float4* __restrict__ Cc) {
+                                float4* __restrict__ Cc, int stride) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t j = sorted ? sorted[k] : (uint32_t)k;
     const uint32_t src = perm ? perm[j] : j;
-    const float4 p = pos[src];
-    const float4 f = flux[src];
+    const float4 p = pos[(int64_t)src * stride];
+    const float4 f = flux[(int64_t)src * stride];
     if (!Cc) {  // compact: the records as stored
         A[k] = p;
         B[k] = f;
@@ -632,14 +632,15 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 }
 
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
-                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C, float4* boxes) {
+                          const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C, float4* boxes,
+                          int stride) {
     if (n <= 0) return;
     if (boxes && !C) {
         const int64_t threads = gather_box_count(n) * GATHER_BLOCK;
         hipLaunchKernelGGL(k_gather_layout_boxes, dim3(grid1(threads, 256)), dim3(256), 0, s, pos, flux, perm,
-                           sorted, n, A, B, boxes);
+                           sorted, n, A, B, boxes, stride);
         return;
     }
     hipLaunchKernelGGL(k_gather_layout, dim3(grid1(n, 256)), dim3(256), 0, s, pos, flux, perm, sorted, n, A, B,
-                       C);
+                       C, stride);
 }

@@ -167,7 +167,8 @@ __global__ void k_origin_fill(int64_t n, uint32_t sentinel, uint32_t* __restrict
 __global__ __launch_bounds__(256) void k_origin_point_keys(
     const float4* __restrict__ pos, const float4* __restrict__ flux, const uint8_t* __restrict__ rank,
     const uint32_t* __restrict__ start, int64_t n, const unsigned long long* __restrict__ count, GridParams g,
-    uint32_t sentinel, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ perm) {
+    uint32_t sentinel, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ perm,
+    int stride) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     if (count && j >= (int64_t)min((unsigned long long)n, *count)) {
@@ -176,10 +177,10 @@ __global__ __launch_bounds__(256) void k_origin_point_keys(
         perm[j] = (uint32_t)j;
         return;
     }
-    const uint32_t i = __float_as_uint(flux[j].w) >> 6;
+    const uint32_t i = __float_as_uint(flux[j * stride].w) >> 6;
     const int64_t p = (int64_t)start[i] + rank[j];
     if (p >= n) return;
-    const float4 q = pos[j];
+    const float4 q = pos[j * stride];
     keys[p] = point_key(q.x, q.y, q.z, g);
     vals[p] = (uint32_t)j;
     perm[p] = (uint32_t)j;
@@ -387,12 +388,12 @@ void launch_origin_starts(hipStream_t s, const uint32_t* count, int64_t photons,
 
 void launch_origin_point_keys(hipStream_t s, const float4* pos, const float4* flux, const uint8_t* rank,
                               const uint32_t* start, int64_t n, const unsigned long long* count, GridParams g,
-                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm) {
+                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm, int stride) {
     if (n <= 0) return;
     if (count)
         hipLaunchKernelGGL(k_origin_fill, dim3(grid1(n, 256)), dim3(256), 0, s, n, sentinel, keys, vals, perm);
     hipLaunchKernelGGL(k_origin_point_keys, dim3(grid1(n, 256)), dim3(256), 0, s, pos, flux, rank, start, n, count,
-                       g, sentinel, keys, vals, perm);
+                       g, sentinel, keys, vals, perm, stride);
 }
 
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,

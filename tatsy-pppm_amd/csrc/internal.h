@@ -65,7 +65,7 @@ void launch_origin_starts(hipStream_t s, const uint32_t* count, int64_t photons,
 // slots and positions keep their place with key `sentinel`)
 void launch_origin_point_keys(hipStream_t s, const float4* pos, const float4* flux, const uint8_t* rank,
                               const uint32_t* start, int64_t n, const unsigned long long* count, GridParams g,
-                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm);
+                              uint32_t sentinel, uint32_t* keys, uint32_t* vals, uint32_t* perm, int stride = 1);
 // off[h] = number of sorted keys whose bucket (key >> shift) is < h, for h = 0..size
 // (scratch: at least (size + 1) / 4096 + 1 free words, or null)
 void launch_bucket_offsets(hipStream_t s, const uint32_t* sorted_keys, int64_t n, uint32_t size,
@@ -133,8 +133,10 @@ struct PhotonPassArgs {
     uint32_t scene_id;
     int32_t max_bounces;
     float ray_eps;
+    // record slot j at ph_pos[j * rec_stride], ph_flux[j * rec_stride]
     float4* ph_pos;   // xyz, oct normal bits
     float4* ph_flux;  // rgb, origin key bits (photon << 6 | bounce)
+    int32_t rec_stride = 1;
     Counters* counters;
     const PassDev* pd = nullptr;  // when set, the pass index is read from it
     PropDev prop;     // emission proposal (side 0: plain cosine emission)
@@ -240,7 +242,7 @@ inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GAT
 #define GATHER_PAD 4
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
                           const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C,
-                          float4* boxes = nullptr);
+                          float4* boxes = nullptr, int stride = 1);
 
 // origin keys of the stored photons, and the permutation applied to the records;
 // with a device count, slots i >= min(*count, n) get key `sentinel`
@@ -252,7 +254,7 @@ void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_
 void launch_pass_advance(hipStream_t s, PassDev* pd, const double* radii);
 void launch_capacity_check(hipStream_t s, PassDev* pd, const Counters* counters, int64_t cap);
 void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,
-                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out);
+                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out, int stride = 1);
 void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,
                     uint32_t* out);
 void launch_zero_u32(hipStream_t s, void* p, int64_t words);
@@ -321,8 +323,9 @@ struct AdamArgs {
     int32_t apply;
 };
 struct ContribArgs {
-    const float4* ph_pos;    // photon records as appended
+    const float4* ph_pos;    // photon records as appended (slot j at [j * rec_stride])
     const float4* ph_flux;
+    int32_t rec_stride;
     const uint32_t* perm;    // origin order
     int64_t n;
     const uint32_t* used;    // per record slot: a hit point's gather accepted it

@@ -420,8 +420,8 @@ __global__ void k_prop_table(const float* prob, int32_t anchors, float* cdf) {
     c[PROP_CELLS - 1] = 1.0f;
 }
 
-__device__ __forceinline__ uint32_t photon_of(const float4* ph_flux, const uint32_t* perm, int64_t k) {
-    return __float_as_uint(ph_flux[perm[k]].w) >> 6;
+__device__ __forceinline__ uint32_t photon_of(const ContribArgs& a, int64_t k) {
+    return __float_as_uint(a.ph_flux[(int64_t)a.perm[k] * a.rec_stride].w) >> 6;
 }
 
 // At the first record of each photon (records in origin order): the summed
@@ -431,12 +431,12 @@ __global__ __launch_bounds__(256) void k_contrib(ContribArgs a) {
 #pragma clang fp contract(off)  // the sum is restated on the CPU bit for bit
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= a.n) return;
-    const uint32_t ph = photon_of(a.ph_flux, a.perm, k);
-    const bool first = k == 0 || photon_of(a.ph_flux, a.perm, k - 1) != ph;
+    const uint32_t ph = photon_of(a, k);
+    const bool first = k == 0 || photon_of(a, k - 1) != ph;
     float c = 0.0f;
     if (first) {
         for (int64_t j = k; j < a.n; j++) {
-            const float4 f = a.ph_flux[a.perm[j]];
+            const float4 f = a.ph_flux[(int64_t)a.perm[j] * a.rec_stride];
             if ((__float_as_uint(f.w) >> 6) != ph) break;
             if (a.used[a.perm[j]]) c += ((f.x + f.y) + f.z) * (1.0f / 3.0f);
         }
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_contrib_write(ContribArgs a) {
     if (!(c > 0.0f)) return;
     const int64_t pos = (int64_t)a.scan[k];
     if (pos < total - a.ring_cap) return;
-    const uint32_t ph = photon_of(a.ph_flux, a.perm, k);
+    const uint32_t ph = photon_of(a, k);
     Rng rng;
     rng.init(a.seed, KIND_PHOTON, a.pass, ph, a.scene_id);
     const float pu = rng.at(1), pv = rng.at(2);

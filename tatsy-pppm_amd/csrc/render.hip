@@ -178,8 +178,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     if (m.type & PPPM_BSDF_LAMBERTIAN) {
                         unsigned long long slot = wave_append(&a.counters->photons_stored);
                         if (slot < (unsigned long long)a.cap) {
-                            a.ph_pos[slot] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
-                            a.ph_flux[slot] = make_float4(flux.x, flux.y, flux.z,
+                            a.ph_pos[slot * a.rec_stride] = make_float4(hp.x, hp.y, hp.z, __uint_as_float(oct_encode(on)));
+                            a.ph_flux[slot * a.rec_stride] = make_float4(flux.x, flux.y, flux.z,
                                                           __uint_as_float((i << 6) | (uint32_t)bounce));
                             if (a.rec_rank) a.rec_rank[slot] = (uint8_t)stored;
                         }
@@ -410,13 +410,13 @@ __global__ void k_capacity_check(PassDev* pd, const Counters* c, int64_t cap) {
 
 __global__ void k_permute2(const float4* __restrict__ a_in, const float4* __restrict__ b_in,
                            const uint32_t* __restrict__ perm, const uint32_t* __restrict__ perm2,
-                           int64_t n, float4* __restrict__ a_out, float4* __restrict__ b_out) {
+                           int64_t n, float4* __restrict__ a_out, float4* __restrict__ b_out, int stride) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t j = perm2 ? perm2[k] : (uint32_t)k;
     uint32_t src = perm ? perm[j] : j;
-    a_out[k] = a_in[src];
-    b_out[k] = b_in[src];
+    a_out[k] = a_in[(int64_t)src * stride];
+    b_out[k] = b_in[(int64_t)src * stride];
 }
 
 __global__ void k_compose(const uint32_t* perm1, const uint32_t* perm2, int64_t n, uint32_t* out) {
@@ -486,10 +486,10 @@ void launch_capacity_check(hipStream_t s, PassDev* pd, const Counters* counters,
 }
 
 void launch_permute2(hipStream_t s, const float4* a_in, const float4* b_in, const uint32_t* perm,
-                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out) {
+                     const uint32_t* perm2, int64_t n, float4* a_out, float4* b_out, int stride) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_permute2, dim3(grid1(n, 256)), dim3(256), 0, s, a_in, b_in, perm, perm2, n,
-                       a_out, b_out);
+                       a_out, b_out, stride);
 }
 
 void launch_compose(hipStream_t s, const uint32_t* perm1, const uint32_t* perm2, int64_t n,
