@@ -106,7 +106,7 @@ typedef struct {
     uint32_t scene_id;     /* RNG stream id of this scene */
     int32_t hash_size;     /* 0: the number of stored photons of each pass */
     double capacity_factor;/* photon buffer = photons * factor records (0: 8) */
-    int32_t gather_mode;   /* PPPM_GATHER_TILED (default) or PPPM_GATHER_LANES */
+    int32_t gather_mode;   /* PPPM_GATHER_AUTO (0, the default: resolves to LANES), LANES, TILED or WAVE */
     int32_t reserved;
 } pppm_render_params;
 
@@ -268,7 +268,7 @@ int pppm_dipole_eval(void* ctx, const pppm_bsdf* material, const float* pts, con
 
 /* ---- learned emission proposal.  No reference counterpart: it replaces the
  * cosine emission of sources/sampler.cc:17-32 (the light contract of
- * sources/envmap.cc:62-86, used by sources/ppm_probabilistic.cc:166-187) by
+ * sources/envmap.cc:62-86, used by sources/ppm_probabilistic.cc:136-146) by
  * the defensive mixture  p(w | x) = lambda cos(theta) / pi
  * + (1 - lambda) q_a(x)[k(w)] / (2 pi / 1024)  over a 32 x 32 equal-area grid
  * of directions about the light normal (cell k = kz * 32 + kphi with

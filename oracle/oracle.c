@@ -18,10 +18,19 @@
  * total_reflection              sources/reflectance.h:7-34                 compiled reference (_ref); Fresnel 0.04
  * bsdf_sample                   sources/brdf.cc:23-26, 49-52, 117-142      compiled reference (_ref)
  * camera_ray                    sources/perspective_camera.cc:42-46        centre/corner closed forms
- * orc_trace_photons             sources/ppm_probabilistic.cc:166-251       flux conservation, mean-store closed form
- * orc_eye_pass                  sources/ppm_probabilistic.cc:253-387       mirror-chain / direct-emission closed form
- * orc_gather                    sources/ppm_probabilistic.cc:343-377       brute force; uniform-density closed form
- * orc_render                    sources/ppm_probabilistic.cc:67-164        furnace-style energy check; convergence
+ * orc_trace_photons             sources/ppm_probabilistic.cc:123-208       first-deposit flux = pi Le A / N exactly;
+ *                                                                          mean stores 1 / (1 - rho) in a closed box
+ * orc_eye_pass (eye_path)       sources/ppm_probabilistic.cc:210-343       mirror-corridor closed form: hit point by
+ *                                                                          unfolding, weight prod(rho) / pi / roulettes,
+ *                                                                          emitter seen through mirrors
+ * orc_gather                    sources/ppm_probabilistic.cc:300-334       brute force; uniform-density closed form
+ * orc_render / orc_render_ex    sources/ppm_probabilistic.cc:24-121        sphere furnace L = rho Phi / (pi A (1 - rho));
+ *                                                                          the independent path tracer on the box scene
+ * orc_render_ex (subsurface)    ppm_probabilistic.cc:272-298;              translucent plane under a parallel light:
+ *                               subsurface_integrator.cc:234-238, 329-365  rho F_r Le + F_t (1 - F_dr) Le I_Rd / pi
+ * orc_pathtrace                 none (next-event path tracer; the          furnace closed form; agreement with the
+ *                               reference's path_tracing.cc:136-198        photon mapping render
+ *                               cannot see area lights)
  * orc_dipole / orc_dipole_sum   sources/bssrdf.cc:33-104;                  compiled reference DipoleBSSRDF (_ref golden,
  *                               subsurface_integrator.cc:147-167, 234-238  fast exp); closed forms of R_d at d = 0 and
  *                                                                          of its clamped area integral
@@ -538,10 +547,10 @@ static v3 on_hemisphere(v3 w, double r1, double r2) {
 }
 
 /* reflectance.h:7-34 */
-static int total_reflection(int into, v3 in, v3 n, v3 on, v3* refl, v3* refr, double* fre,
+static int total_reflection(int into, v3 in, v3 n, v3 on, double ior, v3* refl, v3* refr, double* fre,
                             double* ftr) {
     *refl = reflect(in, n);
-    double nnt = into ? IOR_VACCUM / IOR_OBJECT : IOR_OBJECT / IOR_VACCUM;
+    double nnt = into ? IOR_VACCUM / ior : ior / IOR_VACCUM;
     double ddn = dot(in, on);
     double cos2t = 1.0 - nnt * nnt * (1.0 - ddn * ddn);
     if (cos2t < 0.0) {
@@ -551,7 +560,7 @@ static int total_reflection(int into, v3 in, v3 n, v3 on, v3* refl, v3* refr, do
         return 1;
     }
     *refr = normalized(sub(scl(in, nnt), scl(n, (into ? 1.0 : -1.0) * (ddn * nnt + sqrt(cos2t)))));
-    double a = IOR_OBJECT - IOR_VACCUM, b = IOR_OBJECT + IOR_VACCUM;
+    double a = ior - IOR_VACCUM, b = ior + IOR_VACCUM;
     double R0 = (a * a) / (b * b);
     double c = 1.0 - (into ? -ddn : dot(*refr, neg(on)));
     *fre = R0 + (1.0 - R0) * (c * c * c * c * c);
@@ -576,7 +585,9 @@ static v3 bsdf_sample(const orc_bsdf* b, v3 in, v3 n, double r1, double r2, doub
     if (dot(in, n) >= 0.0) { into = 0; on = neg(n); }
     v3 rd, td;
     double fre, ftr;
-    if (total_reflection(into, in, n, on, &rd, &td, &fre, &ftr)) {
+    /* the material's index (IOR_OBJECT, common.h:16, when it names none) */
+    double ior = b->ior > 0.0f ? (double)b->ior : IOR_OBJECT;
+    if (total_reflection(into, in, n, on, ior, &rd, &td, &fre, &ftr)) {
         *pdf = 1.0;
         return rd;
     }
@@ -597,7 +608,7 @@ static inline v3 spawn(v3 p, v3 ng, v3 dir, double eps) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Photon pass (ppm_probabilistic.cc:166-251, area-light emission)           */
+/* Photon pass (ppm_probabilistic.cc:123-208, area-light emission)           */
 /* ------------------------------------------------------------------------ */
 
 typedef struct { float p[3], f[3], n[3]; int32_t bounce; } deposit;
@@ -676,14 +687,21 @@ static void emit_photon(const orc_scene* s, const orc_proposal* pr, rng_t* r, in
     *flux = V(L->emission[0] * k, L->emission[1] * k, L->emission[2] * k);
 }
 
+/* One photon path (ppm_probabilistic.cc:132-193).  Lambertian hits store a
+ * deposit in out[] (up to cap_out, the count is returned whatever the cap);
+ * with irr != NULL a hit on a BSSRDF surface arriving from outside also stores
+ * an irradiance deposit (the photons of the subsurface pre-pass,
+ * subsurface_integrator.cc:240-327, taken from this pass's own photons) in
+ * irr[] (count in *n_irr), and the path goes on through the dielectric
+ * boundary. */
 static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, const orc_proposal* pr,
-                            int64_t idx, deposit* out, int cap_out) {
+                            int64_t idx, deposit* out, int cap_out, deposit* irr, int cap_irr, int* n_irr) {
     rng_t r;
     rng_init(&r, P->seed, KIND_PHOTON, (uint32_t)P->pass_index, (uint32_t)idx, P->scene_id);
     v3 pos, dir, flux, nl;
     emit_photon(s, pr, &r, P->photons, &pos, &dir, &flux, &nl);
     v3 o = add(pos, scl(nl, P->ray_eps));
-    int nd = 0;
+    int nd = 0, ni = 0;
     for (int bounce = 0;; bounce++) {
         if (bounce >= P->max_bounces || vmax3(flux) <= 0.0) break;
         double t;
@@ -696,6 +714,16 @@ static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, const 
         v3 ng = tri_normal(s, tri);
         v3 on = dot(ng, dir) < 0.0 ? ng : neg(ng);
         if (b->type & ORC_EMITTER) break;
+        if (irr && (b->type & ORC_BSSRDF) && dot(ng, dir) < 0.0) {
+            if (ni < cap_irr) {
+                deposit* d = &irr[ni];
+                d->p[0] = (float)hp.x; d->p[1] = (float)hp.y; d->p[2] = (float)hp.z;
+                d->f[0] = (float)flux.x; d->f[1] = (float)flux.y; d->f[2] = (float)flux.z;
+                d->n[0] = (float)on.x; d->n[1] = (float)on.y; d->n[2] = (float)on.z;
+                d->bounce = bounce;
+            }
+            ni++;
+        }
         if (b->type & ORC_LAMBERTIAN) {
             if (nd < cap_out) {
                 deposit* d = &out[nd];
@@ -719,63 +747,120 @@ static int trace_one_photon(const orc_scene* s, const orc_pass_params* P, const 
         } else {
             double pdf = 1.0;
             /* the unoriented normal, as the eye pass passes it (ppm_probabilistic.cc:338);
-             * the photon pass's oriented normal (:188) would make every glass hit "into" */
+             * the photon pass's oriented normal (:184) would make every glass hit "into" */
             v3 nd_dir = bsdf_sample(b, dir, ng, r0, r1, &pdf);
             o = spawn(hp, ng, nd_dir, P->ray_eps);
             dir = nd_dir;
             flux = scl(mul(flux, refl_of(b)), 1.0 / pdf);
         }
     }
+    if (n_irr) *n_irr = ni;
     return nd;
 }
 
 #define MAX_DEP 64
+#define CACHE_MAIN 4
+#define CACHE_IRR 2
+
+/* All deposits of a pass in (photon index, bounce) order -- the order the
+ * sort of the photon map leaves them in on the GPU -- as two malloc'd lists
+ * (main: Lambertian stores; irr: BSSRDF-surface stores, only when want_irr).
+ * Photons with more deposits than the per-photon cache are traced again (the
+ * path is a function of its index). */
+static int64_t trace_pass(const orc_scene* s, const orc_pass_params* P, const orc_proposal* pr, int want_irr,
+                          deposit** main_out, int32_t** main_photon, deposit** irr_out, int32_t** irr_photon,
+                          int64_t* n_irr_out) {
+    int64_t N = P->photons;
+    int32_t* cm = (int32_t*)calloc((size_t)(N + 1), sizeof(int32_t));
+    int32_t* ci = (int32_t*)calloc((size_t)(N + 1), sizeof(int32_t));
+    deposit* tm = (deposit*)malloc(sizeof(deposit) * (size_t)(N ? N : 1) * CACHE_MAIN);
+    deposit* ti = want_irr ? (deposit*)malloc(sizeof(deposit) * (size_t)(N ? N : 1) * CACHE_IRR) : NULL;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < N; i++) {
+        deposit buf[MAX_DEP], ib[MAX_DEP];
+        int ni = 0;
+        int nd = trace_one_photon(s, P, pr, i, buf, MAX_DEP, want_irr ? ib : NULL, MAX_DEP, &ni);
+        cm[i + 1] = nd;
+        ci[i + 1] = ni;
+        for (int k = 0; k < nd && k < CACHE_MAIN; k++) tm[i * CACHE_MAIN + k] = buf[k];
+        for (int k = 0; k < ni && k < CACHE_IRR; k++) ti[i * CACHE_IRR + k] = ib[k];
+    }
+    int64_t* om = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
+    int64_t* oi = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
+    om[0] = 0;
+    oi[0] = 0;
+    for (int64_t i = 0; i < N; i++) {
+        om[i + 1] = om[i] + (cm[i + 1] < MAX_DEP ? cm[i + 1] : MAX_DEP);
+        oi[i + 1] = oi[i] + (ci[i + 1] < MAX_DEP ? ci[i + 1] : MAX_DEP);
+    }
+    deposit* dm = (deposit*)malloc(sizeof(deposit) * (size_t)(om[N] ? om[N] : 1));
+    int32_t* pm = (int32_t*)malloc(sizeof(int32_t) * (size_t)(om[N] ? om[N] : 1));
+    deposit* di = (deposit*)malloc(sizeof(deposit) * (size_t)(oi[N] ? oi[N] : 1));
+    int32_t* pi = (int32_t*)malloc(sizeof(int32_t) * (size_t)(oi[N] ? oi[N] : 1));
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < N; i++) {
+        int nd = cm[i + 1], ni = ci[i + 1];
+        deposit buf[MAX_DEP], ib[MAX_DEP];
+        const deposit* sm = &tm[i * CACHE_MAIN];
+        const deposit* si = ti ? &ti[i * CACHE_IRR] : NULL;
+        if (nd > CACHE_MAIN || ni > CACHE_IRR) {
+            int dummy = 0;
+            trace_one_photon(s, P, pr, i, buf, MAX_DEP, want_irr ? ib : NULL, MAX_DEP, &dummy);
+            sm = buf;
+            si = ib;
+        }
+        for (int k = 0; k < nd && k < MAX_DEP; k++) { dm[om[i] + k] = sm[k]; pm[om[i] + k] = (int32_t)i; }
+        for (int k = 0; k < ni && k < MAX_DEP; k++) { di[oi[i] + k] = si[k]; pi[oi[i] + k] = (int32_t)i; }
+    }
+    int64_t total = om[N];
+    *n_irr_out = oi[N];
+    *main_out = dm;
+    *main_photon = pm;
+    *irr_out = di;
+    *irr_photon = pi;
+    free(om); free(oi); free(tm); free(ti); free(cm); free(ci);
+    return total;
+}
 
 int64_t orc_trace_photons(const orc_scene* s, const orc_pass_params* P, float* pos, float* flux,
                           float* nrm, uint32_t* key, int64_t cap) {
     return orc_trace_photons_prop(s, P, NULL, pos, flux, nrm, key, cap);
 }
 
+static void copy_deposits(const deposit* d, const int32_t* ph, int64_t n, int64_t cap, float* pos, float* flux,
+                          float* nrm, uint32_t* key) {
+    for (int64_t j = 0; j < n && j < cap; j++) {
+        memcpy(pos + 3 * j, d[j].p, 12);
+        memcpy(flux + 3 * j, d[j].f, 12);
+        memcpy(nrm + 3 * j, d[j].n, 12);
+        if (key) key[j] = ((uint32_t)ph[j] << 6) | (uint32_t)d[j].bounce;
+    }
+}
+
 int64_t orc_trace_photons_prop(const orc_scene* s, const orc_pass_params* P, const orc_proposal* pr,
                                float* pos, float* flux, float* nrm, uint32_t* key, int64_t cap) {
-    int64_t N = P->photons;
-    int32_t* counts = (int32_t*)calloc((size_t)(N + 1), sizeof(int32_t));
-    deposit* tmp = (deposit*)malloc(sizeof(deposit) * (size_t)(N ? N : 1) * 4);
-    int64_t tmp_cap = N * 4;
-    /* first pass: count and keep up to 4 deposits per photon; retrace the rest */
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t i = 0; i < N; i++) {
-        deposit buf[MAX_DEP];
-        int nd = trace_one_photon(s, P, pr, i, buf, MAX_DEP);
-        counts[i + 1] = nd;
-        for (int k = 0; k < nd && k < 4; k++) tmp[i * 4 + k] = buf[k];
-    }
-    (void)tmp_cap;
-    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N + 1));
-    off[0] = 0;
-    for (int64_t i = 0; i < N; i++) off[i + 1] = off[i] + counts[i + 1];
-    int64_t total = off[N];
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t i = 0; i < N; i++) {
-        int nd = counts[i + 1];
-        deposit buf[MAX_DEP];
-        const deposit* src = &tmp[i * 4];
-        if (nd > 4) { trace_one_photon(s, P, pr, i, buf, MAX_DEP); src = buf; }
-        for (int k = 0; k < nd && k < MAX_DEP; k++) {
-            int64_t j = off[i] + k;
-            if (j >= cap) break;
-            memcpy(pos + 3 * j, src[k].p, 12);
-            memcpy(flux + 3 * j, src[k].f, 12);
-            memcpy(nrm + 3 * j, src[k].n, 12);
-            if (key) key[j] = ((uint32_t)i << 6) | (uint32_t)src[k].bounce;
-        }
-    }
-    free(off); free(tmp); free(counts);
+    deposit *dm, *di;
+    int32_t *pm, *pi;
+    int64_t ni;
+    int64_t total = trace_pass(s, P, pr, 0, &dm, &pm, &di, &pi, &ni);
+    copy_deposits(dm, pm, total, cap, pos, flux, nrm, key);
+    free(dm); free(pm); free(di); free(pi);
     return total;
 }
 
+int64_t orc_trace_irradiance_photons(const orc_scene* s, const orc_pass_params* P, float* pos, float* flux,
+                                     float* nrm, uint32_t* key, int64_t cap) {
+    deposit *dm, *di;
+    int32_t *pm, *pi;
+    int64_t ni;
+    trace_pass(s, P, NULL, 1, &dm, &pm, &di, &pi, &ni);
+    copy_deposits(di, pi, ni, cap, pos, flux, nrm, key);
+    free(dm); free(pm); free(di); free(pi);
+    return ni;
+}
+
 /* ------------------------------------------------------------------------ */
-/* Eye pass (ppm_probabilistic.cc:253-344)                                   */
+/* Eye pass (ppm_probabilistic.cc:210-343)                                   */
 /* ------------------------------------------------------------------------ */
 
 /* perspective_camera.cc:42-46 with an orthonormal (U, V) basis and the field
@@ -793,9 +878,17 @@ static void camera_ray(const orc_camera* c, double px, double py_up, v3* o, v3* 
     *d = normalized(dir);
 }
 
-/* Returns 1 with a hit point, 0 with a direct contribution in *direct. */
+/* Outcomes of an eye path */
+#define EYE_NONE 0   /* black environment, killed, or an emitter seen (*direct) */
+#define EYE_HIT 1    /* hit point on a Lambertian surface */
+#define EYE_SSS 2    /* transmitted into a BSSRDF body: the dipole estimate at *hpos */
+
+/* ppm_probabilistic.cc:237-343 (executePathTracing, radiance) in the P-KZ
+ * form.  EYE_HIT: *hw = throughput * rho / (pi * roulette).  EYE_SSS:
+ * *hw = throughput * F_t / (P_t * roulette) (the transmitted branch of
+ * :272-290, divided by the roulette) and *mat = the material. */
 static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P,
-                    int64_t pix, v3* hpos, v3* hnrm, v3* hw, v3* direct) {
+                    int64_t pix, v3* hpos, v3* hnrm, v3* hw, v3* direct, int32_t* mat) {
     int W = cam->width, H = cam->height;
     int x = (int)(pix % W), row = (int)(pix / W);
     rng_t r;
@@ -809,7 +902,7 @@ static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_pa
     for (int bounce = 0;; bounce++) {
         double t;
         int32_t tri = bounce >= P->max_bounces ? -1 : scene_intersect(s, o, d, &t);
-        if (tri < 0) return 0; /* black environment */
+        if (tri < 0) return EYE_NONE; /* black environment */
         int dd = EYE_DIM_BOUNCE + 3 * bounce;
         double r0 = rng_at(&r, dd), r1 = rng_at(&r, dd + 1), r2 = rng_at(&r, dd + 2);
         const orc_bsdf* b = &s->bsdf[s->mat[tri]];
@@ -818,12 +911,12 @@ static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_pa
         if (b->type & ORC_EMITTER) {
             if (dot(d, ng) < 0.0)
                 *direct = mul(thr, V(b->emission[0], b->emission[1], b->emission[2]));
-            return 0;
+            return EYE_NONE;
         }
         v3 rho = refl_of(b);
         double roulette = vmax3(rho);
         if (bounce >= 3) {
-            if (r0 > roulette) return 0;
+            if (r0 > roulette) return EYE_NONE;
         } else {
             roulette = 1.0;
         }
@@ -832,10 +925,17 @@ static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_pa
             *hpos = hp;
             *hnrm = on;
             *hw = scl(mul(thr, rho), 1.0 / (PI * roulette));
-            return 1;
+            return EYE_HIT;
         }
         double pdf = 1.0;
         v3 nd = bsdf_sample(b, d, ng, r1, r2, &pdf);
+        if ((b->type & ORC_BSSRDF) && mat && ((dot(nd, ng) < 0.0) == (dot(d, ng) < 0.0))) {
+            *hpos = hp;
+            *hnrm = on;
+            *hw = scl(thr, 1.0 / (pdf * roulette));
+            *mat = s->mat[tri];
+            return EYE_SSS;
+        }
         thr = scl(mul(thr, rho), 1.0 / (pdf * roulette));
         o = spawn(hp, ng, nd, P->ray_eps);
         d = nd;
@@ -849,8 +949,9 @@ void orc_eye_pass(const orc_scene* s, const orc_camera* cam, const orc_pass_para
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < np; i++) {
         v3 hp = V(0, 0, 0), hn = V(0, 0, 0), hw = V(0, 0, 0), dr;
-        int h = eye_path(s, cam, P, i, &hp, &hn, &hw, &dr);
+        int h = eye_path(s, cam, P, i, &hp, &hn, &hw, &dr, NULL) == EYE_HIT;
         has_hit[i] = h;
+        if (!h) hp = hn = hw = V(0, 0, 0);
         hit_pos[3 * i] = (float)hp.x; hit_pos[3 * i + 1] = (float)hp.y; hit_pos[3 * i + 2] = (float)hp.z;
         hit_nrm[3 * i] = (float)hn.x; hit_nrm[3 * i + 1] = (float)hn.y; hit_nrm[3 * i + 2] = (float)hn.z;
         hit_w[3 * i] = (float)hw.x; hit_w[3 * i + 1] = (float)hw.y; hit_w[3 * i + 2] = (float)hw.z;
@@ -859,7 +960,7 @@ void orc_eye_pass(const orc_scene* s, const orc_camera* cam, const orc_pass_para
 }
 
 /* ------------------------------------------------------------------------ */
-/* Gather (P-KZ form of ppm_probabilistic.cc:343-377)                        */
+/* Gather (P-KZ form of ppm_probabilistic.cc:300-334)                        */
 /* ------------------------------------------------------------------------ */
 
 #define CONE_K 1.1
@@ -937,21 +1038,39 @@ void orc_gather(const float* ph_pos, const float* ph_flux, const float* ph_nrm, 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Pass loop (ppm_probabilistic.cc:67-164)                                   */
+/* Pass loop (ppm_probabilistic.cc:24-121)                                   */
 /* ------------------------------------------------------------------------ */
 
-int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
-                   int32_t first_pass, double r1, double alpha, double ray_eps, uint64_t seed,
-                   uint32_t scene_id, int32_t hash_size, double* image, int64_t* stats) {
+/* (1 - F_dr) / pi * area * sum_j R_d(|x - p_j|^2) E_j for one point x and
+ * material b (subsurface_integrator.cc:147-167 without the octree cut, and
+ * :234-238), FP64 with the true exponential */
+static void dipole_point(const orc_bsdf* b, const float* pts, const double* E, int64_t n_pts, double area, v3 x,
+                         double out[3]) {
+    double sa[3] = {b->sigma_a[0], b->sigma_a[1], b->sigma_a[2]};
+    double ss[3] = {b->sigmap_s[0], b->sigmap_s[1], b->sigmap_s[2]};
+    double f = (1.0 - orc_dipole_fdr(b->eta)) / PI * area;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < n_pts; j++) {
+        double dx = x.x - pts[3 * j], dy = x.y - pts[3 * j + 1], dz = x.z - pts[3 * j + 2];
+        double rd[3];
+        orc_dipole(sa, ss, b->eta, dx * dx + dy * dy + dz * dz, 0, rd);
+        for (int c = 0; c < 3; c++) acc[c] += rd[c] * E[3 * j + c];
+    }
+    for (int c = 0; c < 3; c++) out[c] = f * acc[c];
+}
+
+int32_t orc_render_ex(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
+                      int32_t first_pass, double r1, double alpha, double ray_eps, uint64_t seed,
+                      uint32_t scene_id, int32_t hash_size, const orc_render_opts* opts, double* image,
+                      int64_t* stats) {
     int64_t np = (int64_t)cam->width * cam->height;
+    const orc_proposal* pr = opts ? opts->prop : NULL;
+    const int sss = opts && opts->sss_n > 0 && opts->sss_pts;
     double* radii = (double*)malloc(sizeof(double) * (size_t)(first_pass + n_passes));
     orc_radius_schedule(r1, alpha, first_pass + n_passes, 0, radii);
     double bmin[3] = {s->bmin.x - r1, s->bmin.y - r1, s->bmin.z - r1};
-    int64_t cap = photons * 8 + 1024;
-    float* pp = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
-    float* pf = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
-    float* pn = (float*)malloc(sizeof(float) * 3 * (size_t)cap);
-    int32_t* has = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+    int32_t* kind = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+    int32_t* smat = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
     float* hp = (float*)malloc(sizeof(float) * 3 * (size_t)np);
     float* hn = (float*)malloc(sizeof(float) * 3 * (size_t)np);
     float* hw = (float*)malloc(sizeof(float) * 3 * (size_t)np);
@@ -962,7 +1081,13 @@ int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, i
     float* cn = (float*)malloc(sizeof(float) * 3 * (size_t)np);
     float* cw = (float*)malloc(sizeof(float) * 3 * (size_t)np);
     double* sum = (double*)calloc(3 * (size_t)np, sizeof(double));
-    int64_t tot_dep = 0, tot_hit = 0;
+    double* E = sss ? (double*)malloc(sizeof(double) * 3 * (size_t)opts->sss_n) : NULL;
+    float* ptw = NULL;
+    if (sss) {
+        ptw = (float*)malloc(sizeof(float) * 3 * (size_t)opts->sss_n);
+        for (int64_t j = 0; j < 3 * opts->sss_n; j++) ptw[j] = (float)(1.0 / PI);
+    }
+    int64_t tot_dep = 0, tot_hit = 0, tot_irr = 0, tot_sss = 0;
     int32_t rc = 0;
     for (int32_t k = 0; k < n_passes; k++) {
         int32_t pass = first_pass + k;
@@ -975,13 +1100,47 @@ int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, i
         P.ray_eps = ray_eps;
         P.seed = seed;
         P.scene_id = scene_id;
-        int64_t nd = orc_trace_photons(s, &P, pp, pf, pn, NULL, cap);
-        if (nd > cap) { rc = -1; break; }
+        deposit *dm, *di;
+        int32_t *pm, *pi;
+        int64_t ni;
+        int64_t nd = trace_pass(s, &P, pr, sss, &dm, &pm, &di, &pi, &ni);
         tot_dep += nd;
-        orc_eye_pass(s, cam, &P, has, hp, hn, hw, dr);
+        tot_irr += ni;
+        float* pp = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+        float* pf = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+        float* pn = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+        copy_deposits(dm, pm, nd, nd, pp, pf, pn, NULL);
+        if (sss) {
+            /* irradiance at the surface points: the fixed-radius gather of the
+             * irradiance photons with weight 1 / pi (subsurface_integrator.cc:329-365) */
+            if (ni > 0) {
+                float* ip = (float*)malloc(sizeof(float) * 3 * (size_t)ni);
+                float* iff = (float*)malloc(sizeof(float) * 3 * (size_t)ni);
+                float* in_ = (float*)malloc(sizeof(float) * 3 * (size_t)ni);
+                copy_deposits(di, pi, ni, ni, ip, iff, in_, NULL);
+                int32_t Si = hash_size > 0 ? hash_size : (int32_t)ni;
+                orc_gather(ip, iff, in_, ni, opts->sss_pts, opts->sss_nrm, ptw, opts->sss_n, P.radius, Si, bmin, E,
+                           NULL);
+                free(ip); free(iff); free(in_);
+            } else {
+                memset(E, 0, sizeof(double) * 3 * (size_t)opts->sss_n);
+            }
+        }
+        free(dm); free(pm); free(di); free(pi);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < np; i++) {
+            v3 a = V(0, 0, 0), n_ = V(0, 0, 0), w = V(0, 0, 0), d;
+            int32_t m = -1;
+            kind[i] = eye_path(s, cam, &P, i, &a, &n_, &w, &d, sss ? &m : NULL);
+            smat[i] = m;
+            hp[3 * i] = (float)a.x; hp[3 * i + 1] = (float)a.y; hp[3 * i + 2] = (float)a.z;
+            hn[3 * i] = (float)n_.x; hn[3 * i + 1] = (float)n_.y; hn[3 * i + 2] = (float)n_.z;
+            hw[3 * i] = (float)w.x; hw[3 * i + 1] = (float)w.y; hw[3 * i + 2] = (float)w.z;
+            dr[3 * i] = d.x; dr[3 * i + 1] = d.y; dr[3 * i + 2] = d.z;
+        }
         int64_t nh = 0;
         for (int64_t i = 0; i < np; i++)
-            if (has[i]) {
+            if (kind[i] == EYE_HIT) {
                 hidx[nh] = i;
                 memcpy(cp + 3 * nh, hp + 3 * i, 12);
                 memcpy(cn + 3 * nh, hn + 3 * i, 12);
@@ -995,12 +1154,148 @@ int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, i
             for (int c = 0; c < 3; c++) sum[3 * i + c] += dr[3 * i + c];
         for (int64_t j = 0; j < nh; j++)
             for (int c = 0; c < 3; c++) sum[3 * hidx[j] + c] += g[3 * j + c];
+        if (sss) {
+            int64_t ns = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : ns)
+            for (int64_t i = 0; i < np; i++) {
+                if (kind[i] != EYE_SSS) continue;
+                ns++;
+                double L[3];
+                dipole_point(&s->bsdf[smat[i]], opts->sss_pts, E, opts->sss_n, opts->sss_area,
+                             V(hp[3 * i], hp[3 * i + 1], hp[3 * i + 2]), L);
+                for (int c = 0; c < 3; c++) sum[3 * i + c] += (double)hw[3 * i + c] * L[c];
+            }
+            tot_sss += ns;
+        }
+        free(pp); free(pf); free(pn);
     }
     for (int64_t i = 0; i < 3 * np; i++) image[i] = sum[i] / (double)n_passes;
-    if (stats) { stats[0] = tot_dep; stats[1] = tot_hit; }
-    free(radii); free(pp); free(pf); free(pn); free(has); free(hp); free(hn); free(hw);
-    free(dr); free(g); free(hidx); free(cp); free(cn); free(cw); free(sum);
+    if (stats) { stats[0] = tot_dep; stats[1] = tot_hit; stats[2] = tot_irr; stats[3] = tot_sss; }
+    free(radii); free(kind); free(smat); free(hp); free(hn); free(hw);
+    free(dr); free(g); free(hidx); free(cp); free(cn); free(cw); free(sum); free(E); free(ptw);
     return rc;
+}
+
+int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
+                   int32_t first_pass, double r1, double alpha, double ray_eps, uint64_t seed,
+                   uint32_t scene_id, int32_t hash_size, double* image, int64_t* stats) {
+    int64_t st[4];
+    int32_t rc = orc_render_ex(s, cam, photons, n_passes, first_pass, r1, alpha, ray_eps, seed, scene_id,
+                               hash_size, NULL, image, st);
+    if (stats) { stats[0] = st[0]; stats[1] = st[1]; }
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Independent FP64 path tracer with next-event estimation                   */
+/* ------------------------------------------------------------------------ */
+
+#define KIND_PT 3u
+/* dims: 0-1 jitter; bounce b uses 2+6b .. 7+6b: roulette, direction (2),
+ * light pick, light point (2) */
+#define PT_DIM_BOUNCE 2
+
+/* L at the camera through each pixel: emission seen directly or through
+ * specular chains, and at every diffuse vertex the direct light by sampling
+ * a point on a light (shadow ray) plus the indirect light by continuing the
+ * path with a cosine-distributed direction.  An emitter reached from a
+ * diffuse vertex adds nothing (its light was counted by the light sample).
+ * Consistent with the photon mapping estimate in the limit of zero radius;
+ * shares only the scene, intersection and the pinned BSDF sampling with it.
+ * (The reference's own PathTracing, path_tracing.cc:136-198, sees only the
+ * environment map, so it cannot render an area-lit scene.) */
+static v3 pt_path(const orc_scene* s, const orc_camera* cam, int64_t pix, int32_t sample, int32_t max_bounces,
+                  double ray_eps, uint64_t seed, uint32_t scene_id) {
+    int W = cam->width, H = cam->height;
+    int x = (int)(pix % W), row = (int)(pix / W);
+    rng_t r;
+    rng_init(&r, seed, KIND_PT, (uint32_t)sample, (uint32_t)pix, scene_id);
+    v3 o, d;
+    camera_ray(cam, x + rng_at(&r, 0), (H - 1 - row) + rng_at(&r, 1), &o, &d);
+    v3 thr = V(1, 1, 1), L = V(0, 0, 0);
+    int specular_prev = 1;
+    for (int bounce = 0; bounce < max_bounces; bounce++) {
+        double t;
+        int32_t tri = scene_intersect(s, o, d, &t);
+        if (tri < 0) break;
+        const orc_bsdf* b = &s->bsdf[s->mat[tri]];
+        v3 hp = add(o, scl(d, t));
+        v3 ng = tri_normal(s, tri);
+        if (b->type & ORC_EMITTER) {
+            if (specular_prev && dot(d, ng) < 0.0)
+                L = add(L, mul(thr, V(b->emission[0], b->emission[1], b->emission[2])));
+            break;
+        }
+        int dd = PT_DIM_BOUNCE + 6 * bounce;
+        v3 rho = refl_of(b);
+        if (bounce >= 3) {
+            double q = vmax3(rho);
+            if (rng_at(&r, dd) >= q) break;
+            thr = scl(thr, 1.0 / q);
+        }
+        v3 on = dot(ng, d) < 0.0 ? ng : neg(ng);
+        if (b->type & ORC_LAMBERTIAN) {
+            if (s->nl > 0) {
+                double target = rng_at(&r, dd + 3) * s->light_total;
+                int l = 0;
+                while (l < s->nl - 1 && s->light_cdf[l + 1] <= target) l++;
+                const orc_light* Lt = &s->lights[l];
+                v3 c = V(Lt->corner[0], Lt->corner[1], Lt->corner[2]);
+                v3 eu = V(Lt->edge_u[0], Lt->edge_u[1], Lt->edge_u[2]);
+                v3 ev = V(Lt->edge_v[0], Lt->edge_v[1], Lt->edge_v[2]);
+                v3 y = add(c, add(scl(eu, rng_at(&r, dd + 4)), scl(ev, rng_at(&r, dd + 5))));
+                v3 cr = cross(eu, ev);
+                double area = norm(cr);
+                v3 nl = scl(cr, 1.0 / area);
+                double pl = (s->light_cdf[l + 1] - s->light_cdf[l]) / s->light_total;
+                v3 w = sub(y, hp);
+                double dist2 = dot(w, w), dist = sqrt(dist2);
+                w = scl(w, 1.0 / dist);
+                double cx = dot(on, w), cl = -dot(nl, w);
+                if (cx > 0.0 && cl > 0.0) {
+                    v3 so = spawn(hp, ng, w, ray_eps);
+                    v3 ws = sub(y, so);
+                    double ds = norm(ws);
+                    double ts;
+                    int32_t occ = scene_intersect(s, so, scl(ws, 1.0 / ds), &ts);
+                    int visible = occ < 0 || ts >= ds * (1.0 - 1e-9) - 1e-9 ||
+                                  (s->bsdf[s->mat[occ]].type & ORC_EMITTER);
+                    if (visible) {
+                        double gterm = cx * cl / dist2 * area / pl / PI;
+                        L = add(L, scl(mul(mul(thr, rho), V(Lt->emission[0], Lt->emission[1], Lt->emission[2])),
+                                       gterm));
+                    }
+                }
+            }
+            v3 nd = on_hemisphere(on, rng_at(&r, dd + 1), rng_at(&r, dd + 2));
+            thr = mul(thr, rho);
+            o = spawn(hp, ng, nd, ray_eps);
+            d = nd;
+            specular_prev = 0;
+        } else {
+            double pdf = 1.0;
+            v3 nd = bsdf_sample(b, d, ng, rng_at(&r, dd + 1), rng_at(&r, dd + 2), &pdf);
+            thr = scl(mul(thr, rho), 1.0 / pdf);
+            o = spawn(hp, ng, nd, ray_eps);
+            d = nd;
+            specular_prev = 1;
+        }
+    }
+    return L;
+}
+
+void orc_pathtrace(const orc_scene* s, const orc_camera* cam, int32_t spp, int32_t first_sample,
+                   int32_t max_bounces, double ray_eps, uint64_t seed, uint32_t scene_id, double* image) {
+    int64_t np = (int64_t)cam->width * cam->height;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < np; i++) {
+        v3 acc = V(0, 0, 0);
+        for (int32_t k = 0; k < spp; k++)
+            acc = add(acc, pt_path(s, cam, i, first_sample + k, max_bounces, ray_eps, seed, scene_id));
+        image[3 * i] = acc.x / spp;
+        image[3 * i + 1] = acc.y / spp;
+        image[3 * i + 2] = acc.z / spp;
+    }
 }
 
 int32_t orc_num_threads(void) {
@@ -1025,7 +1320,7 @@ int32_t orc_total_reflection(int32_t into, const double in[3], const double n[3]
                              double refl[3], double refr[3], double* fre, double* ftr) {
     v3 rd, td;
     int r = total_reflection(into, V(in[0], in[1], in[2]), V(n[0], n[1], n[2]),
-                             V(on[0], on[1], on[2]), &rd, &td, fre, ftr);
+                             V(on[0], on[1], on[2]), IOR_OBJECT, &rd, &td, fre, ftr);
     refl[0] = rd.x; refl[1] = rd.y; refl[2] = rd.z;
     refr[0] = td.x; refr[1] = td.y; refr[2] = td.z;
     return r;

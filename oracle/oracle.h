@@ -159,13 +159,44 @@ void orc_gather(const float* ph_pos, const float* ph_flux, const float* ph_nrm, 
                 double radius, int32_t hash_size, const double bbox_min[3],
                 double* out_rgb, int64_t* counters);
 
-/* Full P-KZ render: passes 1..n_passes, radius r1 shrinking by alpha, FP64
- * image (top row first) averaged over passes.  stats (may be NULL):
- * [0] total deposits, [1] total hit points. */
+/* Full P-KZ render: passes first_pass..first_pass+n_passes-1, radius r1
+ * shrinking by alpha, FP64 image (top row first) averaged over passes.
+ * stats (may be NULL): [0] total deposits, [1] total hit points. */
 int32_t orc_render(const orc_scene* s, const orc_camera* cam, int64_t photons_per_pass,
                    int32_t n_passes, int32_t first_pass, double r1, double alpha,
                    double ray_eps, uint64_t seed, uint32_t scene_id, int32_t hash_size,
                    double* image, int64_t* stats);
+
+/* Options of orc_render_ex (NULL: plain emission, no subsurface). */
+typedef struct {
+    const orc_proposal* prop;  /* learned-proposal emission (NULL: cosine emission) */
+    const float* sss_pts;      /* [n][3] surface points of the BSSRDF materials (NULL: no subsurface) */
+    const float* sss_nrm;      /* [n][3] their outward normals */
+    int64_t sss_n;
+    double sss_area;           /* surface area each point stands for */
+} orc_render_opts;
+
+/* orc_render with a proposal and/or the subsurface estimate: every pass also
+ * stores the photons reaching BSSRDF surfaces from outside, gathers their
+ * irradiance E_j at the points (weight 1 / pi), and an eye path transmitted
+ * into a BSSRDF body adds W (1 - F_dr) / pi * area * sum_j R_d(|x - p_j|^2) E_j.
+ * stats (may be NULL): [0] deposits, [1] hit points, [2] irradiance photons,
+ * [3] transmitted eye paths. */
+int32_t orc_render_ex(const orc_scene* s, const orc_camera* cam, int64_t photons_per_pass,
+                      int32_t n_passes, int32_t first_pass, double r1, double alpha,
+                      double ray_eps, uint64_t seed, uint32_t scene_id, int32_t hash_size,
+                      const orc_render_opts* opts, double* image, int64_t* stats);
+
+/* Irradiance photons of a pass (BSSRDF surfaces, arriving from outside), in
+ * (photon index, bounce) order; returns the count (only cap written). */
+int64_t orc_trace_irradiance_photons(const orc_scene* s, const orc_pass_params* p, float* pos, float* flux,
+                                     float* normal, uint32_t* key, int64_t cap);
+
+/* Independent FP64 path tracer with next-event estimation to the area
+ * lights: image [height][width][3] (top row first) = mean over samples
+ * first_sample .. first_sample + spp - 1 of each pixel's radiance. */
+void orc_pathtrace(const orc_scene* s, const orc_camera* cam, int32_t spp, int32_t first_sample,
+                   int32_t max_bounces, double ray_eps, uint64_t seed, uint32_t scene_id, double* image);
 
 /* Single-sample pieces, exported to pin them against the compiled reference:
  * sampler.cc:17-32, reflectance.h:7-34, brdf.cc, perspective_camera.cc:42-46,

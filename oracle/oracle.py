@@ -30,6 +30,11 @@ class Proposal(C.Structure):
     _fields_ = [("cdf", C.c_void_p), ("lam", C.c_double), ("side", C.c_int32), ("pad", C.c_int32)]
 
 
+class RenderOpts(C.Structure):
+    _fields_ = [("prop", C.c_void_p), ("sss_pts", C.c_void_p), ("sss_nrm", C.c_void_p), ("sss_n", C.c_int64),
+                ("sss_area", C.c_double)]
+
+
 def proposal(cdf, side, lam):
     """orc_proposal over a [side^2][1024] float32 CDF array (kept alive on the struct)."""
     cdf = np.ascontiguousarray(cdf, np.float32).reshape(side * side, 1024)
@@ -60,6 +65,10 @@ _ODECL = {
     "orc_gather": (None, [_P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double, C.c_int32, _P, _P, _P]),
     "orc_render": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                C.c_double, C.c_uint64, C.c_uint32, C.c_int32, _P, _P]),
+    "orc_render_ex": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                  C.c_double, C.c_uint64, C.c_uint32, C.c_int32, _P, _P, _P]),
+    "orc_trace_irradiance_photons": (C.c_int64, [_P, C.POINTER(PassParams), _P, _P, _P, _P, C.c_int64]),
+    "orc_pathtrace": (None, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_uint32, _P]),
     "orc_num_threads": (C.c_int32, []),
     "orc_on_hemisphere": (None, [_P, C.c_double, C.c_double, _P]),
     "orc_total_reflection": (C.c_int32, [C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
@@ -190,13 +199,46 @@ class OracleScene:
         return has, hp, hn, hw, dr
 
     def render(self, cam, photons, passes, r1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
-               hash_size=0, first_pass=1):
+               hash_size=0, first_pass=1, prop=None, sss_points=None):
+        """P-KZ render.  prop: a Proposal (fixed tables) for the emission;
+        sss_points: (pos [n][3], normals [n][3], area per point) of the BSSRDF
+        surfaces for the subsurface estimate.  Returns (image, stats) with
+        stats = deposits, hit points, irradiance photons, transmitted eye paths."""
         img = np.zeros((cam.height, cam.width, 3), np.float64)
-        stats = np.zeros(2, np.int64)
-        rc = orc().orc_render(self.h, C.byref(cam), photons, passes, first_pass, r1, alpha, ray_eps,
-                              seed, scene_id, hash_size, _ptr(img), _ptr(stats))
+        stats = np.zeros(4, np.int64)
+        opts = RenderOpts()
+        keep = []
+        if prop is not None:
+            opts.prop = C.cast(C.pointer(prop), C.c_void_p)
+            keep.append(prop)
+        if sss_points is not None:
+            pts = np.ascontiguousarray(sss_points[0], np.float32).reshape(-1, 3)
+            nrm = np.ascontiguousarray(sss_points[1], np.float32).reshape(-1, 3)
+            keep += [pts, nrm]
+            opts.sss_pts, opts.sss_nrm = pts.ctypes.data, nrm.ctypes.data
+            opts.sss_n, opts.sss_area = len(pts), float(sss_points[2])
+        rc = orc().orc_render_ex(self.h, C.byref(cam), photons, passes, first_pass, r1, alpha, ray_eps,
+                                 seed, scene_id, hash_size, C.byref(opts), _ptr(img), _ptr(stats))
         assert rc == 0
         return img, stats
+
+    def pathtrace(self, cam, spp, first_sample=0, max_bounces=64, ray_eps=1e-4, seed=0x7A75, scene_id=0):
+        """The independent next-event path tracer: mean radiance per pixel."""
+        img = np.zeros((cam.height, cam.width, 3), np.float64)
+        orc().orc_pathtrace(self.h, C.byref(cam), spp, first_sample, max_bounces, ray_eps, seed, scene_id,
+                            _ptr(img))
+        return img
+
+    def trace_irradiance_photons(self, photons, pass_index, ray_eps, seed, scene_id=0, cap=None):
+        cap = cap or photons * 8 + 1024
+        pos = np.zeros((cap, 3), np.float32)
+        flux = np.zeros((cap, 3), np.float32)
+        nrm = np.zeros((cap, 3), np.float32)
+        key = np.zeros(cap, np.uint32)
+        p = self._params(photons, pass_index, 0.0, ray_eps, seed, scene_id)
+        n = orc().orc_trace_irradiance_photons(self.h, C.byref(p), _ptr(pos), _ptr(flux), _ptr(nrm), _ptr(key), cap)
+        assert n <= cap
+        return pos[:n], flux[:n], nrm[:n], key[:n]
 
 
 def hashgrid_points(pos, hash_size, hash_scale, bbox_min):

@@ -1,7 +1,7 @@
 // proposal.hip -- learned photon-emission proposal on gfx950.
 //
 // No reference counterpart: it replaces the cosine emission of
-// sources/sampler.cc:17-32 (called by ppm_probabilistic.cc:166-187 through
+// sources/sampler.cc:17-32 (called by ppm_probabilistic.cc:136-146 through
 // the light contract of sources/envmap.cc:62-86) with the mixture of
 // pppm_device.h, whose pdf is exact, so the estimator stays unbiased.
 //

@@ -1,7 +1,7 @@
 // render.hip -- the probabilistic PPM pass on gfx950: photon pass, eye pass,
 // fixed-radius gather with Kahan accumulation, and the small kernels between.
 //
-// Restates sources/ppm_probabilistic.cc:166-387 (tracePhotons, traceRays,
+// Restates sources/ppm_probabilistic.cc:123-343 (tracePhotons, traceRays,
 // radiance) in the P-KZ form: area-light emission, photons stored at
 // Lambertian hits with Russian roulette by mean albedo, mirror / dielectric
 // continuation, eye Russian roulette from bounce 3, one hit point per pixel
