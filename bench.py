@@ -61,13 +61,36 @@ def read_traffic(name="gather_pmc_caustic_box_1024.txt"):
 
 
 def cpu_model():
+    """Model name, family / model / stepping, and the logical CPUs of the host."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
     except OSError:
         pass
-    return platform.processor()
+    name = info.get("model name", platform.processor())
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}), {os.cpu_count()} logical CPUs")
+
+
+def smt_state():
+    """SMT as the kernel reports it (on / off / notsupported), with threads per core."""
+    state = "unknown"
+    try:
+        state = open("/sys/devices/system/cpu/smt/control").read().strip()
+    except OSError:
+        pass
+    try:
+        sib = open("/sys/devices/system/cpu/cpu0/topology/thread_siblings_list").read().strip()
+        n = sum(len(range(int(a), int(b) + 1)) if "-" in r else 1
+                for r in sib.split(",") for a, _, b in [r.partition("-")])
+        state += f", {n} threads per core"
+    except (OSError, ValueError):
+        pass
+    return state
 
 
 def cpu_baseline(scene, photons, r1, ray_eps, seed):
@@ -85,7 +108,9 @@ def cpu_baseline(scene, photons, r1, ray_eps, seed):
         "unit": "photons/s",
         "cores": int(O.orc().orc_num_threads()),
         "kind": "port",
-        "sample": (f"one full pass of the same workload on the host CPU ({cpu_model()}): "
+        "cpu": cpu_model(),
+        "smt": smt_state(),
+        "sample": (f"one full pass of the same workload on the host CPU ({cpu_model()}, SMT {smt_state()}): "
                    f"{photons} photons, {scene.camera.width}x{scene.camera.height} eye paths, "
                    f"FP64 OpenMP oracle; pass {t_pass:.2f} s, QBVH build {t_build:.2f} s excluded"),
     }, img
@@ -275,15 +300,7 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
     8 photons per pixel, in one hipGraph-captured pass loop after the
     autotuner; against the same scenes rendered one after another with
     pppm_render_pass."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for k in range(n_scenes):
-        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-        res = int(rng.choice([128, 256, 512, 1024]))
-        kind = P.SCENE_RANDOM if k % 2 else P.SCENE_BOX
-        per = 20.0 if kind == P.SCENE_RANDOM else 40.0   # 20 f^2 triangles per icosphere
-        f = max(1, int(round((tris / per) ** 0.5)))
-        specs.append((kind, 1000 + k, f, res))
+    specs = P.batch_specs(n_scenes, seed)
     t0 = time.perf_counter()
     b = P.Batch(branches=branches)
     scenes = []
@@ -422,12 +439,16 @@ def main():
     segs = sum(s["photon_segments"] + s["eye_segments"] for s in stats)
     gather_ms = sum(s["gather_ms"] for s in stats) / args.steps
     gbytes = sum(gather_bytes(s) for s in stats) / args.steps
-    achieved = gbytes / (gather_ms * 1e-3) / 1e9
+    ref_cell_rate = gbytes / (gather_ms * 1e-3) / 1e9
     photon_kernel_ms = sum(s["photon_ms"] for s in stats) / args.steps
     photon_rays = sum(s["photon_segments"] for s in stats) / args.steps
     trav_bytes = traversal_bytes(photon_rays, nodes_per_ray, tris_per_ray)
     trav_gbps = trav_bytes / (photon_kernel_ms * 1e-3) / 1e9
     useful = sum(useful_bytes(s) for s in stats) / args.steps
+    achieved = useful / (gather_ms * 1e-3) / 1e9
+    tested = sum(s["gather_tested"] for s in stats) / args.steps
+    hits = sum(s["hitpoints"] for s in stats) / args.steps
+    read_bytes = hits * (HIT_RECORD_BYTES + ACCUM_BYTES) + PHOTON_BYTES * tested
     out = {
         "metric": METRIC,
         "value": photons_per_s,
@@ -452,22 +473,30 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_gather",
+            "kernel": "k_gather_lanes",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": read_traffic(),
-            "algorithmic_bytes_per_launch": gbytes,
+            "algorithmic_bytes_per_launch": useful,
+            "bytes_formula": "H (40 + 48) + 24 per in-radius photon: hit point record, Kahan pair read and "
+                             "written, position and flux of every photon the estimate sums",
             "avg_launch_ms": gather_ms,
             "note": "timed region runs passes pipelined: the gather shares the GPU with the next pass's photons",
-            "isolated": {"achieved": iso_bytes / (iso_ms * 1e-3) / 1e9, "avg_launch_ms": iso_ms,
-                         "frac": iso_bytes / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
-            "useful": {"bytes_per_launch": useful, "achieved": useful / (gather_ms * 1e-3) / 1e9,
-                       "isolated_achieved": iso_useful / (iso_ms * 1e-3) / 1e9,
-                       "note": "in-radius photons only: H (40 + 48) + 24 per accepted photon"},
-            "records_tested_per_launch": sum(s["gather_tested"] for s in stats) / args.steps,
+            "isolated": {"achieved": iso_useful / (iso_ms * 1e-3) / 1e9, "avg_launch_ms": iso_ms,
+                         "frac": iso_useful / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
+            "records_read_rate": {"bytes_per_launch": read_bytes,
+                                  "achieved": read_bytes / (gather_ms * 1e-3) / 1e9,
+                                  "note": "H (40 + 48) + 24 per photon record the kernel distance-tests"},
+            "reference_cell_equivalent": {
+                "bytes_per_launch": gbytes, "rate": ref_cell_rate,
+                "isolated_rate": iso_bytes / (iso_ms * 1e-3) / 1e9,
+                "note": "H (40 + 48) + 8 per bucket + 24 per record of the buckets of the reference's 2r "
+                        "cells, whether read or not; a work-equivalent rate, not bandwidth"},
+            "records_tested_per_launch": tested,
             "records_in_buckets_per_launch": sum(s["gather_candidates"] for s in stats) / args.steps,
+            "records_accepted_per_launch": sum(s["gather_inradius"] for s in stats) / args.steps,
         },
         "traversal": {
             "kernel": "k_photon",
@@ -490,6 +519,7 @@ def main():
         "rays_per_s": segs / elapsed,
         "photon_kernel_photons_per_s": args.photons / (photon_kernel_ms * 1e-3),
         "gather_gbps": achieved,
+        "gather_reference_cell_equivalent_gbps": ref_cell_rate,
         "stage_ms": {k: sum(s[k] for s in stats) / args.steps
                      for k in ("photon_ms", "order_ms", "grid_ms", "eye_ms", "hitsort_ms", "gather_ms",
                                "total_ms")},

@@ -169,6 +169,25 @@ int pppm_bvh_build(void* ctx, pppm_bvh_info* info);
  * QBVHAccel::intersect (sources/qbvh_accel.cc:205-285). */
 int pppm_intersect(void* ctx, const pppm_ray* rays, int64_t n, pppm_hit* hits);
 
+/* One 4-wide BVH node as the GPU builds it (128 bytes): the four child boxes
+ * in SoA FP32 (an empty slot has lo > hi), then four child refs: ref >= 0 is
+ * an inner node index, ref < 0 a leaf whose ~ref = (first << 3) | (count - 1)
+ * indexes the leaf-ordered triangles, 0x7FFFFFFF an empty slot. */
+typedef struct {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+} pppm_bvh_node;
+
+/* Host copy of the built BVH, for scalar re-entrant traversal on the CPU
+ * (the reference's QBVHAccel::intersect is const and called concurrently,
+ * sources/qbvh_accel.cc:205-285, ppm_probabilistic.cc:157, 256): nodes
+ * [n_nodes], leaf-ordered triangles tris [n_tris][12] (v0 xyz + original
+ * triangle index as float bits, v1 xyz + 0, v2 xyz + 0) and the root ref.
+ * With null buffers only the counts are returned. */
+int pppm_bvh_download(void* ctx, pppm_bvh_node* nodes, int64_t node_cap, float* tris, int64_t tri_cap,
+                      int64_t* n_nodes, int64_t* n_tris, int32_t* root);
+
 /* ---- hash grid: replaces HashGrid<Ty> (sources/hash_grid.h:8-39,
  * sources/hash_grid_detail.h:26-72) ---- */
 /* Point insertion add(i, p_i, p_i), i = 0..n-1: sorted_idx[cell_start[h] ..
@@ -233,6 +252,11 @@ int pppm_gather(void* ctx, const float* ph_pos, const float* ph_flux, const floa
  * reference counterpart): the known-traffic kernel that scales profiler
  * FETCH_SIZE / WRITE_SIZE counters, and a measured copy bandwidth. */
 int pppm_stream_copy(void* ctx, int64_t bytes, double* ms);
+/* Accumulator test hook (no reference counterpart): lane k (< lanes <= 64)
+ * adds values[(i + k) % n], i = 0 .. n-1, with the image's FP32 compensated
+ * (Kahan) addition; sum[k] and comp[k] receive the pair, whose value is
+ * sum - comp (as pppm_image_read reads it). */
+int pppm_kahan_probe(void* ctx, const float* values, int64_t n, int32_t lanes, float* sum, float* comp);
 /* Issue-rate calibration (no reference counterpart): cycles per wave64
  * instruction on one SIMD at the reported clock for a full-rate v_fma_f32
  * and for v_exp_f32, measured with independent chains; the roofline of the
@@ -339,6 +363,21 @@ int pppm_batch_run(void* batch, int32_t passes, double* ms);
 int pppm_batch_image(void* batch, int32_t index, double* rgb, int64_t n_values);
 int pppm_batch_scene_info(void* batch, int32_t index, int32_t* gather_mode, int32_t* photon_grid, int32_t* branch,
                           double* cost_ms);
+
+/* ---- mesh files: replace Trimesh::load (sources/trimesh.cc:125-259) ---- */
+typedef struct {
+    int64_t n_vertices;
+    float* vertices;  /* [n_vertices][3] */
+    int64_t n_faces;
+    int32_t* faces;   /* [n_faces][3] vertex indices, 0-based */
+} pppm_mesh;
+/* Reads a .ply (binary little / big endian or ASCII; x, y, z among any vertex
+ * properties; face lists of any integer types, polygons fan-triangulated) or
+ * .obj ("v" and "f" lines; 1-based, negative and slash-separated indices)
+ * into host memory the library allocates; on failure err (may be NULL) gets
+ * a message.  No device is touched. */
+int pppm_mesh_load(const char* path, pppm_mesh* out, char* err, int32_t err_len);
+void pppm_mesh_free(pppm_mesh* mesh);
 
 /* ---- procedural scenes (no reference counterpart: the reference loads PLY
  * meshes, sources/trimesh.cc:125-217) ---- */

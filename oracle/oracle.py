@@ -88,6 +88,7 @@ _RDECL = {
     "ref_bsdf_sample": (None, [C.c_int32, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
     "ref_camera_ray": (None, [_P, _P, _P, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P]),
     "ref_dipole": (None, [_P, _P, C.c_double, C.c_double, _P, _P]),
+    "ref_trimesh_place": (C.c_int64, [C.c_char_p, C.c_int32, _P, C.c_double, C.c_double, _P, _P, _P, C.c_int64]),
 }
 
 

@@ -18,6 +18,8 @@
 #include "reflectance.h"
 #include "sampler.h"
 #include "triangle.h"
+#include "trimesh.h"
+#include "plane.h"
 
 void* enabler = nullptr;  // declared extern in common.h:41
 
@@ -167,6 +169,28 @@ void ref_dipole(const double sigma_a[3], const double sigmap_s[3], double eta, d
     Vector3D v = b(d2);
     out[0] = v.x(); out[1] = v.y(); out[2] = v.z();
     *fdr = b.Fdr();
+}
+
+// Trimesh::load + the placement of sources/main/appmain.cc:34-38 (fitToBBox,
+// scale, putOnPlane, translate); writes the triangles [n][9] in FP64 and
+// returns their count (-1 above cap).  do_place = 0: the triangles as loaded.
+int64_t ref_trimesh_place(const char* path, int32_t do_place, const double box[6], double scale_all,
+                          double plane_d, const double plane_n[3], const double move[3], double* tris,
+                          int64_t cap) {
+    Trimesh m;
+    m.load(path);
+    if (do_place) {
+        m.fitToBBox(BBox(box[0], box[1], box[2], box[3], box[4], box[5]));
+        m.scale(scale_all);
+        m.putOnPlane(Plane(plane_d, Vector3D(plane_n[0], plane_n[1], plane_n[2])));
+        m.translate(Vector3D(move[0], move[1], move[2]));
+    }
+    std::vector<Triangle> t = m.triangulate();
+    if ((int64_t)t.size() > cap) return -1;
+    for (size_t i = 0; i < t.size(); i++)
+        for (int k = 0; k < 3; k++)
+            for (int d = 0; d < 3; d++) tris[9 * i + 3 * k + d] = t[i].p(k)[d];
+    return (int64_t)t.size();
 }
 
 }  // extern "C"

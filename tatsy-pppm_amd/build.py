@@ -27,8 +27,10 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "i
 
 HIP_SOURCES = ["radix.hip", "hash_grid.hip", "bvh.hip", "render.hip", "gather.hip", "proposal.hip", "subsurface.hip",
                "capi.hip", "pass.hip", "sss_host.hip", "proposal_host.hip", "batch.hip"]
-CXX_SOURCES = ["scenegen.cc", "pppm_amd.cc"]
+CXX_SOURCES = ["scenegen.cc", "pppm_amd.cc", "mesh_io.cc"]
 FACADE_TEST = os.path.join(ROOT, "tests", "cpp", "facade_main")
+# C++ programs over the reference-shaped interface, built beside their sources
+CPP_TESTS = ["facade_main", "appmain_shape", "mesh_main"]
 
 
 def _run(cmd):
@@ -62,9 +64,11 @@ def build_lib(jobs=None, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    # the C++ interface driven like the reference's application (a test binary)
-    _run(["g++"] + CXX_FLAGS + [FACADE_TEST + ".cc", "-o", FACADE_TEST, "-L" + HERE, "-lpppm_hip",
-                                "-Wl,-rpath,$ORIGIN/../../tatsy-pppm_amd"])
+    # the C++ interface driven like the reference's application (test binaries)
+    for name in CPP_TESTS:
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        _run(["g++"] + CXX_FLAGS + [exe + ".cc", "-o", exe, "-L" + HERE, "-lpppm_hip", "-pthread",
+                                    "-Wl,-rpath,$ORIGIN/../../tatsy-pppm_amd"])
     return LIB
 
 

@@ -382,6 +382,27 @@ int pppm_intersect(void* ctx_raw, const pppm_ray* rays, int64_t n, pppm_hit* hit
     return PPPM_OK;
 }
 
+int pppm_bvh_download(void* ctx_raw, pppm_bvh_node* nodes, int64_t node_cap, float* tris, int64_t tri_cap,
+                      int64_t* n_nodes, int64_t* n_tris, int32_t* root) {
+    CHECK_CTX(ctx);
+    if (!ctx->has_bvh) return ctx->fail(PPPM_ERR_STATE, "no BVH built");
+    static_assert(sizeof(pppm_bvh_node) == sizeof(Node4), "pppm_bvh_node mirrors the device node");
+    const int64_t nn = ctx->binfo.nodes, nt = ctx->n_tris;
+    if (n_nodes) *n_nodes = nn;
+    if (n_tris) *n_tris = nt;
+    if (root) *root = ctx->root;
+    if ((nodes && node_cap < nn) || (tris && tri_cap < nt))
+        return ctx->fail(PPPM_ERR_CAPACITY, "bvh download: buffers smaller than the tree");
+    (void)hipSetDevice(ctx->device);
+    if (nodes && nn > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(nodes, ctx->nodes, sizeof(Node4) * (size_t)nn, hipMemcpyDeviceToHost, ctx->stream));
+    if (tris && nt > 0)
+        TRY_HIP(ctx, hipMemcpyAsync(tris, ctx->tris4, sizeof(float4) * 3 * (size_t)nt, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PPPM_OK;
+}
+
 int pppm_hashgrid_build(void* ctx_raw, const float* pos, int64_t n, int32_t hash_size,
                         double hash_scale, const double bbox_min[3], int32_t* sorted_idx,
                         int32_t* cell_start, int32_t* cell_end) {
@@ -756,6 +777,29 @@ int pppm_stream_copy(void* ctx_raw, int64_t bytes, double* ms) {
     (void)hipFree(b);
     if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
     if (ms) *ms = t;
+    return PPPM_OK;
+}
+
+int pppm_kahan_probe(void* ctx_raw, const float* values, int64_t n, int32_t lanes, float* sum, float* comp) {
+    CHECK_CTX(ctx);
+    if (!values || n <= 0 || lanes < 1 || lanes > 64 || !sum || !comp)
+        return ctx->fail(PPPM_ERR_INVALID, "kahan probe: n > 0 values and 1..64 lanes");
+    (void)hipSetDevice(ctx->device);
+    void* buf = nullptr;
+    TRY_HIP(ctx, hipMalloc(&buf, sizeof(float) * ((size_t)n + 128)));
+    float* dv = static_cast<float*>(buf);
+    float* ds = dv + n;
+    float* dc = ds + 64;
+    hipError_t e = hipMemcpyAsync(dv, values, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_kahan_probe(ctx->stream, dv, n, lanes, ds, dc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(sum, ds, sizeof(float) * (size_t)lanes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(comp, dc, sizeof(float) * (size_t)lanes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return ctx->fail(PPPM_ERR_HIP, std::string("kahan probe: ") + hipGetErrorString(e));
     return PPPM_OK;
 }
 

@@ -182,6 +182,8 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 // img += the direct emission stored in the pixel slots without hit point
 // (an eye pass run with img_sum = null)
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp);
+// lane k < lanes (<= 64) Kahan-adds v[(i + k) % n], i = 0 .. n-1, into sum[k], comp[k]
+void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, float* sum, float* comp);
 
 #define GATHER_TILED PPPM_GATHER_TILED
 #define GATHER_LANES PPPM_GATHER_LANES

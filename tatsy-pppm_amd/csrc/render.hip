@@ -381,6 +381,16 @@ __global__ void k_direct_add(const float4* __restrict__ hp_w, int64_t np, float*
     }
 }
 
+// Accumulator probe: lane k Kahan-adds v[(i + k) % n] for i = 0 .. n-1 into
+// sum[k] / comp[k] with the image accumulator's own kahan_add.
+__global__ void k_kahan_probe(const float* __restrict__ v, int64_t n, int lanes, float* sum, float* comp) {
+    const int k = threadIdx.x;
+    if (k >= lanes) return;
+    sum[k] = 0.0f;
+    comp[k] = 0.0f;
+    for (int64_t i = 0; i < n; i++) kahan_add(sum, comp, k, v[(i + k) % n]);
+}
+
 // Sort key of a photon record: its photon index.  A photon's records are
 // appended by the lane that traces it, in bounce order, at increasing slots,
 // so the stable sort by index alone orders them by (index, bounce): the
@@ -469,6 +479,10 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp) {
     if (np <= 0) return;
     hipLaunchKernelGGL(k_direct_add, dim3(grid1(np, 256)), dim3(256), 0, s, hp_w, np, img_sum, img_comp);
+}
+
+void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, float* sum, float* comp) {
+    hipLaunchKernelGGL(k_kahan_probe, dim3(1), dim3(64), 0, s, v, n, lanes, sum, comp);
 }
 
 void launch_origin_keys(hipStream_t s, const float4* ph_flux, int64_t n, uint32_t* keys,

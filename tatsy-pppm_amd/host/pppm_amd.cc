@@ -1,6 +1,7 @@
 // pppm_amd.cc -- the reference-shaped C++ classes over the C ABI.
 #include "pppm_amd.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -72,8 +73,25 @@ void BSDF::setBssrdf(const BSSRDF& bssrdf) {
 }
 
 BSDF LambertianBRDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_LAMBERTIAN, r); }
+BSDF PhongBRDF::factory(const Vector3D& r, double n) {
+    (void)n;
+    return make_bsdf(PPPM_BSDF_PHONG, r);
+}
 BSDF SpecularBRDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_SPECULAR, r); }
 BSDF RefractionBSDF::factory(const Vector3D& r) { return make_bsdf(PPPM_BSDF_REFRACTION, r); }
+
+PerspectiveCamera::PerspectiveCamera() { std::memset(&_c, 0, sizeof(_c)); }
+
+Ray PerspectiveCamera::getRay(double pixelX, double pixelY) const {
+    const Vector3D D = direction().normalized();
+    const Vector3D U = Vector3D::cross(D, Vector3D(_c.up[0], _c.up[1], _c.up[2])).normalized();
+    const Vector3D V = Vector3D::cross(U, D);
+    const double th = std::tan(_c.fov_y_deg * 0.5 * M_PI / 180.0);
+    const double aspect = (double)_c.width / (double)_c.height;
+    const Vector3D dir = D + U * (th * aspect * (2.0 * pixelX / _c.width - 1.0)) +
+                         V * (th * (2.0 * pixelY / _c.height - 1.0));
+    return Ray(center(), dir.normalized());
+}
 
 PerspectiveCamera::PerspectiveCamera(const Vector3D& center, const Vector3D& direction, const Vector3D& up,
                                      double fov, int w, int h, double sensitivity)
@@ -157,6 +175,178 @@ static void batch_intersect(void* ctx, const Device& dev, const std::vector<Ray>
     dev.check(pppm_intersect(ctx, r.data(), (int64_t)r.size(), hits->data()));
 }
 
+// ---- Trimesh (sources/trimesh.cc) -------------------------------------------
+Trimesh::Trimesh(const std::vector<Vector3D>& vertices, const std::vector<Triplet>& faces)
+    : _vertices(vertices), _faces(faces) {
+    _normals.resize(_faces.size());
+    for (size_t i = 0; i < _faces.size(); i++) {
+        const Vector3D& v0 = _vertices.at((size_t)_faces[i][0]);
+        _normals[i] = Vector3D::cross(_vertices.at((size_t)_faces[i][1]) - v0, _vertices.at((size_t)_faces[i][2]) - v0)
+                          .normalized();
+    }
+}
+
+void Trimesh::load(const std::string& filename) {
+    pppm_mesh m;
+    char err[512] = {0};
+    const int rc = pppm_mesh_load(filename.c_str(), &m, err, (int32_t)sizeof(err));
+    if (rc != PPPM_OK) throw Error(rc, std::string("Trimesh::load: ") + err);
+    _vertices.resize((size_t)m.n_vertices);
+    for (int64_t i = 0; i < m.n_vertices; i++)
+        _vertices[(size_t)i] = Vector3D(m.vertices[3 * i], m.vertices[3 * i + 1], m.vertices[3 * i + 2]);
+    _faces.resize((size_t)m.n_faces);
+    _normals.resize((size_t)m.n_faces);
+    for (int64_t i = 0; i < m.n_faces; i++) {
+        const int32_t* f = m.faces + 3 * i;
+        _faces[(size_t)i] = Triplet(f[0], f[1], f[2]);
+        _normals[(size_t)i] =
+            Vector3D::cross(_vertices[(size_t)f[1]] - _vertices[(size_t)f[0]], _vertices[(size_t)f[2]] - _vertices[(size_t)f[0]])
+                .normalized();
+    }
+    pppm_mesh_free(&m);
+    _accel.reset();
+}
+
+void Trimesh::translate(const Vector3D& move) {
+    for (Vector3D& v : _vertices) v += move;
+}
+
+void Trimesh::scale(double sx, double sy, double sz) {
+    for (Vector3D& v : _vertices) v = Vector3D(v.x() * sx, v.y() * sy, v.z() * sz);
+}
+
+void Trimesh::scale(double s) { scale(s, s, s); }
+
+void Trimesh::putOnPlane(const Plane& plane) {
+    // the lowest vertex along the normal moves onto the plane (trimesh.cc:283-293)
+    double lowest = 1.0e20;
+    for (const Vector3D& v : _vertices) lowest = std::min(lowest, Vector3D::dot(plane.normal(), v));
+    for (Vector3D& v : _vertices) v -= (lowest + plane.distance()) * plane.normal();
+}
+
+void Trimesh::fitToBBox(const BBox& bbox) {
+    // uniform scale to the tightest axis, then centre to centre (trimesh.cc:295-312)
+    BBox org;
+    for (const Vector3D& v : _vertices) org.merge(v);
+    const Vector3D target = bbox.posMax() - bbox.posMin();
+    const Vector3D size = org.posMax() - org.posMin();
+    const double s = std::min(target.x() / size.x(), std::min(target.y() / size.y(), target.z() / size.z()));
+    scale(s);
+    const Vector3D from = (org.posMin() + org.posMax()) * (0.5 * s);
+    const Vector3D to = (bbox.posMin() + bbox.posMax()) * 0.5;
+    translate(to - from);
+}
+
+Triangle Trimesh::getTriangle(int id) const {
+    const Triplet& f = _faces.at((size_t)id);
+    return Triangle(_vertices[(size_t)f[0]], _vertices[(size_t)f[1]], _vertices[(size_t)f[2]]);
+}
+
+std::vector<Triangle> Trimesh::triangulate() const {
+    std::vector<Triangle> t(_faces.size());
+    for (size_t i = 0; i < _faces.size(); i++) t[i] = getTriangle((int)i);
+    return t;
+}
+
+double Trimesh::area() const {
+    double a = 0.0;
+    for (size_t i = 0; i < _faces.size(); i++) a += getTriangle((int)i).area();
+    return a;
+}
+
+void Trimesh::buildAccel() {
+    _accel = std::make_shared<QBVHAccel>(Device::shared());
+    _accel->construct(triangulate());
+}
+
+bool Trimesh::intersect(const Ray& ray, Hitpoint* hitpoint) const {
+    if (!_accel) throw Error(PPPM_ERR_STATE, "Trimesh::buildAccel not called");
+    hitpoint->setDistance(1.0e20);
+    return _accel->intersect(ray, hitpoint) != -1;
+}
+
+// ---- HostBvh ---------------------------------------------------------------
+void HostBvh::download(const Device& dev) {
+    int64_t nn = 0, nt = 0;
+    int32_t root = 0x7FFFFFFF;
+    dev.check(pppm_bvh_download(dev.handle(), nullptr, 0, nullptr, 0, &nn, &nt, &root));
+    _nodes.resize((size_t)nn);
+    _tris.resize(12 * (size_t)nt);
+    dev.check(pppm_bvh_download(dev.handle(), _nodes.data(), nn, _tris.data(), nt, &nn, &nt, &root));
+    _root = root;
+}
+
+int HostBvh::intersect(const Ray& ray, double tmax, double* t_out) const {
+    const double EPS = 1.0e-12;  // sources/common.h:14
+    if (empty()) return -1;
+    const Vector3D o = ray.origin(), d = ray.direction();
+    float org[3], inv[3];
+    for (int k = 0; k < 3; k++) {
+        org[k] = (float)o[k];
+        inv[k] = d[k] == 0.0 ? 1.0e20f : (float)(1.0 / d[k]);  // qbvh_accel.cc:211-233
+    }
+    double best = tmax;
+    int hit = -1;
+    int32_t stack[256];
+    int sp = 0;
+    stack[sp++] = _root;
+    while (sp > 0) {
+        const int32_t ref = stack[--sp];
+        if (ref >= 0) {
+            const pppm_bvh_node& n = _nodes[(size_t)ref];
+            const float* lo[3] = {n.lox, n.loy, n.loz};
+            const float* hi[3] = {n.hix, n.hiy, n.hiz};
+            float tn[4];
+            int order[4], m = 0;
+            for (int c = 0; c < 4; c++) {
+                if (n.child[c] == 0x7FFFFFFF) continue;
+                float a = 0.0f, b = (float)best;
+                for (int k = 0; k < 3; k++) {
+                    float t0 = (lo[k][c] - org[k]) * inv[k], t1 = (hi[k][c] - org[k]) * inv[k];
+                    a = std::fmax(a, std::fmin(t0, t1));
+                    b = std::fmin(b, std::fmax(t0, t1) * 1.00000024f);  // conservative exit (Ize 2013)
+                }
+                if (a <= b) {
+                    tn[c] = a;
+                    order[m++] = c;
+                }
+            }
+            // push far to near: the nearest child is popped first
+            for (int i = 1; i < m; i++)
+                for (int j = i; j > 0 && tn[order[j]] > tn[order[j - 1]]; j--) std::swap(order[j], order[j - 1]);
+            for (int i = 0; i < m && sp < 256; i++) stack[sp++] = n.child[order[i]];
+        } else {
+            const int32_t e = ~ref;
+            const int32_t first = e >> 3, count = (e & 7) + 1;
+            for (int k = 0; k < count; k++) {
+                const float* t = &_tris[12 * (size_t)(first + k)];
+                const Vector3D p0(t[0], t[1], t[2]), p1(t[4], t[5], t[6]), p2(t[8], t[9], t[10]);
+                // sources/triangle.cc:53-78 (Moller-Trumbore, FP64)
+                const Vector3D e1 = p1 - p0, e2 = p2 - p0;
+                const Vector3D pv = Vector3D::cross(d, e2);
+                const double det = Vector3D::dot(e1, pv);
+                if (det > -EPS && det < EPS) continue;
+                const double invdet = 1.0 / det;
+                const Vector3D tv = o - p0;
+                const double u = Vector3D::dot(tv, pv) * invdet;
+                if (u < 0.0 || u > 1.0) continue;
+                const Vector3D qv = Vector3D::cross(tv, e1);
+                const double v = Vector3D::dot(d, qv) * invdet;
+                if (v < 0.0 || u + v > 1.0) continue;
+                const double tt = Vector3D::dot(e2, qv) * invdet;
+                if (tt > EPS && tt < best) {
+                    best = tt;
+                    int32_t id;
+                    std::memcpy(&id, &t[3], 4);
+                    hit = id;
+                }
+            }
+        }
+    }
+    if (t_out) *t_out = best;
+    return hit;
+}
+
 // ---- QBVHAccel -------------------------------------------------------------
 QBVHAccel::QBVHAccel() : _dev(std::make_shared<Device>(0)) {}
 QBVHAccel::QBVHAccel(std::shared_ptr<Device> dev) : _dev(std::move(dev)) {}
@@ -172,18 +362,19 @@ void QBVHAccel::construct(const std::vector<Triangle>& triangles) {
     l.edge_v[2] = 1.0f;
     _dev->check(pppm_scene_upload(_dev->handle(), v.data(), ids.data(), (int64_t)triangles.size(), &b, 1, &l, 1));
     _dev->check(pppm_bvh_build(_dev->handle(), nullptr));
+    _host.download(*_dev);
     _built = true;
 }
 
 int QBVHAccel::intersect(const Ray& ray, Hitpoint* hitpoint) const {
     if (!_built) throw Error(PPPM_ERR_STATE, "QBVHAccel::construct not called");
-    std::vector<pppm_hit> h;
-    batch_intersect(_dev->handle(), *_dev, std::vector<Ray>(1, ray), &h, (float)hitpoint->distance());
-    if (h[0].tri < 0) return -1;
-    hitpoint->setDistance(h[0].t);
-    hitpoint->setPosition(ray.origin() + h[0].t * ray.direction());
-    hitpoint->setNormal(_tris[h[0].tri].normal());
-    return h[0].tri;
+    double t = 0.0;
+    const int id = _host.intersect(ray, hitpoint->distance(), &t);
+    if (id < 0) return -1;
+    hitpoint->setDistance(t);
+    hitpoint->setPosition(ray.origin() + t * ray.direction());
+    hitpoint->setNormal(_tris[(size_t)id].normal());
+    return id;
 }
 
 void QBVHAccel::intersect(const std::vector<Ray>& rays, std::vector<Intersection>* out) const {
@@ -261,17 +452,23 @@ void Scene::upload() const {
 
 void Scene::setAccelerator() {
     upload();
+    _host.download(*_dev);
     _accel = true;
 }
 
 bool Scene::intersect(const Ray& ray, Intersection& isect) const {
     if (!_accel) throw Error(PPPM_ERR_STATE, "Scene::setAccelerator not called");
-    std::vector<pppm_hit> h;
-    batch_intersect(_dev->handle(), *_dev, std::vector<Ray>(1, ray), &h);
-    std::vector<Intersection> out;
-    fill_hits(h, std::vector<Ray>(1, ray), _triangles, &out);
-    isect = out[0];
-    return h[0].tri >= 0;
+    double t = 0.0;
+    const int id = _host.intersect(ray, 1.0e20, &t);
+    Hitpoint hp;
+    if (id >= 0) {
+        hp.setDistance(t);
+        hp.setPosition(ray.origin() + t * ray.direction());
+        hp.setNormal(_triangles[(size_t)id].normal());
+    }
+    isect.setHitpoint(hp);
+    isect.setObjectId(id);
+    return id >= 0;
 }
 
 BBox Scene::bounds() const {
@@ -284,7 +481,12 @@ BBox Scene::bounds() const {
 const double ProgressivePhotonMappingProb::ALPHA = 0.7;  // ppm_probabilistic.cc:9
 
 void ProgressivePhotonMappingProb::render(const Scene& scene, const Camera& camera,
-                                          const RenderParameters& params) {
+                                          const RenderParameters& params, RandomSamplerType randomSamplerType) {
+    if (randomSamplerType != RANDOM_SAMPLER_PSEUDO_RANDOM)
+        throw Error(PPPM_ERR_INVALID, "only RANDOM_SAMPLER_PSEUDO_RANDOM is provided: the GPU path draws counter-based "
+                                      "Philox numbers and has no permuted Halton sequence");
+    if (camera.width() <= 0 || camera.height() <= 0)
+        throw Error(PPPM_ERR_INVALID, "the camera has no image: assign a PerspectiveCamera with a size");
     std::shared_ptr<Device> dev = scene.device();
     void* ctx = dev->handle();
     scene.upload();  // the scene on this device, BVH included

@@ -138,6 +138,65 @@ public:
     std::vector<Triangle> triangulate() const override { return std::vector<Triangle>(1, *this); }
 };
 
+// ---- sources/plane.h:16-41 ---------------------------------------------------
+// The plane dot(n, x) + distance = 0 (the reference's putOnPlane convention);
+// a placement target, not renderable geometry.
+class Plane {
+    double _distance = 0.0;
+    Vector3D _normal;
+
+public:
+    Plane() {}
+    Plane(double distance, const Vector3D& normal) : _distance(distance), _normal(normal) {}
+    double distance() const { return _distance; }
+    Vector3D normal() const { return _normal; }
+};
+
+// ---- sources/trimesh.h:19-140 ------------------------------------------------
+class Triplet {
+    int _d[3] = {0, 0, 0};
+
+public:
+    Triplet() {}
+    Triplet(int i, int j, int k) : _d{i, j, k} {}
+    int operator[](int i) const { return _d[i]; }
+};
+
+class QBVHAccel;
+
+// Indexed triangle mesh in FP64 with the reference's placement transforms
+// (translate, scale, putOnPlane, fitToBBox: the same arithmetic in the same
+// order, so the same triangles bit for bit).  load() reads .ply / .obj
+// through pppm_mesh_load.
+class Trimesh : public IGeometry {
+    std::vector<Vector3D> _vertices;
+    std::vector<Vector3D> _normals;  // per face
+    std::vector<Triplet> _faces;
+    std::shared_ptr<QBVHAccel> _accel;
+
+public:
+    Trimesh() {}
+    explicit Trimesh(const std::string& filename) { load(filename); }
+    Trimesh(const std::vector<Vector3D>& vertices, const std::vector<Triplet>& faces);
+    void load(const std::string& filename);
+    void translate(const Vector3D& move);
+    void scale(double scaleX, double scaleY, double scaleZ);
+    void scale(double scaleAll);
+    void putOnPlane(const Plane& plane);
+    void fitToBBox(const class BBox& bbox);
+    std::vector<Triangle> triangulate() const override;
+    double area() const;
+    size_t numVerts() const { return _vertices.size(); }
+    size_t numFaces() const { return _faces.size(); }
+    std::vector<Triplet> getIndices() const { return _faces; }
+    Vector3D getVertex(int id) const { return _vertices.at((size_t)id); }
+    Vector3D getNormal(int id) const { return _normals.at((size_t)id); }
+    Triangle getTriangle(int id) const;
+    // GPU BVH over the mesh alone, and a ray against it (trimesh.cc:89-94, 117-129)
+    void buildAccel();
+    bool intersect(const Ray& ray, Hitpoint* hitpoint) const;
+};
+
 // ---- sources/bsdf.h:19-59, sources/brdf.h:37-101 ---------------------------
 enum BsdfType {
     BSDF_TYPE_NONE = 0x00,
@@ -185,6 +244,11 @@ public:
 struct LambertianBRDF {
     static BSDF factory(const Vector3D& reflectance);
 };
+// sources/brdf.h:75-101.  Phong lobes have no GPU kernel (no workload uses
+// them): a scene holding one is rejected by Scene::setAccelerator.
+struct PhongBRDF {
+    static BSDF factory(const Vector3D& reflectance, double n = 128.0);
+};
 struct SpecularBRDF {
     static BSDF factory(const Vector3D& reflectance);
 };
@@ -192,23 +256,52 @@ struct RefractionBSDF {
     static BSDF factory(const Vector3D& reflectance);
 };
 
-// ---- sources/perspective_camera.h:24-31 ------------------------------------
-// fov is the vertical field of view in DEGREES (the unit the reference's own
-// caller passes, sources/main/appmain.cc:123; its implementation reads
-// radians, sources/perspective_camera.cc:18).
-class PerspectiveCamera {
-    pppm_camera _c;
-    double _sensitivity;
+// ---- sources/size.h ---------------------------------------------------------
+template <class T>
+class Size {
+    T _w = T(), _h = T();
 
 public:
+    Size() {}
+    Size(T w, T h) : _w(w), _h(h) {}
+    T width() const { return _w; }
+    T height() const { return _h; }
+};
+
+// ---- sources/perspective_camera.h:16-31, camera_interface.h:12-81 -----------
+// fov is the vertical field of view in DEGREES (the unit the reference's own
+// caller passes, sources/main/appmain.cc:123; its implementation reads
+// radians, sources/perspective_camera.cc:18), and the image basis is
+// orthonormal (camera_interface.h:40-41 leaves it unnormalised).  The
+// default camera has an empty image; assign a real one before rendering, as
+// the reference's application does (appmain.cc:17, :123).
+class PerspectiveCamera {
+    pppm_camera _c;
+    double _sensitivity = 0.0;
+
+public:
+    PerspectiveCamera();
     PerspectiveCamera(const Vector3D& center, const Vector3D& direction, const Vector3D& up,
                       double fov, int imageWidth, int imageHeight, double sensitivity);
+    // camera ray through image position (pixelX, pixelY), pixelY upward from
+    // the bottom edge (perspective_camera.cc:42-46): origin at the centre,
+    // unit direction
+    Ray getRay(double pixelX, double pixelY) const;
     const pppm_camera& raw() const { return _c; }
     int width() const { return _c.width; }
     int height() const { return _c.height; }
+    Size<int> imagesize() const { return Size<int>(_c.width, _c.height); }
+    Vector3D center() const { return Vector3D(_c.center[0], _c.center[1], _c.center[2]); }
+    Vector3D direction() const { return Vector3D(_c.direction[0], _c.direction[1], _c.direction[2]); }
     double sensitivity() const { return _sensitivity; }
 };
 typedef PerspectiveCamera Camera;
+
+// ---- sources/random_sampler.h:21-24 ------------------------------------------
+// The GPU path draws counter-based Philox-4x32-10 numbers keyed by (seed,
+// pass, path, dimension) whichever sampler is named; the permuted Halton
+// sequence (quasi-Monte Carlo) is not provided and is rejected by render().
+enum RandomSamplerType { RANDOM_SAMPLER_PSEUDO_RANDOM, RANDOM_SAMPLER_QUASI_MONTE_CARLO };
 
 // ---- sources/render_parameters.h:4-44 --------------------------------------
 class RenderParameters {
@@ -270,10 +363,29 @@ public:
     static std::shared_ptr<Device> shared(int device = 0);
 };
 
+// Host copy of a GPU-built 4-wide BVH (pppm_bvh_download) and its scalar
+// closest-hit traversal on the CPU: const, allocation-free and re-entrant, so
+// any number of host threads may call it at once (the reference's OpenMP
+// loops call QBVHAccel::intersect concurrently, ppm_probabilistic.cc:157, 256).
+// Boxes are tested in FP32 with a conservative exit distance, triangles by
+// the reference's FP64 Moller-Trumbore test (sources/triangle.cc:53-78).
+class HostBvh {
+    std::vector<pppm_bvh_node> _nodes;
+    std::vector<float> _tris;  // [n][12] leaf order, v0.w = upload index bits
+    int32_t _root = 0x7FFFFFFF;
+
+public:
+    void download(const Device& dev);
+    bool empty() const { return _root == 0x7FFFFFFF; }
+    // closest hit with t in (1e-12, tmax): upload index of the triangle, or -1
+    int intersect(const Ray& ray, double tmax, double* t) const;
+};
+
 // ---- sources/qbvh_accel.h:13-62 --------------------------------------------
 class QBVHAccel {
     std::shared_ptr<Device> _dev;
     std::vector<Triangle> _tris;
+    HostBvh _host;
     bool _built = false;
 
 public:
@@ -281,9 +393,10 @@ public:
     explicit QBVHAccel(std::shared_ptr<Device> dev);
     // GPU build over the triangle soup (replaces the serial object-median build)
     void construct(const std::vector<Triangle>& triangles);
-    // triangle id or -1; hitpoint->distance() is the ray's tMax on input
+    // triangle id or -1; hitpoint->distance() is the ray's tMax on input.
+    // Host traversal of the downloaded tree: re-entrant.
     int intersect(const Ray& ray, Hitpoint* hitpoint) const;
-    // batch form: the production path
+    // batch form on the GPU: the production path
     void intersect(const std::vector<Ray>& rays, std::vector<Intersection>* out) const;
 };
 
@@ -294,6 +407,7 @@ class Scene {
     std::vector<int> _bsdfIds;
     std::vector<BSDF> _bsdfs;
     std::vector<pppm_light> _lights;
+    HostBvh _host;
     bool _accel = false;
 
 public:
@@ -321,6 +435,7 @@ public:
     size_t numTriangles() const { return _triangles.size(); }
     void clear();
     void setAccelerator();
+    // host traversal of the tree setAccelerator built: const and re-entrant
     bool intersect(const Ray& ray, Intersection& isect) const;
     BBox bounds() const;
     std::shared_ptr<Device> device() const { return _dev; }
@@ -421,7 +536,8 @@ public:
     ProgressivePhotonMappingProb() {}
     explicit ProgressivePhotonMappingProb(const PPMSettings& s) : _settings(s) {}
     // Runs params.spp() passes of params.photons() photons each on the GPU.
-    void render(const Scene& scene, const Camera& camera, const RenderParameters& params);
+    void render(const Scene& scene, const Camera& camera, const RenderParameters& params,
+                RandomSamplerType randomSamplerType = RANDOM_SAMPLER_PSEUDO_RANDOM);
     const Image& result() const { return _result; }
     const std::vector<pppm_pass_stats>& passStats() const { return _stats; }
     // called after every pass with the running average (the reference saves a

@@ -74,6 +74,16 @@ class ProposalParams(C.Structure):
                 ("batch", C.c_int32), ("seed", C.c_uint64)]
 
 
+class Mesh(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("vertices", C.POINTER(C.c_float)), ("n_faces", C.c_int64),
+                ("faces", C.POINTER(C.c_int32))]
+
+
+class BvhNode(C.Structure):
+    _fields_ = [("lox", C.c_float * 4), ("hix", C.c_float * 4), ("loy", C.c_float * 4), ("hiy", C.c_float * 4),
+                ("loz", C.c_float * 4), ("hiz", C.c_float * 4), ("child", C.c_int32 * 4), ("pad", C.c_int32 * 4)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("n_tris", C.c_int64), ("tri_xyz", C.POINTER(C.c_float)),
                 ("bsdf_id", C.POINTER(C.c_int32)), ("n_bsdf", C.c_int32),
@@ -94,6 +104,10 @@ _DECLS = {
     "pppm_scene_upload": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(Bsdf), C.c_int32, C.POINTER(Light), C.c_int32]),
     "pppm_bvh_build": (C.c_int, [_P, C.POINTER(BvhInfo)]),
     "pppm_intersect": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "pppm_bvh_download": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32)]),
+    "pppm_mesh_load": (C.c_int, [C.c_char_p, C.POINTER(Mesh), C.c_char_p, C.c_int32]),
+    "pppm_mesh_free": (None, [C.POINTER(Mesh)]),
     "pppm_hashgrid_build": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
     "pppm_hashgrid_build_boxes": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P,
                                             C.c_int64, C.POINTER(C.c_int64), _P, _P]),
@@ -110,6 +124,7 @@ _DECLS = {
     "pppm_gather": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double,
                               C.c_int32, _P, C.c_int32, _P, C.POINTER(PassStats)]),
     "pppm_stream_copy": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
+    "pppm_kahan_probe": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
     "pppm_issue_probe": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pppm_subsurface_configure": (C.c_int, [_P, C.c_int64]),
     "pppm_subsurface_points": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
@@ -215,6 +230,40 @@ def scenegen(kind, seed=1, frequency=0, width=256, height=256):
     return s
 
 
+def load_mesh(path):
+    """Vertices [n][3] float32 and triangles [m][3] int32 of a .ply / .obj file."""
+    m = Mesh()
+    err = C.create_string_buffer(512)
+    rc = lib().pppm_mesh_load(str(path).encode(), C.byref(m), err, 512)
+    if rc != OK:
+        raise PPPMError(rc, err.value.decode())
+    try:
+        v = np.ctypeslib.as_array(m.vertices, shape=(m.n_vertices, 3)).copy() if m.n_vertices else np.zeros((0, 3), np.float32)
+        f = np.ctypeslib.as_array(m.faces, shape=(m.n_faces, 3)).copy() if m.n_faces else np.zeros((0, 3), np.int32)
+    finally:
+        lib().pppm_mesh_free(C.byref(m))
+    return v, f
+
+
+def batch_specs(n_scenes, seed):
+    """Scene specifications of a batch drawn like the batched-scenes workload:
+    triangle counts log-uniform in [1k, 500k], square resolutions from
+    {128, 256, 512, 1024}, procedural scenes alternating between the box with
+    two icospheres and a random-object scene.  Returns (kind, scene seed,
+    icosphere frequency, resolution) per scene; photons per pass are 8 per
+    pixel."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n_scenes):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        kind = SCENE_RANDOM if k % 2 else SCENE_BOX
+        per = 20.0 if kind == SCENE_RANDOM else 40.0   # 20 f^2 triangles per icosphere
+        f = max(1, int(round((tris / per) ** 0.5)))
+        specs.append((kind, 1000 + k, f, res))
+    return specs
+
+
 def radius_schedule(r1, alpha, n, literal=False):
     out = np.zeros(n, np.float64)
     rc = lib().pppm_radius_schedule(r1, alpha, n, 1 if literal else 0, _ptr(out))
@@ -283,6 +332,16 @@ class Context:
         hits = np.zeros(len(rays), HIT_DTYPE)
         self._chk(self.L.pppm_intersect(self.h, _ptr(rays), len(rays), _ptr(hits)))
         return hits
+
+    def bvh_download(self):
+        """Host copy of the built BVH: nodes (structured), leaf-ordered triangles [n][12], root ref."""
+        nn, nt, root = C.c_int64(), C.c_int64(), C.c_int32()
+        self._chk(self.L.pppm_bvh_download(self.h, None, 0, None, 0, C.byref(nn), C.byref(nt), C.byref(root)))
+        nodes = (BvhNode * max(nn.value, 1))()
+        tris = np.zeros((max(nt.value, 1), 12), np.float32)
+        self._chk(self.L.pppm_bvh_download(self.h, nodes, nn.value, _ptr(tris), nt.value, C.byref(nn), C.byref(nt),
+                                           C.byref(root)))
+        return nodes, tris[:nt.value], root.value
 
     def hashgrid(self, pos, hash_size, hash_scale, bbox_min):
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
@@ -433,6 +492,13 @@ class Context:
         self._chk(self.L.pppm_mlp_train_step(self.h, _ptr(rec), len(rec), learning_rate, step, _ptr(grad),
                                              C.byref(loss)))
         return grad, loss.value
+
+    def kahan_probe(self, values, lanes=64):
+        v = np.ascontiguousarray(values, np.float32)
+        s = np.zeros(lanes, np.float32)
+        c = np.zeros(lanes, np.float32)
+        self._chk(self.L.pppm_kahan_probe(self.h, _ptr(v), len(v), lanes, _ptr(s), _ptr(c)))
+        return s, c
 
     def issue_probe(self):
         v, t, c = C.c_double(), C.c_double(), C.c_double()

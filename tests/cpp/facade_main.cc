@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <thread>
 
 #include "../../tatsy-pppm_amd/host/pppm_amd.h"
 
@@ -78,6 +79,40 @@ int main(int argc, char** argv) {
     }
     CHECK(agree >= total - 1, "Scene::intersect agrees with brute force on %d of %d rays", agree, total);
     std::printf("intersect: %d/%d agree\n", agree, total);
+
+    // --- Scene::intersect from many host threads at once: const and
+    // re-entrant (the reference's OpenMP loops call it concurrently,
+    // ppm_probabilistic.cc:157, 256) --------------------------------------------
+    {
+        const int R = 4000, T = 8;
+        std::vector<Ray> rays;
+        for (int k = 0; k < R; k++)
+            rays.push_back(Ray(Vector3D(U(rng), U(rng), U(rng)), Vector3D(N(rng), N(rng), N(rng)).normalized()));
+        std::vector<int> ids(R);
+        std::vector<double> ts(R);
+        for (int k = 0; k < R; k++) {
+            Intersection is;
+            scene.intersect(rays[(size_t)k], is);
+            ids[(size_t)k] = is.objectID();
+            ts[(size_t)k] = is.hittingDistance();
+        }
+        std::vector<int> bad(T, 0);
+        std::vector<std::thread> pool;
+        for (int w = 0; w < T; w++)
+            pool.emplace_back([&, w] {
+                for (int rep = 0; rep < 3; rep++)
+                    for (int k = w; k < R; k += 1 + (w % 3)) {
+                        Intersection is;
+                        scene.intersect(rays[(size_t)k], is);
+                        if (is.objectID() != ids[(size_t)k] || is.hittingDistance() != ts[(size_t)k]) bad[(size_t)w]++;
+                    }
+            });
+        for (std::thread& t : pool) t.join();
+        int nbad = 0;
+        for (int b : bad) nbad += b;
+        CHECK(nbad == 0, "concurrent Scene::intersect differs from serial on %d queries", nbad);
+        std::printf("concurrent intersect: %d threads, %d mismatches\n", T, nbad);
+    }
 
     // --- QBVHAccel on its own ------------------------------------------------
     {

@@ -84,3 +84,19 @@ def test_batch_rejects_bad_input():
     with pytest.raises(P.PPPMError):
         b.run(3)  # more passes than begun with
     b.close()
+
+
+def test_batched_scene_equals_solo_at_power_of_two_hash_size(gpu_ctx):
+    """A hash size of exactly 2^23: single and enqueued passes must order each
+    bucket's records alike (the in-bucket Morton bits are derived the same way)."""
+    say("batch vs solo at hash size 2^23")
+    passes, S = 2, 1 << 23
+    s = P.scenegen(P.SCENE_BOX, 3, 6, 48, 48)
+    n, r1 = 40000, 0.01 * s.diagonal()
+    b = P.Batch(branches=1)
+    b.add(s, n, passes, r1, hash_size=S, seed=0x321, capacity_factor=4.0, gather_mode=P.GATHER_LANES)
+    b.run(passes)
+    got = b.image(0)
+    want = _solo(gpu_ctx, s, n, passes, r1, S, P.GATHER_LANES, 0x321)
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    b.close()

@@ -1,0 +1,172 @@
+"""Whole renders of every workload shape against the oracle, through the C ABI.
+
+Each configuration runs at a reduced size the FP64 oracle finishes in
+seconds, with the same seeds on both sides: the GPU and the oracle then draw
+the same random numbers, so their images differ only where FP32 and FP64
+arithmetic take different paths.  Bar: relative RMSE against the oracle
+below 5 % of the seed-to-seed Monte Carlo noise and below 1e-2.
+
+* the occluded-light scene (two rooms joined by a slit), plain emission;
+* the same scene with a fixed learned-proposal table (the mixture emission);
+* the translucent scene (dipole BSSRDF sphere, irradiance at surface points);
+* scenes of the batched loop, each against the oracle, and a 64-scene batch
+  drawn from the batched workload's distribution with property checks.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pppm_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.mean(b))
+
+
+def _setup(ctx, s, photons, passes, r1, seed=0x7A75, points=None, hash_size=0, mode=P.GATHER_AUTO):
+    ctx.upload(s)
+    ctx.build_bvh()
+    ctx.set_camera(s.camera)
+    if points is not None:
+        ctx.subsurface_configure(points)
+    ctx.begin(photons, passes, r1, seed=seed, hash_size=hash_size, gather_mode=mode)
+
+
+def _check(g, o, o2, what):
+    e, noise = rel_rmse(g, o), rel_rmse(o2, o)
+    say(f"{what}: relative RMSE vs oracle {e:.3e}, seed-to-seed noise {noise:.3e}")
+    assert np.all(np.isfinite(g)) and g.min() >= 0.0
+    assert e < 0.05 * noise and e < 1e-2, (e, noise)
+
+
+def test_two_rooms_plain_matches_oracle(gpu_ctx):
+    say("two rooms, plain emission, vs oracle")
+    s = P.scenegen(P.SCENE_TWO_ROOMS, 1, 12, 64, 64)
+    N, passes, r1 = 150000, 4, 0.01 * s.diagonal()
+    _setup(gpu_ctx, s, N, passes, r1)
+    for _ in range(passes):
+        gpu_ctx.render_pass()
+    g = gpu_ctx.image()
+    os_ = O.OracleScene(s)
+    o, _ = os_.render(s.camera, N, passes, r1)
+    o2, _ = os_.render(s.camera, N, passes, r1, seed=0x7A76)
+    _check(g, o, o2, "two rooms plain")
+
+
+def test_two_rooms_fixed_proposal_matches_oracle(gpu_ctx):
+    say("two rooms, fixed proposal table, vs oracle")
+    s = P.scenegen(P.SCENE_TWO_ROOMS, 1, 12, 64, 64)
+    N, passes, r1, anchors, lam = 150000, 4, 0.01 * s.diagonal(), 4, 0.3
+    rng = np.random.default_rng(11)
+    # a peaked, position-dependent table: every anchor prefers other cells
+    prob = rng.gamma(0.3, size=(anchors * anchors, P.PROPOSAL_CELLS)).astype(np.float32) + 1e-4
+    _setup(gpu_ctx, s, N, passes, r1)
+    gpu_ctx.proposal_set_table(prob, anchors, lam)
+    sts = [gpu_ctx.render_pass() for _ in range(passes)]
+    assert all(t["proposal_active"] for t in sts)
+    g = gpu_ctx.image()
+    cdf, side = gpu_ctx.proposal_table()
+    assert side == anchors
+    pr = O.proposal(cdf, side, lam)
+    os_ = O.OracleScene(s)
+    o, _ = os_.render(s.camera, N, passes, r1, prop=pr)
+    o2, _ = os_.render(s.camera, N, passes, r1, prop=pr, seed=0x7A76)
+    _check(g, o, o2, "two rooms fixed proposal")
+    # the mixture changes where photons go, not what they estimate
+    plain, _ = os_.render(s.camera, N, passes, r1)
+    assert abs(g.mean() / plain.mean() - 1.0) < 0.1
+
+
+def test_translucent_render_matches_oracle(gpu_ctx):
+    say("translucent two rooms (dipole BSSRDF), vs oracle")
+    s = P.scenegen(P.SCENE_TWO_ROOMS_SSS, 1, 24, 64, 64)
+    N, passes, r1, points = 400000, 3, 0.01 * s.diagonal(), 20000
+    _setup(gpu_ctx, s, N, passes, r1, points=points)
+    sts = [gpu_ctx.render_pass() for _ in range(passes)]
+    assert all(t["sss_hits"] > 0 and t["irradiance_photons"] > 0 for t in sts)
+    g = gpu_ctx.image()
+    pos, nrm, area = gpu_ctx.subsurface_points()
+    assert len(pos) == points
+    os_ = O.OracleScene(s)
+    o, st = os_.render(s.camera, N, passes, r1, sss_points=(pos, nrm, area))
+    o2, _ = os_.render(s.camera, N, passes, r1, sss_points=(pos, nrm, area), seed=0x7A76)
+    assert st[3] == sum(t["sss_hits"] for t in sts)        # same transmitted eye paths
+    assert st[2] == sum(t["irradiance_photons"] for t in sts)  # same irradiance photons
+    _check(g, o, o2, "translucent")
+
+
+BATCH = [  # kind, scene seed, frequency, resolution
+    (P.SCENE_BOX, 11, 6, 48),
+    (P.SCENE_RANDOM, 12, 10, 64),
+    (P.SCENE_BOX_CAUSTIC, 13, 6, 64),
+    (P.SCENE_TWO_ROOMS, 14, 6, 48),
+]
+
+
+def test_batched_scenes_match_oracle(gpu_ctx):
+    say("batched scenes vs oracle")
+    passes = 3
+    b = P.Batch(branches=2)
+    scenes = []
+    for k, (kind, sd, f, res) in enumerate(BATCH):
+        s = P.scenegen(kind, sd, f, res, res)
+        # 64 photons per pixel (the batched workload uses 8): at 8 the seed-to-seed
+        # noise of these small images is ~50 % and a few FP32 / FP64 path
+        # divergences alone reach 1e-2
+        n, r1 = 64 * res * res, 0.01 * s.diagonal()
+        b.add(s, n, passes, r1, hash_size=n, seed=0x500 + k, scene_id=k, capacity_factor=4.0)
+        scenes.append((s, n, r1, 0x500 + k, k))
+    b.tune()
+    b.run(passes)
+    for k, (s, n, r1, seed, sid) in enumerate(scenes):
+        g = b.image(k)
+        os_ = O.OracleScene(s)
+        o, _ = os_.render(s.camera, n, passes, r1, seed=seed, scene_id=sid, hash_size=n)
+        o2, _ = os_.render(s.camera, n, passes, r1, seed=seed + 0x10000, scene_id=sid, hash_size=n)
+        _check(g, o, o2, f"batched scene {k} (kind {BATCH[k][0]})")
+    b.close()
+
+
+def test_batch_of_64_scenes_from_the_batched_workload(gpu_ctx):
+    say("64-scene batch from the batched workload's distribution")
+    specs = P.batch_specs(64, 7)
+    passes = 2
+    b = P.Batch(branches=4)
+    scenes = []
+    for k, (kind, sd, f, res) in enumerate(specs):
+        s = P.scenegen(kind, sd, f, res, res)
+        n = 8 * res * res
+        b.add(s, n, passes, 0.01 * s.diagonal(), hash_size=n, seed=0x900 + k, capacity_factor=4.0)
+        scenes.append((s, n))
+    tris = np.array([s.n_tris for s, _ in scenes])
+    assert tris.min() >= 500 and tris.max() <= 600_000
+    assert {s.camera.width for s, _ in scenes} <= {128, 256, 512, 1024}
+    b.run(1)
+    b.run(1)  # the captured graph replayed
+    for k, (s, n) in enumerate(scenes):
+        img = b.image(k)
+        assert img.shape == (s.camera.height, s.camera.width, 3)
+        assert np.all(np.isfinite(img)) and img.min() >= 0.0 and img.max() > 0.0, k
+        info = b.info(k)
+        assert 0 <= info["branch"] < 4 and info["gather_mode"] in (P.GATHER_LANES, P.GATHER_TILED, P.GATHER_WAVE)
+    # the largest scene equals its solo render bit for bit
+    k = int(np.argmax([s.n_tris * s.camera.width for s, _ in scenes]))
+    s, n = scenes[k]
+    gpu_ctx.upload(s)
+    gpu_ctx.build_bvh()
+    gpu_ctx.set_camera(s.camera)
+    gpu_ctx.begin(n, passes, 0.01 * s.diagonal(), seed=0x900 + k, hash_size=n,
+                  gather_mode=b.info(k)["gather_mode"], capacity_factor=4.0)
+    for _ in range(passes):
+        gpu_ctx.render_pass()
+    assert np.array_equal(gpu_ctx.image(), b.image(k))
+    b.close()

@@ -61,3 +61,14 @@ def test_cpp_translucent_and_learned_renders_match_c_abi(gpu_ctx, tmp_path):
     for _ in range(4):
         gpu_ctx.render_pass()
     assert np.array_equal(np.fromfile(out[2], np.float64).reshape(48, 48, 3), gpu_ctx.image())
+
+
+def test_reference_application_shape_runs(tmp_path):
+    """sources/main/appmain.cc:11-26's call pattern (default Camera assigned in
+    setScene, a placed .ply mesh, render with the default sampler) on the GPU."""
+    print("[case] application-shaped program", flush=True)
+    exe = os.path.join(ROOT, "tests", "cpp", "appmain_shape")
+    r = subprocess.run([exe, "64", "48", "2"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0 and r.stdout.rstrip().endswith("OK")

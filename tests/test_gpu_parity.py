@@ -408,3 +408,26 @@ def test_traversal_counting_kernels_change_nothing_but_the_counts(gpu_ctx, c1_sm
         assert s1["photon_nodes"] >= s1["photon_segments"] > 0
         assert s1["eye_nodes"] >= s1["eye_segments"] > 0
         assert 0 < s1["photon_tris"] and 0 < s1["eye_tris"]
+
+
+# ---------------------------------------------------------------------------
+# Image accumulation: FP32 compensated (Kahan) sums against FP64
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["constant", "uniform"])
+def test_kahan_accumulation_matches_fp64(gpu_ctx, kind):
+    say(f"Kahan accumulator, 10^6 adds ({kind})")
+    n = 1_000_000
+    if kind == "constant":
+        v = np.full(n, 1e-4, np.float32)
+    else:
+        v = np.random.default_rng(3).uniform(0.0, 2e-4, n).astype(np.float32)
+    s, c = gpu_ctx.kahan_probe(v, lanes=64)
+    ref = v.astype(np.float64).sum()  # the same FP32 addends summed in FP64
+    got = s.astype(np.float64) - c.astype(np.float64)
+    # compensated: within a few FP32 ulps of the total
+    assert np.all(np.abs(got - ref) <= 4 * np.spacing(np.float32(ref))), (got - ref).max()
+    # the plain FP32 running sum of the same addends is far off (the test can tell)
+    naive = np.float32(0.0)
+    for x in v[:200_000]:
+        naive = np.float32(naive + x)
+    assert abs(float(naive) - v[:200_000].astype(np.float64).sum()) > 100 * np.spacing(np.float32(ref))
