@@ -107,8 +107,20 @@ typedef struct {
     int32_t hash_size;     /* 0: the number of stored photons of each pass */
     double capacity_factor;/* photon buffer = photons * factor records (0: 8) */
     int32_t gather_mode;   /* PPPM_GATHER_AUTO (0, the default: resolves to LANES), LANES, TILED or WAVE */
-    int32_t reserved;
+    int32_t method;        /* PPPM_METHOD_PROBABILISTIC (0) or PPPM_METHOD_HACHISUKA */
 } pppm_render_params;
+
+/* Render methods.  PROBABILISTIC: ProgressivePhotonMappingProb
+ * (sources/ppm_probabilistic.cc), one global radius shrinking per pass (the
+ * P-KZ form).  HACHISUKA: ProgressivePhotonMapping
+ * (sources/progressive_photon_mapping.cc:22-287), a render point per pixel
+ * with its own radius, photon count and flux, updated photon by photon with
+ * g = (n alpha + alpha) / (n alpha + 1); radius1 is unused (each pass's initial
+ * radius comes from the hit points' extent, :116-125).  No learned proposal
+ * and no BSSRDF materials with HACHISUKA; pppm_render_passes runs its passes
+ * one by one. */
+#define PPPM_METHOD_PROBABILISTIC 0
+#define PPPM_METHOD_HACHISUKA 1
 
 #define PPPM_GATHER_AUTO 0  /* the kernel measured fastest on gfx950 (currently LANES) */
 #define PPPM_GATHER_LANES 1 /* one lane per Morton-sorted hit point over its own buckets */
@@ -159,7 +171,8 @@ int pppm_device_info(void* ctx, char* name, int32_t name_len, int32_t* cu_count,
                      int64_t* hbm_bytes);
 
 /* ---- scene: replaces Scene::add / setAccelerator (sources/scene.h:38-56,
- * sources/scene.cc:35-38) and QBVHAccel::construct (sources/qbvh_accel.h:49) ---- */
+ * sources/scene.cc:35-38) and QBVHAccel::construct (sources/qbvh_accel.h:49).
+ * nl may be 0 when an environment map provides the light. ---- */
 int pppm_scene_upload(void* ctx, const float* tri_xyz /* [n][9] */, const int32_t* bsdf_id,
                       int64_t n, const pppm_bsdf* table, int32_t nb, const pppm_light* lights,
                       int32_t nl);
@@ -205,6 +218,23 @@ int pppm_hashgrid_build_boxes(void* ctx, const double* box_min /* [n][3] */,
                               int64_t cap, int64_t* n_pairs, int32_t* cell_start,
                               int32_t* cell_end);
 
+/* ---- environment map: replaces Envmap (sources/envmap.cc:20-163) as an
+ * emitter beside the area lights, and the radiance of eye rays that leave the
+ * scene (ppm_probabilistic.cc:248-251) ---- */
+/* rgb [height][width][3] linear radiance, lat-long: row 0 looks along +y,
+ * column x at phi = 2 pi (x + 0.5) / width from +x towards +z; width,
+ * height >= 64.  rgb = NULL removes the map.  Set after pppm_scene_upload
+ * (which removes it) and before pppm_render_begin.  Photons take it by power
+ * against the area lights: a direction from the reference's 64 x 64
+ * importance cells, a point on a disc of the scene's bounding-sphere radius
+ * facing that direction, flux L pi R^2 / (p(w) N P) with the exact
+ * solid-angle density p(w) of the cell (the reference's flux scale omits the
+ * disc area, envmap.cc:81-83). */
+int pppm_envmap_set(void* ctx, const float* rgb, int32_t width, int32_t height);
+/* The importance cells alone, on the host (no device): pdf [4096] and running
+ * sums cdf [4096] of Envmap::createImportanceMap (envmap.cc:88-163). */
+int pppm_envmap_importance(const float* rgb, int32_t width, int32_t height, double* pdf, double* cdf);
+
 /* ---- radius schedule (sources/ppm_probabilistic.cc:94), FP64 on the host ---- */
 /* literal = 0: Knaus-Zwicker r2_{i+1} = r2_i (i+alpha)/(i+1); literal = 1: the
  * reference's r_{t+1} = (t+1)/(t+alpha) r_t.  out[k] = r_{k+1}. */
@@ -231,6 +261,22 @@ int pppm_count_traversal(void* ctx, int32_t enable);
 /* FP64 image [height][width][3], row 0 at the top, averaged over the passes
  * done: (sum - compensation) / passes. */
 int pppm_image_read(void* ctx, double* rgb, int64_t n_values);
+/* The current image (as pppm_image_read averages it) encoded on the device
+ * and written to path by its extension: .png / .bmp 8-bit RGB after
+ * c^(1 / gamma) (gamma <= 0: none) clamped to [0, 1] and truncated to a byte
+ * (Image::gamma + Image::savePng / saveBmp, sources/image.cc:202-213,
+ * 484-496, 530-533); .hdr Radiance RGBE (Image::saveHdr, :436-482); .pfm
+ * linear FP32.  Only the encoded bytes cross to the host. */
+int pppm_image_save(void* ctx, const char* path, double gamma);
+/* Host image files without a device: rgb [height][width][3] FP64, top row
+ * first, written by extension as pppm_image_save writes them. */
+int pppm_image_write(const char* path, const double* rgb, int32_t width, int32_t height, double gamma);
+/* Reads .hdr (Radiance RGBE, run-length or flat scanlines; Image::loadHdr,
+ * sources/image.cc:296-434), .pfm or 24-bit .bmp into an FP64 [height][width][3]
+ * buffer the library allocates (release with pppm_image_free). */
+int pppm_image_load(const char* path, double** rgb, int32_t* width, int32_t* height, char* err,
+                    int32_t err_len);
+void pppm_image_free(double* rgb);
 /* Test hooks for the last pass: photons in (photon index, bounce) order,
  * key = photon index << 6 | bounce, and the hash grid over them. */
 int pppm_last_photons(void* ctx, float* pos, float* flux, float* normal, uint32_t* key,

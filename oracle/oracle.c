@@ -179,6 +179,13 @@ struct orc_scene {
     int32_t* tri_order;
     int32_t depth;
     v3 bmin, bmax;
+    /* environment map (env_w == 0: none) */
+    int32_t env_w, env_h, env_last;
+    double* env_rgb;        /* [h][w][3] */
+    double env_pdf[4096], env_cdf[4096];
+    double env_power;       /* selection weight against the area lights */
+    v3 env_c;               /* bounding sphere of the triangles */
+    double env_r;
 };
 
 /* triangle.cc:53-78 (Moller-Trumbore, FP64).  Returns t or -1. */
@@ -392,6 +399,7 @@ orc_scene* orc_scene_create(const float* tri, const int32_t* bsdf_id, int64_t n,
 
 void orc_scene_destroy(orc_scene* s) {
     if (!s) return;
+    free(s->env_rgb);
     free(s->p0); free(s->p1); free(s->p2); free(s->mat); free(s->bsdf);
     free(s->lights); free(s->light_cdf); free(s->nodes); free(s->tri_order);
     free(s);
@@ -408,6 +416,135 @@ int32_t orc_qbvh_stats(const orc_scene* s, int64_t* nodes, int64_t* leaves, int3
 
 static int32_t scene_intersect(const orc_scene* s, v3 o, v3 d, double* t) {
     return qbvh_intersect(s, o, d, 0.0, INFTY, t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Environment map (sources/envmap.cc)                                       */
+/* ------------------------------------------------------------------------ */
+
+#define ENV_CELLS 64  /* IMPORTANCE_MAP_SIZE, envmap.h:31 */
+
+/* Envmap::sampleFromDir (envmap.cc:47-60): nearest texel of the lat-long
+ * image, theta = acos(y) from +y down the rows, phi = atan2(z, x) in [0, 2 pi)
+ * across the columns */
+static v3 env_lookup(const orc_scene* s, v3 d) {
+    double theta = acos(d.y);
+    double phi = atan2(d.z, d.x);
+    if (phi < 0.0) phi += 2.0 * PI;
+    double x = phi / (2.0 * PI) * s->env_w;
+    double y = theta / PI * s->env_h;
+    int ix = (int)x, iy = (int)y;
+    ix = ix < 0 ? 0 : (ix > s->env_w - 1 ? s->env_w - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > s->env_h - 1 ? s->env_h - 1 : iy);
+    const double* p = s->env_rgb + 3 * ((size_t)iy * s->env_w + ix);
+    return V(p[0], p[1], p[2]);
+}
+
+/* the direction of importance-map coordinates (u, v) (envmap.cc:76-80):
+ * equal-area cells, y = 1 - 2 v, phi = 2 pi u */
+static v3 env_direction(double u, double v) {
+    double phi = u * PI * 2.0;
+    double y = (1.0 - v) * 2.0 - 1.0;
+    return V(sqrt(1.0 - y * y) * cos(phi), y, sqrt(1.0 - y * y) * sin(phi));
+}
+
+int32_t orc_scene_set_envmap(orc_scene* s, const float* rgb, int32_t w, int32_t h) {
+    free(s->env_rgb);
+    s->env_rgb = NULL;
+    s->env_w = s->env_h = 0;
+    if (!rgb || w < ENV_CELLS || h < ENV_CELLS) return -1;
+    s->env_rgb = (double*)malloc(sizeof(double) * 3 * (size_t)w * h);
+    for (size_t i = 0; i < 3 * (size_t)w * h; i++) s->env_rgb[i] = rgb[i];
+    s->env_w = w;
+    s->env_h = h;
+    /* Envmap::createImportanceMap, the warped part (envmap.cc:128-162): per
+     * 64 x 64 cell the squared luminance of the mean of (W/64) x (H/64)
+     * lookups, normalised by (total + EPS), and its running sum */
+    const int sx_n = w / ENV_CELLS, sy_n = h / ENV_CELLS;
+    double total = 0.0;
+    for (int iy = 0; iy < ENV_CELLS; iy++)
+        for (int ix = 0; ix < ENV_CELLS; ix++) {
+            v3 acc = V(0, 0, 0);
+            for (int sy = 0; sy < sy_n; sy++)
+                for (int sx = 0; sx < sx_n; sx++) {
+                    double u = (ix + (double)sx / sx_n) / ENV_CELLS;
+                    double v = (iy + (double)sy / sy_n) / ENV_CELLS;
+                    acc = add(acc, scl(env_lookup(s, env_direction(u, v)), 1.0 / (sx_n * sy_n)));
+                }
+            double l = acc.x * 0.2126 + acc.y * 0.7152 + acc.z * 0.0722;
+            s->env_pdf[iy * ENV_CELLS + ix] = l * l;
+            total += l * l;
+        }
+    s->env_pdf[0] /= (total + EPS);
+    s->env_cdf[0] = s->env_pdf[0];
+    for (int i = 1; i < ENV_CELLS * ENV_CELLS; i++) {
+        s->env_pdf[i] /= (total + EPS);
+        s->env_cdf[i] = s->env_pdf[i] + s->env_cdf[i - 1];
+    }
+    s->env_last = 0;
+    for (int i = 0; i < ENV_CELLS * ENV_CELLS; i++)
+        if (s->env_pdf[i] > 0.0) s->env_last = i;
+    /* photons leave a disc as wide as the triangles' bounding sphere */
+    s->env_c = scl(add(s->bmin, s->bmax), 0.5);
+    s->env_r = 0.5 * norm(sub(s->bmax, s->bmin));
+    /* selection weight: pi R^2 times the integral of the mean channel over
+     * the sphere, texel by texel (texel solid angle (2 pi / w)(cos t0 - cos t1)) */
+    double pw = 0.0;
+    for (int iy = 0; iy < h; iy++) {
+        double dom = (2.0 * PI / w) * (cos(PI * iy / h) - cos(PI * (iy + 1) / h));
+        double row = 0.0;
+        for (int ix = 0; ix < w; ix++) {
+            const double* p = s->env_rgb + 3 * ((size_t)iy * w + ix);
+            row += (p[0] + p[1] + p[2]) / 3.0;
+        }
+        pw += row * dom;
+    }
+    s->env_power = PI * s->env_r * s->env_r * pw;
+    return 0;
+}
+
+/* first cell whose running sum reaches r (std::lower_bound, envmap.cc:67-69);
+ * above the last sum (at most EPS below 1) the last cell with mass */
+static int env_cell(const orc_scene* s, double r) {
+    int lo = 0, hi = ENV_CELLS * ENV_CELLS;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (s->env_cdf[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < ENV_CELLS * ENV_CELLS ? lo : s->env_last;
+}
+
+void orc_envmap_cells(const orc_scene* s, double* pdf, double* cdf) {
+    memcpy(pdf, s->env_pdf, sizeof(s->env_pdf));
+    memcpy(cdf, s->env_cdf, sizeof(s->env_cdf));
+}
+
+void orc_envmap_lookup(const orc_scene* s, const double* dirs, int64_t n, double* rgb) {
+    for (int64_t k = 0; k < n; k++) {
+        v3 c = env_lookup(s, V(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        rgb[3 * k] = c.x;
+        rgb[3 * k + 1] = c.y;
+        rgb[3 * k + 2] = c.z;
+    }
+}
+
+void orc_envmap_sample_dir(const orc_scene* s, const double* r, int64_t n, double* dir, double* l_over_pdf,
+                           int32_t* cell) {
+    for (int64_t k = 0; k < n; k++) {
+        int idx = env_cell(s, r[3 * k]);
+        double u = ((idx % ENV_CELLS) + r[3 * k + 1]) / ENV_CELLS;
+        double v = ((idx / ENV_CELLS) + r[3 * k + 2]) / ENV_CELLS;
+        v3 w = env_direction(u, v);
+        v3 L = env_lookup(s, w);
+        dir[3 * k] = w.x;
+        dir[3 * k + 1] = w.y;
+        dir[3 * k + 2] = w.z;
+        l_over_pdf[3 * k] = L.x / s->env_pdf[idx];
+        l_over_pdf[3 * k + 1] = L.y / s->env_pdf[idx];
+        l_over_pdf[3 * k + 2] = L.z / s->env_pdf[idx];
+        if (cell) cell[k] = idx;
+    }
 }
 
 void orc_intersect_batch(const orc_scene* s, const float* rays, int64_t n, int32_t brute,
@@ -669,8 +806,34 @@ static double emit_direction(const orc_proposal* pr, v3 nl, double pu, double pv
 static void emit_photon(const orc_scene* s, const orc_proposal* pr, rng_t* r, int64_t N, v3* pos, v3* dir,
                         v3* flux, v3* nl_out) {
     double ul = rng_at(r, 0);
+    double env_p = s->env_w > 0 ? s->env_power : 0.0;
+    double total = s->light_total + env_p;
+    double target = ul * total;
+    if (env_p > 0.0 && (s->nl == 0 || target >= s->light_cdf[s->nl])) {
+        /* environment: a direction w by the importance map (envmap.cc:62-80),
+         * then a point on a disc of the bounding sphere's radius facing w,
+         * travelling along -w; flux = L(w) pi R^2 / (p(w) N P_env) with the
+         * solid-angle density p(w) = pdf(cell) 4096 / (4 pi) of the equal-area
+         * cells (the reference's flux scale, :81-83, has no disc area) */
+        int idx = env_cell(s, rng_at(r, 3));
+        double u = ((idx % ENV_CELLS) + rng_at(r, 4)) / ENV_CELLS;
+        double v = ((idx / ENV_CELLS) + rng_at(r, 5)) / ENV_CELLS;
+        v3 w = env_direction(u, v);
+        v3 L = env_lookup(s, w);
+        double k = PI * s->env_r * s->env_r * (4.0 * PI / (ENV_CELLS * ENV_CELLS)) /
+                   (s->env_pdf[idx] * (double)N * (env_p / total));
+        *flux = scl(L, k);
+        v3 a;
+        if (fabs(w.x) > EPS) a = normalized(cross(V(0.0, 1.0, 0.0), w));
+        else a = normalized(cross(V(1.0, 0.0, 0.0), w));
+        v3 b = cross(w, a);
+        double rad = s->env_r * sqrt(rng_at(r, 1)), ang = 2.0 * PI * rng_at(r, 2);
+        *pos = add(add(s->env_c, scl(w, s->env_r * 1.0001)), add(scl(a, rad * cos(ang)), scl(b, rad * sin(ang))));
+        *dir = neg(w);
+        *nl_out = neg(w);
+        return;
+    }
     int l = 0;
-    double target = ul * s->light_total;
     while (l < s->nl - 1 && s->light_cdf[l + 1] <= target) l++;
     const orc_light* L = &s->lights[l];
     v3 c = V(L->corner[0], L->corner[1], L->corner[2]);
@@ -681,7 +844,7 @@ static void emit_photon(const orc_scene* s, const orc_proposal* pr, rng_t* r, in
     v3 nl = normalized(cross(eu, ev));
     *nl_out = nl;
     double wgt = emit_direction(pr, nl, u1, u2, r, dir, NULL);
-    double pl = (s->light_cdf[l + 1] - s->light_cdf[l]) / s->light_total;
+    double pl = (s->light_cdf[l + 1] - s->light_cdf[l]) / total;
     double area = norm(cross(eu, ev));
     double k = PI * area / (pl * (double)N) * wgt;
     *flux = V(L->emission[0] * k, L->emission[1] * k, L->emission[2] * k);
@@ -887,8 +1050,18 @@ static void camera_ray(const orc_camera* c, double px, double py_up, v3* o, v3* 
  * form.  EYE_HIT: *hw = throughput * rho / (pi * roulette).  EYE_SSS:
  * *hw = throughput * F_t / (P_t * roulette) (the transmitted branch of
  * :272-290, divided by the roulette) and *mat = the material. */
+static int eye_path_rr(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P, int64_t pix,
+                       v3* hpos, v3* hnrm, v3* hw, v3* direct, int32_t* mat, int roulette_on);
+
 static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P,
                     int64_t pix, v3* hpos, v3* hnrm, v3* hw, v3* direct, int32_t* mat) {
+    return eye_path_rr(s, cam, P, pix, hpos, hnrm, hw, direct, mat, 1);
+}
+
+/* roulette_on = 0: no eye Russian roulette (the Hachisuka method's eye pass,
+ * progressive_photon_mapping.cc:289-343) */
+static int eye_path_rr(const orc_scene* s, const orc_camera* cam, const orc_pass_params* P, int64_t pix,
+                       v3* hpos, v3* hnrm, v3* hw, v3* direct, int32_t* mat, int roulette_on) {
     int W = cam->width, H = cam->height;
     int x = (int)(pix % W), row = (int)(pix / W);
     rng_t r;
@@ -902,7 +1075,11 @@ static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_pa
     for (int bounce = 0;; bounce++) {
         double t;
         int32_t tri = bounce >= P->max_bounces ? -1 : scene_intersect(s, o, d, &t);
-        if (tri < 0) return EYE_NONE; /* black environment */
+        if (tri < 0) {
+            /* the environment, or black (ppm_probabilistic.cc:248-251) */
+            if (s->env_w > 0) *direct = mul(thr, env_lookup(s, d));
+            return EYE_NONE;
+        }
         int dd = EYE_DIM_BOUNCE + 3 * bounce;
         double r0 = rng_at(&r, dd), r1 = rng_at(&r, dd + 1), r2 = rng_at(&r, dd + 2);
         const orc_bsdf* b = &s->bsdf[s->mat[tri]];
@@ -915,7 +1092,7 @@ static int eye_path(const orc_scene* s, const orc_camera* cam, const orc_pass_pa
         }
         v3 rho = refl_of(b);
         double roulette = vmax3(rho);
-        if (bounce >= 3) {
+        if (roulette_on && bounce >= 3) {
             if (r0 > roulette) return EYE_NONE;
         } else {
             roulette = 1.0;
@@ -1217,7 +1394,11 @@ static v3 pt_path(const orc_scene* s, const orc_camera* cam, int64_t pix, int32_
     for (int bounce = 0; bounce < max_bounces; bounce++) {
         double t;
         int32_t tri = scene_intersect(s, o, d, &t);
-        if (tri < 0) break;
+        if (tri < 0) {
+            /* the environment is not sampled at diffuse vertices: count it on every miss */
+            if (s->env_w > 0) L = add(L, mul(thr, env_lookup(s, d)));
+            break;
+        }
         const orc_bsdf* b = &s->bsdf[s->mat[tri]];
         v3 hp = add(o, scl(d, t));
         v3 ng = tri_normal(s, tri);
@@ -1296,6 +1477,142 @@ void orc_pathtrace(const orc_scene* s, const orc_camera* cam, int32_t spp, int32
         image[3 * i + 1] = acc.y / spp;
         image[3 * i + 2] = acc.z / spp;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Progressive photon mapping with per-point radii                           */
+/* (sources/progressive_photon_mapping.cc:22-287)                            */
+/* ------------------------------------------------------------------------ */
+
+/* A render point per pixel keeps its radius^2, photon count and flux across
+ * passes.  Per pass: eye paths without roulette to their first Lambertian
+ * surface (:289-343; what they see of emitters and the environment is the
+ * pixel's emission); the initial radius from the hit points' extent
+ * (:116-125); photons; then each render point takes the photons of its box
+ * [x - R, x + R] one at a time -- the deduplicated buckets of its cells in z,
+ * y, x order, each in photon order -- through the update of :244-258
+ * (g = (n alpha + alpha) / (n alpha + 1)).  The reference's threads set the
+ * order of the updates; this is one fixed order.  Image (:83-86):
+ * (emission + tau / (pi r^2)) / passes. */
+int32_t orc_render_hachisuka(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
+                             double alpha, double ray_eps, uint64_t seed, uint32_t scene_id, int32_t hash_size,
+                             double* image, int64_t* stats) {
+    int64_t np = (int64_t)cam->width * cam->height;
+    double* r2 = (double*)calloc((size_t)np, sizeof(double));
+    int64_t* cnt = (int64_t*)calloc((size_t)np, sizeof(int64_t));
+    double* tau = (double*)calloc(3 * (size_t)np, sizeof(double));
+    double* emis = (double*)calloc(3 * (size_t)np, sizeof(double));
+    int32_t* kind = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+    v3* xp = (v3*)malloc(sizeof(v3) * (size_t)np);
+    v3* xn = (v3*)malloc(sizeof(v3) * (size_t)np);
+    v3* xw = (v3*)malloc(sizeof(v3) * (size_t)np);
+    int64_t tot_acc = 0;
+    for (int32_t k = 0; k < n_passes; k++) {
+        orc_pass_params P;
+        memset(&P, 0, sizeof(P));
+        P.photons = photons;
+        P.pass_index = k + 1;
+        P.max_bounces = 64;
+        P.ray_eps = ray_eps;
+        P.seed = seed;
+        P.scene_id = scene_id;
+        /* hit points: FP32 positions, as the GPU stores them */
+        double lo[3] = {INFTY, INFTY, INFTY}, hi[3] = {-INFTY, -INFTY, -INFTY};
+        int64_t nh = 0;
+        for (int64_t i = 0; i < np; i++) {
+            v3 a = V(0, 0, 0), n_ = V(0, 0, 0), w = V(0, 0, 0), d;
+            kind[i] = eye_path_rr(s, cam, &P, i, &a, &n_, &w, &d, NULL, 0);
+            emis[3 * i] += d.x;
+            emis[3 * i + 1] += d.y;
+            emis[3 * i + 2] += d.z;
+            if (kind[i] != EYE_HIT) continue;
+            xp[i] = V((float)a.x, (float)a.y, (float)a.z);
+            xn[i] = V((float)n_.x, (float)n_.y, (float)n_.z);
+            xw[i] = V((float)w.x, (float)w.y, (float)w.z);
+            const double c[3] = {xp[i].x, xp[i].y, xp[i].z};
+            for (int d2 = 0; d2 < 3; d2++) {
+                lo[d2] = fmin(lo[d2], c[d2]);
+                hi[d2] = fmax(hi[d2], c[d2]);
+            }
+            nh++;
+        }
+        deposit *dm, *di;
+        int32_t *pm, *pi;
+        int64_t ni;
+        int64_t nd = trace_pass(s, &P, NULL, 0, &dm, &pm, &di, &pi, &ni);
+        if (nh > 0) {
+            double ext = ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2])) / 3.0;
+            double irad = ext / ((cam->width + cam->height) / 2.0) * 8.0;
+            double bm[3] = {s->bmin.x - irad, s->bmin.y - irad, s->bmin.z - irad};
+            double scale = 1.0 / (2.0 * irad);
+            int32_t S = hash_size > 0 ? hash_size : (int32_t)(nd > 0 ? nd : 1);
+            float* pp = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+            float* pf = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+            float* pn = (float*)malloc(sizeof(float) * 3 * (size_t)(nd ? nd : 1));
+            copy_deposits(dm, pm, nd, nd, pp, pf, pn, NULL);
+            int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nd ? nd : 1));
+            int32_t* cs = (int32_t*)malloc(sizeof(int32_t) * (size_t)S);
+            int32_t* ce = (int32_t*)malloc(sizeof(int32_t) * (size_t)S);
+            orc_hashgrid_points(pp, nd, S, scale, bm, sorted, cs, ce);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tot_acc)
+            for (int64_t i = 0; i < np; i++) {
+                if (kind[i] != EYE_HIT) continue;
+                double rr = r2[i];
+                int64_t n = cnt[i];
+                v3 t = V(tau[3 * i], tau[3 * i + 1], tau[3 * i + 2]);
+                if (n == 0) {
+                    rr = irad * irad;
+                    t = V(0, 0, 0);
+                }
+                double xs[3] = {xp[i].x, xp[i].y, xp[i].z};
+                int32_t cl[3], ch[3];
+                for (int d = 0; d < 3; d++) {
+                    cl[d] = cell_coord(xs[d] - irad, bm[d], scale);
+                    ch[d] = cell_coord(xs[d] + irad, bm[d], scale);
+                    if (ch[d] > cl[d] + 1) ch[d] = cl[d] + 1;
+                }
+                uint32_t seen[8];
+                int ns = 0;
+                for (int32_t iz = cl[2]; iz <= ch[2]; iz++)
+                    for (int32_t iy = cl[1]; iy <= ch[1]; iy++)
+                        for (int32_t ix = cl[0]; ix <= ch[0]; ix++) {
+                            uint32_t h = orc_hash(ix, iy, iz, S);
+                            int dup = 0;
+                            for (int q = 0; q < ns; q++) dup |= seen[q] == h;
+                            if (dup) continue;
+                            seen[ns++] = h;
+                            for (int32_t j = cs[h]; j < ce[h]; j++) {
+                                int32_t id = sorted[j];
+                                v3 pnr = V(pn[3 * id], pn[3 * id + 1], pn[3 * id + 2]);
+                                if (!(dot(pnr, xn[i]) > 0.0)) continue;
+                                v3 df = sub(V(xs[0], xs[1], xs[2]), V(pp[3 * id], pp[3 * id + 1], pp[3 * id + 2]));
+                                if (!(dot(df, df) <= rr)) continue;
+                                double na = (double)n * alpha;
+                                double g = (na + alpha) / (na + 1.0);
+                                rr *= g;
+                                n++;
+                                t = scl(add(t, mul(xw[i], V(pf[3 * id], pf[3 * id + 1], pf[3 * id + 2]))), g);
+                                tot_acc++;
+                            }
+                        }
+                r2[i] = rr;
+                cnt[i] = n;
+                tau[3 * i] = t.x;
+                tau[3 * i + 1] = t.y;
+                tau[3 * i + 2] = t.z;
+            }
+            free(pp); free(pf); free(pn); free(sorted); free(cs); free(ce);
+        }
+        free(dm); free(pm); free(di); free(pi);
+    }
+    for (int64_t i = 0; i < np; i++)
+        for (int c = 0; c < 3; c++) {
+            double tt = cnt[i] > 0 ? tau[3 * i + c] / (PI * r2[i]) : 0.0;
+            image[3 * i + c] = (emis[3 * i + c] + tt) / n_passes;
+        }
+    if (stats) stats[0] = tot_acc;
+    free(r2); free(cnt); free(tau); free(emis); free(kind); free(xp); free(xn); free(xw);
+    return 0;
 }
 
 int32_t orc_num_threads(void) {
@@ -1438,4 +1755,142 @@ void orc_dipole_sum(const double sigma_a[3], const double sigmap_s[3], double et
         }
         for (int c = 0; c < 3; c++) out[3 * i + c] = f * acc[c];
     }
+}
+
+/* Octree over the surface points, FP64 (subsurface_integrator.cc:69-145):
+ * a node's box is the tight box of its points, its children split that box
+ * at the centre, child (x >= mid ? 4 : 0) + (y >= mid ? 2 : 0) + (z >= mid ? 1 : 0) */
+typedef struct {
+    double pos[3], irad[3], area, lo[3], hi[3];
+    int32_t child[8];
+    int32_t leaf;
+} oct_node;
+
+typedef struct {
+    oct_node* n;
+    int64_t len, cap;
+} oct_tree;
+
+static int32_t oct_new(oct_tree* t) {
+    if (t->len == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->n = (oct_node*)realloc(t->n, sizeof(oct_node) * (size_t)t->cap);
+    }
+    memset(&t->n[t->len], 0, sizeof(oct_node));
+    for (int k = 0; k < 8; k++) t->n[t->len].child[k] = -1;
+    return (int32_t)t->len++;
+}
+
+static double luminance3(const double c[3]) { return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]; }
+
+static int32_t oct_build(oct_tree* t, const float* pts, const float* E, int64_t* idx, int64_t m, double area,
+                         int64_t* scratch) {
+    if (m == 0) return -1;
+    double lo[3] = {INFTY, INFTY, INFTY}, hi[3] = {-INFTY, -INFTY, -INFTY};
+    for (int64_t k = 0; k < m; k++)
+        for (int d = 0; d < 3; d++) {
+            double v = pts[3 * idx[k] + d];
+            lo[d] = fmin(lo[d], v);
+            hi[d] = fmax(hi[d], v);
+        }
+    int32_t id = oct_new(t);
+    oct_node* nd = &t->n[id];
+    for (int d = 0; d < 3; d++) {
+        nd->lo[d] = lo[d];
+        nd->hi[d] = hi[d];
+    }
+    if (m == 1 || (lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2])) {
+        /* one point, or points at one position (the reference would split
+         * them forever): a leaf of all of them */
+        nd->leaf = 1;
+        for (int d = 0; d < 3; d++) nd->pos[d] = lo[d];
+        for (int64_t k = 0; k < m; k++)
+            for (int c = 0; c < 3; c++) nd->irad[c] += E[3 * idx[k] + c];
+        for (int c = 0; c < 3; c++) nd->irad[c] /= (double)m;
+        nd->area = area * (double)m;
+        return id;
+    }
+    double mid[3];
+    for (int d = 0; d < 3; d++) mid[d] = (lo[d] + hi[d]) * 0.5;
+    int64_t cnt[8] = {0}, start[8];
+    for (int64_t k = 0; k < m; k++) {
+        const float* v = &pts[3 * idx[k]];
+        int c = (v[0] < mid[0] ? 0 : 4) + (v[1] < mid[1] ? 0 : 2) + (v[2] < mid[2] ? 0 : 1);
+        cnt[c]++;
+    }
+    start[0] = 0;
+    for (int c = 1; c < 8; c++) start[c] = start[c - 1] + cnt[c - 1];
+    int64_t fill[8];
+    memcpy(fill, start, sizeof fill);
+    for (int64_t k = 0; k < m; k++) {
+        const float* v = &pts[3 * idx[k]];
+        int c = (v[0] < mid[0] ? 0 : 4) + (v[1] < mid[1] ? 0 : 2) + (v[2] < mid[2] ? 0 : 1);
+        scratch[fill[c]++] = idx[k];
+    }
+    memcpy(idx, scratch, sizeof(int64_t) * (size_t)m);
+    int32_t ch[8];
+    for (int c = 0; c < 8; c++) ch[c] = oct_build(t, pts, E, idx + start[c], cnt[c], area, scratch);
+    nd = &t->n[id]; /* the array may have moved */
+    double w = 0.0;
+    int n_child = 0;
+    for (int c = 0; c < 8; c++) {
+        nd->child[c] = ch[c];
+        if (ch[c] < 0) continue;
+        const oct_node* cn = &t->n[ch[c]];
+        double lw = luminance3(cn->irad);
+        for (int d = 0; d < 3; d++) {
+            nd->pos[d] += lw * cn->pos[d];
+            nd->irad[d] += cn->irad[d];
+        }
+        nd->area += cn->area;
+        w += lw;
+        n_child++;
+    }
+    if (w > 0.0)
+        for (int d = 0; d < 3; d++) nd->pos[d] /= w;
+    for (int c = 0; c < 3; c++) nd->irad[c] /= (double)n_child;
+    return id;
+}
+
+/* subsurface_integrator.cc:147-167 */
+static void oct_query(const oct_tree* t, int32_t id, const double x[3], const double sa[3], const double ss[3],
+                      double eta, double max_error, int32_t fast, double acc[3], int64_t* visited) {
+    if (id < 0) return;
+    const oct_node* nd = &t->n[id];
+    (*visited)++;
+    double dx = nd->pos[0] - x[0], dy = nd->pos[1] - x[1], dz = nd->pos[2] - x[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    double dw = nd->area / d2;
+    int inside = nd->lo[0] <= x[0] && x[0] <= nd->hi[0] && nd->lo[1] <= x[1] && x[1] <= nd->hi[1] &&
+                 nd->lo[2] <= x[2] && x[2] <= nd->hi[2];
+    if (nd->leaf || (dw < max_error && !inside)) {
+        double rd[3];
+        orc_dipole(sa, ss, eta, d2, fast, rd);
+        for (int c = 0; c < 3; c++) acc[c] += rd[c] * nd->irad[c] * nd->area;
+        return;
+    }
+    for (int c = 0; c < 8; c++) oct_query(t, nd->child[c], x, sa, ss, eta, max_error, fast, acc, visited);
+}
+
+void orc_dipole_octree(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
+                       const float* E, int64_t n_pts, double area, const float* q, int64_t n_q, double max_error,
+                       int32_t fast, double* out, int64_t* visited) {
+    oct_tree t = {NULL, 0, 0};
+    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_pts > 0 ? n_pts : 1));
+    int64_t* scratch = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_pts > 0 ? n_pts : 1));
+    for (int64_t k = 0; k < n_pts; k++) idx[k] = k;
+    int32_t root = oct_build(&t, pts, E, idx, n_pts, area, scratch);
+    double f = (1.0 - orc_dipole_fdr(eta)) / PI;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n_q; i++) {
+        double x[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+        double acc[3] = {0.0, 0.0, 0.0};
+        int64_t v = 0;
+        oct_query(&t, root, x, sigma_a, sigmap_s, eta, max_error, fast, acc, &v);
+        for (int c = 0; c < 3; c++) out[3 * i + c] = f * acc[c];
+        if (visited) visited[i] = v;
+    }
+    free(t.n);
+    free(idx);
+    free(scratch);
 }

@@ -81,6 +81,20 @@ void orc_intersect_batch(const orc_scene* s, const float* rays, int64_t n, int32
                          float* t_out, int32_t* tri_out);
 int32_t orc_qbvh_stats(const orc_scene* s, int64_t* nodes, int64_t* leaves, int32_t* depth);
 
+/* Environment map (sources/envmap.cc): rgb [h][w][3] lat-long, row 0 at +y;
+ * w, h >= 64.  The importance map is the reference's (64 x 64 equal-area
+ * cells, squared luminance of the mean lookup).  Returns 0, or -1 for a map
+ * smaller than 64 x 64. */
+int32_t orc_scene_set_envmap(orc_scene* s, const float* rgb, int32_t w, int32_t h);
+/* The importance cells: pdf [4096] and their running sums cdf [4096] */
+void orc_envmap_cells(const orc_scene* s, double* pdf, double* cdf);
+/* Envmap::sampleFromDir of n directions [n][3] */
+void orc_envmap_lookup(const orc_scene* s, const double* dirs, int64_t n, double* rgb);
+/* The importance-map direction of uniforms r [n][3] (cell pick, u jitter,
+ * v jitter; Envmap::samplePhoton's draws), L(w) / pdf(cell) and the cell. */
+void orc_envmap_sample_dir(const orc_scene* s, const double* r, int64_t n, double* dir, double* l_over_pdf,
+                           int32_t* cell);
+
 /* ---- HashGrid<int> semantics (sources/hash_grid_detail.h) ---- */
 uint32_t orc_hash(int32_t ix, int32_t iy, int32_t iz, int32_t hash_size);
 /* Point insertion: add(i, p_i, p_i) for i = 0..n-1, then the concatenation
@@ -144,6 +158,17 @@ void orc_dipole(const double sigma_a[3], const double sigmap_s[3], double eta, d
 void orc_dipole_sum(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
                     const float* E, int64_t n_pts, double area, const float* q, int64_t n_q, double* out);
 
+/* The same sum over an octree cut (sources/subsurface_integrator.cc:69-167):
+ * a node stands for its points when it is a leaf, or when area / d^2 <
+ * max_error and q lies outside its box; node values follow the reference
+ * (position: luminance-weighted mean of the children's, area: their sum,
+ * irradiance: the mean of the children's).  Points at one position share a
+ * leaf.  fast = 1: the reference's exponential in R_d.  visited (may be
+ * null): nodes visited per query. */
+void orc_dipole_octree(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
+                       const float* E, int64_t n_pts, double area, const float* q, int64_t n_q, double max_error,
+                       int32_t fast, double* out, int64_t* visited);
+
 /* Eye pass: one path per pixel.  For each pixel writes either a hit point
  * (has_hit=1: pos, oriented normal, weight = throughput * rho / pi) or a
  * direct contribution (emission seen through specular chains). */
@@ -191,6 +216,14 @@ int32_t orc_render_ex(const orc_scene* s, const orc_camera* cam, int64_t photons
  * (photon index, bounce) order; returns the count (only cap written). */
 int64_t orc_trace_irradiance_photons(const orc_scene* s, const orc_pass_params* p, float* pos, float* flux,
                                      float* normal, uint32_t* key, int64_t cap);
+
+/* Progressive photon mapping with per-point radii (sources/
+ * progressive_photon_mapping.cc:22-287): render points per pixel, updated
+ * photon by photon in a fixed order.  stats (may be NULL): [0] accepted
+ * photon updates. */
+int32_t orc_render_hachisuka(const orc_scene* s, const orc_camera* cam, int64_t photons, int32_t n_passes,
+                             double alpha, double ray_eps, uint64_t seed, uint32_t scene_id, int32_t hash_size,
+                             double* image, int64_t* stats);
 
 /* Independent FP64 path tracer with next-event estimation to the area
  * lights: image [height][width][3] (top row first) = mean over samples

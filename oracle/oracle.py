@@ -61,6 +61,8 @@ _ODECL = {
     "orc_dipole_fdr": (C.c_double, [C.c_double]),
     "orc_dipole": (None, [_P, _P, C.c_double, C.c_double, C.c_int32, _P]),
     "orc_dipole_sum": (None, [_P, _P, C.c_double, _P, _P, C.c_int64, C.c_double, _P, C.c_int64, _P]),
+    "orc_dipole_octree": (None, [_P, _P, C.c_double, _P, _P, C.c_int64, C.c_double, _P, C.c_int64, C.c_double,
+                                 C.c_int32, _P, _P]),
     "orc_eye_pass": (None, [_P, _P, C.POINTER(PassParams), _P, _P, _P, _P, _P]),
     "orc_gather": (None, [_P, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, C.c_double, C.c_int32, _P, _P, _P]),
     "orc_render": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
@@ -68,6 +70,12 @@ _ODECL = {
     "orc_render_ex": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                   C.c_double, C.c_uint64, C.c_uint32, C.c_int32, _P, _P, _P]),
     "orc_trace_irradiance_photons": (C.c_int64, [_P, C.POINTER(PassParams), _P, _P, _P, _P, C.c_int64]),
+    "orc_scene_set_envmap": (C.c_int32, [_P, _P, C.c_int32, C.c_int32]),
+    "orc_envmap_lookup": (None, [_P, _P, C.c_int64, _P]),
+    "orc_envmap_cells": (None, [_P, _P, _P]),
+    "orc_envmap_sample_dir": (None, [_P, _P, C.c_int64, _P, _P, _P]),
+    "orc_render_hachisuka": (C.c_int32, [_P, _P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_uint64,
+                                         C.c_uint32, C.c_int32, _P, _P]),
     "orc_pathtrace": (None, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_uint32, _P]),
     "orc_num_threads": (C.c_int32, []),
     "orc_on_hemisphere": (None, [_P, C.c_double, C.c_double, _P]),
@@ -88,6 +96,11 @@ _RDECL = {
     "ref_bsdf_sample": (None, [C.c_int32, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
     "ref_camera_ray": (None, [_P, _P, _P, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P]),
     "ref_dipole": (None, [_P, _P, C.c_double, C.c_double, _P, _P]),
+    "ref_dipole_octree": (None, [_P, _P, C.c_double, _P, _P, C.c_int64, C.c_double, _P, C.c_int64, C.c_double, _P]),
+    "ref_image_save": (None, [C.c_char_p, _P, C.c_int32, C.c_int32]),
+    "ref_image_load": (C.c_int64, [C.c_char_p, _P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ref_envmap_sample": (None, [C.c_char_p, _P, C.c_int64, C.c_int32, _P, _P, _P]),
+    "ref_envmap_lookup": (None, [C.c_char_p, _P, C.c_int64, _P]),
     "ref_trimesh_place": (C.c_int64, [C.c_char_p, C.c_int32, _P, C.c_double, C.c_double, _P, _P, _P, C.c_int64]),
 }
 
@@ -145,15 +158,39 @@ class OracleScene:
         self._tri = np.ascontiguousarray(scene.tri, np.float32)
         self._ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
         self._bsdf = scene.bsdf_array()
-        self._lights = scene.light_array()
+        self._lights = scene.light_array() if scene.lights else None
         self.h = orc().orc_scene_create(_ptr(self._tri), _ptr(self._ids), len(self._ids),
                                         C.cast(self._bsdf, C.c_void_p), len(scene.bsdfs),
-                                        C.cast(self._lights, C.c_void_p), len(scene.lights))
+                                        C.cast(self._lights, C.c_void_p) if self._lights is not None else None,
+                                        len(scene.lights))
 
     def __del__(self):
         if getattr(self, "h", None):
             orc().orc_scene_destroy(self.h)
             self.h = None
+
+    def set_envmap(self, rgb):
+        """Lat-long environment [h][w][3] (row 0 at +y), h, w >= 64."""
+        self._env = np.ascontiguousarray(rgb, np.float32)
+        rc = orc().orc_scene_set_envmap(self.h, _ptr(self._env), self._env.shape[1], self._env.shape[0])
+        assert rc == 0
+
+    def envmap_cells(self):
+        pdf, cdf = np.zeros(4096), np.zeros(4096)
+        orc().orc_envmap_cells(self.h, _ptr(pdf), _ptr(cdf))
+        return pdf, cdf
+
+    def envmap_lookup(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        out = np.zeros_like(d)
+        orc().orc_envmap_lookup(self.h, _ptr(d), len(d), _ptr(out))
+        return out
+
+    def envmap_sample_dir(self, r):
+        r = np.ascontiguousarray(r, np.float64).reshape(-1, 3)
+        d, w, c = np.zeros_like(r), np.zeros_like(r), np.zeros(len(r), np.int32)
+        orc().orc_envmap_sample_dir(self.h, _ptr(r), len(r), _ptr(d), _ptr(w), _ptr(c))
+        return d, w, c
 
     def intersect(self, rays, brute=False):
         rays = np.ascontiguousarray(rays)
@@ -222,6 +259,14 @@ class OracleScene:
                                  seed, scene_id, hash_size, C.byref(opts), _ptr(img), _ptr(stats))
         assert rc == 0
         return img, stats
+
+    def render_hachisuka(self, cam, photons, passes, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0, hash_size=0):
+        img = np.zeros((cam.height, cam.width, 3), np.float64)
+        st = np.zeros(1, np.int64)
+        rc = orc().orc_render_hachisuka(self.h, C.byref(cam), photons, passes, alpha, ray_eps, seed, scene_id,
+                                        hash_size, _ptr(img), _ptr(st))
+        assert rc == 0
+        return img, st
 
     def pathtrace(self, cam, spp, first_sample=0, max_bounces=64, ray_eps=1e-4, seed=0x7A75, scene_id=0):
         """The independent next-event path tracer: mean radiance per pixel."""
@@ -314,3 +359,18 @@ def dipole_sum(sigma_a, sigmap_s, eta, pts, E, area, query):
     out = np.zeros((len(q), 3))
     orc().orc_dipole_sum(_ptr(sa), _ptr(ss), eta, _ptr(p), _ptr(e), len(p), area, _ptr(q), len(q), _ptr(out))
     return out
+
+
+def dipole_octree(sigma_a, sigmap_s, eta, pts, E, area, query, max_error=0.05, fast=False):
+    """The dipole sum over the octree cut; returns (out [n_q][3], visited [n_q]).
+    fast: the reference's exponential in R_d."""
+    sa = np.ascontiguousarray(sigma_a, np.float64)
+    ss = np.ascontiguousarray(sigmap_s, np.float64)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(E, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(query, np.float32).reshape(-1, 3)
+    out = np.zeros((len(q), 3))
+    vis = np.zeros(max(len(q), 1), np.int64)
+    orc().orc_dipole_octree(_ptr(sa), _ptr(ss), eta, _ptr(p), _ptr(e), len(p), area, _ptr(q), len(q), max_error,
+                            1 if fast else 0, _ptr(out), _ptr(vis))
+    return out, vis[:len(q)]

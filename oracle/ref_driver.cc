@@ -19,7 +19,28 @@
 #include "sampler.h"
 #include "triangle.h"
 #include "trimesh.h"
+#include "image.h"
+#include "envmap.h"
+#include "random_sequence.h"
 #include "plane.h"
+
+// SubsurfaceIntegrator's octree (subsurface_integrator.cc:69-167) is a private
+// member type: the driver reads that one header with private members opened
+// (every standard header it pulls in is included above first).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#define private public
+#include "subsurface_integrator.h"
+#undef private
 
 void* enabler = nullptr;  // declared extern in common.h:41
 
@@ -191,6 +212,91 @@ int64_t ref_trimesh_place(const char* path, int32_t do_place, const double box[6
         for (int k = 0; k < 3; k++)
             for (int d = 0; d < 3; d++) tris[9 * i + 3 * k + d] = t[i].p(k)[d];
     return (int64_t)t.size();
+}
+
+// Image::save of an FP64 [h][w][3] buffer (top row first) by extension
+// (.png / .bmp through the reference's vendored writer, .hdr its own RGBE).
+void ref_image_save(const char* path, const double* rgb, int32_t w, int32_t h) {
+    Image img(w, h);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const double* p = rgb + 3 * ((size_t)y * w + x);
+            img.pixel(x, y) = Vector3D(p[0], p[1], p[2]);
+        }
+    img.save(path);
+}
+
+// Image::load (.hdr or .bmp) into out [h][w][3]; returns w * h, -1 above cap.
+int64_t ref_image_load(const char* path, double* out, int64_t cap, int32_t* w, int32_t* h) {
+    Image img;
+    img.load(path);
+    *w = (int32_t)img.width();
+    *h = (int32_t)img.height();
+    const int64_t n = (int64_t)img.width() * img.height();
+    if (n > cap) return -1;
+    for (int y = 0; y < (int)img.height(); y++)
+        for (int x = 0; x < (int)img.width(); x++) {
+            const Vector3D& c = img(x, y);
+            double* p = out + 3 * ((size_t)y * img.width() + x);
+            p[0] = c.x();
+            p[1] = c.y();
+            p[2] = c.z();
+        }
+    return n;
+}
+
+// Envmap(path): samplePhoton with the given uniforms (r[3 k .. 3 k + 2]: CDF
+// pick, u jitter, v jitter) for k < n, photon count N: position, flux,
+// direction; and sampleFromDir of the given directions.
+void ref_envmap_sample(const char* path, const double* r, int64_t n, int32_t N, double* pos, double* flux,
+                       double* dir) {
+    Envmap env(path);
+    for (int64_t k = 0; k < n; k++) {
+        RandomSequence rs;
+        rs.resize(3);
+        for (int j = 0; j < 3; j++) rs.set(j, r[3 * k + j]);
+        const Photon ph = env.samplePhoton(rs, N);
+        for (int d = 0; d < 3; d++) {
+            pos[3 * k + d] = ph[d];
+            flux[3 * k + d] = ph.flux()[d];
+            dir[3 * k + d] = ph.direction()[d];
+        }
+    }
+}
+
+void ref_envmap_lookup(const char* path, const double* dirs, int64_t n, double* rgb) {
+    Envmap env(path);
+    for (int64_t k = 0; k < n; k++) {
+        const Vector3D c = env.sampleFromDir(Vector3D(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        rgb[3 * k] = c.x();
+        rgb[3 * k + 1] = c.y();
+        rgb[3 * k + 2] = c.z();
+    }
+}
+
+// SubsurfaceIntegrator::Octree::construct + iradSubsurface with the reference's
+// maximum error, and irradiance()'s factor (subsurface_integrator.cc:234-238):
+// out[i] = (1 - F_dr) / pi * M_o(q_i) for points of equal area
+void ref_dipole_octree(const double sigma_a[3], const double sigmap_s[3], double eta, const float* pts,
+                       const float* E, int64_t n, double area, const float* q, int64_t nq, double max_error,
+                       double* out) {
+    SubsurfaceIntegrator integ;
+    integ._maxError = max_error;
+    std::vector<IrradiancePoint> ip((size_t)n);
+    for (int64_t i = 0; i < n; i++)
+        ip[(size_t)i] = IrradiancePoint(Vector3D(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), Vector3D(0.0, 0.0, 1.0),
+                                        area, Vector3D(E[3 * i], E[3 * i + 1], E[3 * i + 2]));
+    SubsurfaceIntegrator::Octree oct;
+    oct.construct(&integ, ip);
+    BSSRDF b = DipoleBSSRDF::factory(Vector3D(sigma_a[0], sigma_a[1], sigma_a[2]),
+                                     Vector3D(sigmap_s[0], sigmap_s[1], sigmap_s[2]), eta);
+    for (int64_t i = 0; i < nq; i++) {
+        const Vector3D Mo = oct.iradSubsurface(Vector3D(q[3 * i], q[3 * i + 1], q[3 * i + 2]), b);
+        const Vector3D L = (1.0 / PI) * (1.0 - b.Fdr()) * Mo;
+        out[3 * i] = L.x();
+        out[3 * i + 1] = L.y();
+        out[3 * i + 2] = L.z();
+    }
 }
 
 }  // extern "C"

@@ -198,6 +198,8 @@ int pppm_batch_add(void* batch, const float* tri_xyz, const int32_t* bsdf_id, in
     Batch* b = reinterpret_cast<Batch*>(batch);
     if (!b) return PPPM_ERR_INVALID;
     if (!params || params->hash_size <= 0) return b->fail(PPPM_ERR_INVALID, "batched scenes need a fixed hash_size > 0");
+    if (params->method != PPPM_METHOD_PROBABILISTIC)
+        return b->fail(PPPM_ERR_INVALID, "batched scenes render with the probabilistic method");
     for (int32_t i = 0; i < nb; i++)
         if (table && (table[i].type & PPPM_BSDF_BSSRDF))
             return b->fail(PPPM_ERR_INVALID, "batched scenes are plain PPM (no BSSRDF)");

@@ -6,6 +6,7 @@
 // scattering in sss_host.hip, the learned proposal in proposal_host.hip and
 // batched scenes in batch.hip.
 #include "ctx.h"
+#include "../host/image_io.h"
 
 using namespace pppmi;
 
@@ -235,8 +236,8 @@ int pppm_device_info(void* ctx_raw, char* name, int32_t name_len, int32_t* cu_co
 int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_id, int64_t n,
                       const pppm_bsdf* table, int32_t nb, const pppm_light* lights, int32_t nl) {
     CHECK_CTX(ctx);
-    if (n < 0 || (n > 0 && (!tri_xyz || !bsdf_id)) || nb <= 0 || !table || nl <= 0 || !lights)
-        return ctx->fail(PPPM_ERR_INVALID, "scene needs triangles, bsdf ids, a bsdf table and >= 1 light");
+    if (n < 0 || (n > 0 && (!tri_xyz || !bsdf_id)) || nb <= 0 || !table || nl < 0 || (nl > 0 && !lights))
+        return ctx->fail(PPPM_ERR_INVALID, "scene needs triangles, bsdf ids and a bsdf table");
     for (int32_t i = 0; i < nb; i++) {
         int t = table[i].type;
         if (t & PPPM_BSDF_PHONG) return ctx->fail(PPPM_ERR_INVALID, "Phong BRDF is not supported");
@@ -264,6 +265,7 @@ int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_i
     }
     ctx->h_bsdf.assign(table, table + nb);
     ctx->h_lights.assign(lights, lights + nl);
+    ctx->has_env = false;  // an environment map belongs to the scene: set it after the upload
     std::vector<Mat> m(nb);
     for (int32_t i = 0; i < nb; i++) {
         const pppm_bsdf& b = table[i];
@@ -525,6 +527,13 @@ int pppm_image_read(void* ctx_raw, double* rgb, int64_t n_values) {
     const int64_t nv = 3 * (int64_t)ctx->cam.width * ctx->cam.height;
     if (!rgb || n_values < nv) return ctx->fail(PPPM_ERR_INVALID, "image buffer too small");
     (void)hipSetDevice(ctx->device);
+    if (ctx->rp.method == PPPM_METHOD_HACHISUKA) {
+        TRY_HIP(ctx, ctx->ppm_image.ensure(sizeof(double) * (size_t)nv));
+        TRY_HIP(ctx, hachisuka_image(ctx, ctx->ppm_image.as<double>()));
+        TRY_HIP(ctx, hipMemcpyAsync(rgb, ctx->ppm_image.p, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
+        TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return PPPM_OK;
+    }
     std::vector<float> s((size_t)nv), c((size_t)nv), ds((size_t)nv), dc((size_t)nv);
     TRY_HIP(ctx, hipStreamSynchronize(ctx->stream2));
     TRY_HIP(ctx, hipMemcpyAsync(s.data(), ctx->img_sum.p, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost, ctx->stream));
@@ -538,6 +547,46 @@ int pppm_image_read(void* ctx_raw, double* rgb, int64_t n_values) {
         const size_t k = (size_t)i;
         rgb[i] = (((double)s[k] - (double)c[k]) + ((double)ds[k] - (double)dc[k])) * inv;
     }
+    return PPPM_OK;
+}
+
+int pppm_image_save(void* ctx_raw, const char* path, double gamma) {
+    CHECK_CTX(ctx);
+    if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "no render in progress");
+    if (!path) return ctx->fail(PPPM_ERR_INVALID, "no path");
+    std::string ext(path);
+    const size_t dot = ext.find_last_of('.');
+    ext = dot == std::string::npos ? "" : ext.substr(dot);
+    for (char& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    const int fmt = (ext == ".png" || ext == ".bmp") ? 0 : ext == ".hdr" ? 1 : ext == ".pfm" ? 2 : -1;
+    if (fmt < 0) return ctx->fail(PPPM_ERR_INVALID, "images are saved as .png, .bmp, .hdr or .pfm");
+    (void)hipSetDevice(ctx->device);
+    const int32_t w = ctx->cam.width, h = ctx->cam.height;
+    const int64_t np = (int64_t)w * h;
+    if (ctx->rp.method == PPPM_METHOD_HACHISUKA) {  // per-point radii: the FP64 image, encoded on the host
+        std::vector<double> img(3 * (size_t)np);
+        const int rc = pppm_image_read(ctx_raw, img.data(), (int64_t)img.size());
+        if (rc != PPPM_OK) return rc;
+        const int wr = pppm_image_write(path, img.data(), w, h, gamma);
+        return wr == PPPM_OK ? PPPM_OK : ctx->fail(wr, std::string("cannot write ") + path);
+    }
+    const size_t bytes = fmt == 0 ? 3 * (size_t)np : 4 * (size_t)np;  // RGB8, RGBE or 3 x FP32 / 3 bytes
+    const size_t out_bytes = fmt == 2 ? 12 * (size_t)np : bytes;
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream2));
+    TRY_HIP(ctx, ctx->scratch.ensure(out_bytes));
+    const double inv = ctx->passes_done > 0 ? 1.0 / (double)ctx->passes_done : 0.0;
+    launch_image_encode(ctx->stream, ctx->img_sum.as<float>(), ctx->img_comp.as<float>(), ctx->img_dsum.as<float>(),
+                        ctx->img_dcomp.as<float>(), np, inv, fmt, gamma > 0.0 ? 1.0 / gamma : 0.0,
+                        ctx->scratch.as<uint8_t>(), ctx->scratch.as<float>());
+    TRY_HIP(ctx, hipGetLastError());
+    std::vector<unsigned char> host(out_bytes);
+    TRY_HIP(ctx, hipMemcpyAsync(host.data(), ctx->scratch.p, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc;
+    if (fmt == 0) rc = ext == ".png" ? image_write_png8(path, host.data(), w, h) : image_write_bmp8(path, host.data(), w, h);
+    else if (fmt == 1) rc = image_write_rgbe(path, host.data(), w, h);
+    else rc = image_write_pfm(path, reinterpret_cast<const float*>(host.data()), w, h);
+    if (rc != PPPM_OK) return ctx->fail(rc, std::string("cannot write ") + path);
     return PPPM_OK;
 }
 

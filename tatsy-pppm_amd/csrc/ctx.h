@@ -138,6 +138,17 @@ struct Ctx {
     DBuf mlp_params, mlp_m, mlp_v, mlp_wb, mlp_part, mlp_act, mlp_idx, mlp_wsum, mlp_loss, mlp_stage,
         mlp_grad, prop_ring, prop_prob, prop_cdf, prop_uv, prop_flags, prop_used, prop_segc, prop_scan, prop_bsums;
 
+    // environment map of the scene (pppm_envmap_set): texels, importance cells,
+    // and the per-render emission constants (render_begin)
+    bool has_env = false;
+    int32_t env_w = 0, env_h = 0, env_last = 0;
+    std::vector<float> h_env;          // [h][w][3]
+    std::vector<double> h_env_pdf, h_env_cdf;
+    DBuf env_tex, env_pdf, env_cdf;
+    double env_k = 0.0, pick_total = 0.0, pick_area = 0.0;
+    // render points of the Hachisuka method: radius^2, photon count, flux
+    DBuf ppm_r2, ppm_n, ppm_flux, ppm_bounds, ppm_image;
+
     int fail(int code, const std::string& msg) {
         err = msg;
         return code;
@@ -153,6 +164,20 @@ struct Ctx {
         s.root = root;
         s.n_nodes = (int32_t)binfo.nodes;
         s.n_tris = n_tris;
+        s.env = has_env ? env_tex.as<float4>() : nullptr;
+        s.env_pdf = env_pdf.as<double>();
+        s.env_cdf = env_cdf.as<double>();
+        s.env_w = env_w;
+        s.env_h = env_h;
+        s.env_last = env_last;
+        s.env_c = mk((float)(0.5 * (bmin[0] + bmax[0])), (float)(0.5 * (bmin[1] + bmax[1])),
+                     (float)(0.5 * (bmin[2] + bmax[2])));
+        s.env_r = (float)(0.5 * std::sqrt((bmax[0] - bmin[0]) * (bmax[0] - bmin[0]) +
+                                          (bmax[1] - bmin[1]) * (bmax[1] - bmin[1]) +
+                                          (bmax[2] - bmin[2]) * (bmax[2] - bmin[2])));
+        s.env_k = env_k;
+        s.pick_total = pick_total;
+        s.pick_area = pick_area;
         return s;
     }
 };
@@ -204,6 +229,17 @@ double dipole_fdr(double eta);
 uint32_t morton_bits(uint32_t v);
 hipError_t subsurface_setup(Ctx* c, const float* tri_h /* may be null: read back */);
 hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius);
+// environment map (envmap_host.hip): the importance cells of Envmap::
+// createImportanceMap (FP64, bit for bit), and the emission constants of a
+// render with N photons per pass (area-light power total in, pick and disc
+// constants out)
+void envmap_importance(const float* rgb, int32_t w, int32_t h, double* pdf, double* cdf, int32_t* last);
+double envmap_power(const Ctx* c);
+// the Hachisuka method (hachisuka.hip): render-point state, one pass (an
+// error message, empty on success), the FP64 image into device memory
+hipError_t hachisuka_prepare(Ctx* c);
+std::string hachisuka_pass(Ctx* c, pppm_pass_stats* st);
+hipError_t hachisuka_image(Ctx* c, double* out_dev);
 // learned emission proposal (proposal_host.hip)
 uint64_t splitmix64(uint64_t* st);
 std::vector<float> mlp_init_params(uint64_t seed);

@@ -176,12 +176,16 @@ struct EyePassArgs {
     float4* sss_w;
     int64_t sss_cap;
     int32_t count_traversal = 0;  // 1: the kernel that also counts nodes and triangles
+    int32_t roulette = 1;         // 0: no eye Russian roulette (the Hachisuka method)
 };
 // grid_limit: at most this many blocks (a grid-stride loop covers the rest)
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit);
 // img += the direct emission stored in the pixel slots without hit point
 // (an eye pass run with img_sum = null)
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp);
+// FP64 image average encoded per pixel (fmt 0: RGB8 after c^ginv, 1: RGBE, 2: FP32 RGB)
+void launch_image_encode(hipStream_t s, const float* sum, const float* comp, const float* dsum, const float* dcomp,
+                         int64_t npix, double inv, int fmt, double ginv, uint8_t* out8, float* outf);
 // lane k < lanes (<= 64) Kahan-adds v[(i + k) % n], i = 0 .. n-1, into sum[k], comp[k]
 void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, float* sum, float* comp);
 
@@ -270,7 +274,7 @@ void launch_pack_points(hipStream_t s, const float* pos3, const float* a3, const
 #define MLP_IN 32
 #define MLP_HID 64
 #define MLP_OUT 1024
-#define MLP_SPLITS 16  // K chunks of the weight-gradient products
+#define MLP_SPLITS 8  // K chunks of the weight-gradient products (batch / 8: a multiple of 32)
 // FP32 master parameters: W1 [64][32] b1, W2 [64][64] b2, W3 [64][64] b3, W4 [1024][64] b4
 #define OW1 0
 #define OB1 2048

@@ -85,8 +85,11 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     if (!ctx->has_cam) return ctx->fail(PPPM_ERR_STATE, "no camera set");
     if (!p || p->photons <= 0 || p->passes <= 0 || !(p->alpha > 0.0 && p->alpha < 1.0) ||
         !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0 ||
-        (p->gather_mode < PPPM_GATHER_AUTO || p->gather_mode > PPPM_GATHER_WAVE))
+        (p->gather_mode < PPPM_GATHER_AUTO || p->gather_mode > PPPM_GATHER_WAVE) ||
+        (p->method != PPPM_METHOD_PROBABILISTIC && p->method != PPPM_METHOD_HACHISUKA))
         return ctx->fail(PPPM_ERR_INVALID, "bad render parameters");
+    if (p->method == PPPM_METHOD_HACHISUKA && ctx->n_dips > 0)
+        return ctx->fail(PPPM_ERR_INVALID, "the Hachisuka method takes no BSSRDF materials");
     if (p->photons > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "at most 2^26 photons per pass");
     (void)hipSetDevice(ctx->device);
     ctx->rp = *p;
@@ -111,7 +114,18 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
         power[l] = M_PI * area[l] * lum;
         total += power[l];
     }
+    // the environment map competes for photons by its power through the disc
+    const double area_total = total;
+    const double env_power = ctx->has_env ? envmap_power(ctx) : 0.0;
+    total += env_power;
     if (!(total > 0.0)) return ctx->fail(PPPM_ERR_INVALID, "lights emit no power");
+    ctx->pick_total = total;
+    ctx->pick_area = area_total;
+    if (ctx->has_env) {
+        const double dx = ctx->bmax[0] - ctx->bmin[0], dy = ctx->bmax[1] - ctx->bmin[1], dz = ctx->bmax[2] - ctx->bmin[2];
+        const double R = 0.5 * std::sqrt(dx * dx + dy * dy + dz * dz);
+        ctx->env_k = M_PI * R * R * (4.0 * M_PI / (ENV_CELLS * ENV_CELLS)) / ((double)p->photons * (env_power / total));
+    }
     std::vector<LightDev> ld(nl);
     double cum = 0.0;
     for (int l = 0; l < nl; l++) {
@@ -129,7 +143,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
         double k = M_PI * area[l] / (pl * (double)p->photons);
         o.flux = mk((float)(L.emission[0] * k), (float)(L.emission[1] * k), (float)(L.emission[2] * k));
         cum += power[l];
-        o.cdf_hi = (l == nl - 1) ? 2.0f : (float)(cum / total);
+        o.cdf_hi = (l == nl - 1 && !ctx->has_env) ? 2.0f : (float)(cum / total);
     }
     TRY_HIP(ctx, hipMemcpy(ctx->lights.p, ld.data(), sizeof(LightDev) * (size_t)nl, hipMemcpyHostToDevice));
     const int64_t npix = (int64_t)ctx->cam.width * ctx->cam.height;
@@ -166,6 +180,10 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
         TRY_HIP(ctx, ctx->sss_pos.ensure(sizeof(float4) * (size_t)npix));
         TRY_HIP(ctx, ctx->sss_w.ensure(sizeof(float4) * (size_t)npix));
     }
+    if (p->method == PPPM_METHOD_HACHISUKA) {
+        TRY_HIP(ctx, hachisuka_prepare(ctx));
+        TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     ctx->passes_done = 0;
     ctx->rendering = true;
     ctx->last_valid = false;
@@ -179,6 +197,12 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "pppm_render_begin not called");
     if (ctx->passes_done >= ctx->rp.passes) return ctx->fail(PPPM_ERR_STATE, "all passes done");
     (void)hipSetDevice(ctx->device);
+    if (ctx->rp.method == PPPM_METHOD_HACHISUKA) {
+        const std::string e = hachisuka_pass(ctx, st);
+        if (!e.empty()) return ctx->fail(e.find("photon buffer") == 0 ? PPPM_ERR_CAPACITY : PPPM_ERR_HIP, e);
+        ctx->passes_done++;
+        return PPPM_OK;
+    }
     hipStream_t s = ctx->stream;
     const int32_t pass = ctx->passes_done + 1;
     const double radius = ctx->radii[(size_t)pass - 1];
@@ -460,7 +484,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     if (count < 0 || ctx->passes_done + count > ctx->rp.passes)
         return ctx->fail(PPPM_ERR_STATE, "more passes than the render was begun with");
     if (count == 0) return PPPM_OK;
-    if (ctx->prop.enabled || ctx->prop.active || ctx->sss_present) {
+    if (ctx->prop.enabled || ctx->prop.active || ctx->sss_present || ctx->rp.method == PPPM_METHOD_HACHISUKA) {
         // the proposal and the subsurface estimate work between passes on the
         // host: one pass at a time
         for (int32_t k = 0; k < count; k++) {

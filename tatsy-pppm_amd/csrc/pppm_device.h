@@ -171,7 +171,36 @@ struct SceneDev {
     int32_t root;            // root ref (inner index 0, a leaf ref, or EMPTY_REF)
     int32_t n_nodes;         // inner nodes (their bytes fit a 32-bit buffer offset)
     int64_t n_tris;
+    // environment map (env null: black).  Lat-long texels, row 0 at +y; the
+    // importance map's 64 x 64 equal-area cells with their probabilities and
+    // running sums (FP64, as the host built them).
+    const float4* env;
+    const double* env_pdf;
+    const double* env_cdf;
+    int32_t env_w, env_h, env_last;
+    float env_r;             // radius of the emission disc (the triangles' bounding sphere)
+    v3f env_c;               // its centre
+    double env_k;            // pi R^2 (4 pi / 4096) / (N P_env)
+    double pick_total;       // emitted power of area lights + environment
+    double pick_area;        // of the area lights: a photon whose light uniform u has u * total >= this
 };
+
+#define ENV_CELLS 64
+
+// Envmap::sampleFromDir (sources/envmap.cc:47-60) in FP64: the texel of
+// direction d, theta = acos(y) down the rows, phi = atan2(z, x) in [0, 2 pi)
+// across the columns.
+__device__ inline v3f env_lookup(const SceneDev& sc, double dx, double dy, double dz) {
+    const double theta = acos(fmin(1.0, fmax(-1.0, dy)));
+    double phi = atan2(dz, dx);
+    if (phi < 0.0) phi += 2.0 * 3.14159265358979323846;
+    int ix = (int)(phi / (2.0 * 3.14159265358979323846) * sc.env_w);
+    int iy = (int)(theta / 3.14159265358979323846 * sc.env_h);
+    ix = ix < 0 ? 0 : (ix > sc.env_w - 1 ? sc.env_w - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > sc.env_h - 1 ? sc.env_h - 1 : iy);
+    const float4 t = sc.env[(int64_t)iy * sc.env_w + ix];
+    return mk(t.x, t.y, t.z);
+}
 
 struct Hit {
     float t;

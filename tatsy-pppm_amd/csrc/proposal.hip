@@ -12,8 +12,8 @@
 //    those deposits.  Records are written in
 //    photon order (a prefix scan, no atomics) into a ring.
 //  * MLP: Fourier encoding of (u, v) (32) -> 64 -> 64 -> 64 (ReLU) -> 1024
-//    logits, BF16 weights and activations on v_mfma_f32_16x16x16_bf16 with
-//    FP32 accumulation; training minimises the contribution-weighted cross
+//    logits, BF16 weights and activations on gfx950's v_mfma_f32_16x16x32_bf16
+//    with FP32 accumulation; training minimises the contribution-weighted cross
 //    entropy of the record's cell (the KL divergence to the contribution
 //    density); Adam in FP32.  Weight gradients are split-K partial sums
 //    reduced in a fixed order, so training is deterministic.
@@ -23,7 +23,7 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -33,12 +33,21 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// D += A B for one 16 x 16 x 16 tile.  Lane l = r + 16 g holds A[r][4g .. 4g+3],
-// B[4g .. 4g+3][r] and D[4g + i][r] (i = 0..3).
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// D += A B for one 16 x 16 x 32 tile.  Lane l = r + 16 g holds A[r][8g .. 8g+7],
+// B[8g .. 8g+7][r] and D[4g + i][r] (i = 0..3).
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *(const bf16x4*)p; }
+// eight BF16 values from a 16-byte aligned address
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
+__device__ __forceinline__ float bf16_sum8(bf16x8 v) {
+    const uint4 u = __builtin_bit_cast(uint4, v);
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) s += __uint_as_float(w[i] << 16) + __uint_as_float(w[i] & 0xFFFF0000u);
+    return s;
+}
 
 // four consecutive samples of feature row `row` of a feature-major [F][n] array
 __device__ __forceinline__ void st4(uint16_t* base, int64_t n, int row, int64_t s, const uint16_t v[4]) {
@@ -48,9 +57,9 @@ __device__ __forceinline__ void st4(uint16_t* base, int64_t n, int row, int64_t 
     *(uint2*)(base + (int64_t)row * n + s) = w;
 }
 
-#define LDX 36  // LDS row strides in bf16 (padded against bank conflicts)
-#define LDH 68
-#define LDG 20
+#define LDX 40  // LDS row strides in bf16: 16-byte aligned rows, padded against bank conflicts
+#define LDH 72
+#define LDG 40
 
 // Fourier feature k of coordinate x: k = 2q (sin) or 2q + 1 (cos) of pi 2^q x,
 // with the argument reduced exactly in revolutions
@@ -65,14 +74,14 @@ template <int K>
 __device__ __forceinline__ void dense_relu(const uint16_t* A, int lda, const uint16_t* W, const float* b,
                                            uint16_t* H, uint16_t* act, int64_t n, int act_row, int64_t s0) {
     const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
-    bf16x4 av[K / 16];
+    bf16x8 av[K / 32];
 #pragma unroll
-    for (int kt = 0; kt < K / 16; kt++) av[kt] = ld4(A + r * lda + kt * 16 + 4 * g);
+    for (int kt = 0; kt < K / 32; kt++) av[kt] = ld8(A + r * lda + kt * 32 + 8 * g);
 #pragma unroll
     for (int nt = 0; nt < MLP_HID / 16; nt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kt = 0; kt < K / 16; kt++) acc = mfma16(av[kt], ld4(W + (nt * 16 + r) * K + kt * 16 + 4 * g), acc);
+        for (int kt = 0; kt < K / 32; kt++) acc = mfma32(av[kt], ld8(W + (nt * 16 + r) * K + kt * 32 + 8 * g), acc);
         const int col = nt * 16 + r;
         const float bc = b[col];
         uint16_t hv[4];
@@ -89,14 +98,14 @@ __device__ __forceinline__ void dense_relu(const uint16_t* A, int lda, const uin
 __device__ __forceinline__ void dense_back(const uint16_t* Din, const uint16_t* WT, const uint16_t* H,
                                            uint16_t* Dout, uint16_t* act, int64_t n, int act_row, int64_t s0) {
     const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
-    bf16x4 av[4];
+    bf16x8 av[2];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) av[kt] = ld4(Din + r * LDH + kt * 16 + 4 * g);
+    for (int kt = 0; kt < 2; kt++) av[kt] = ld8(Din + r * LDH + kt * 32 + 8 * g);
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) acc = mfma16(av[kt], ld4(WT + (nt * 16 + r) * 64 + kt * 16 + 4 * g), acc);
+        for (int kt = 0; kt < 2; kt++) acc = mfma32(av[kt], ld8(WT + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
         const int col = nt * 16 + r;
         uint16_t dv[4];
 #pragma unroll
@@ -113,10 +122,10 @@ __device__ __forceinline__ void dense_back(const uint16_t* Din, const uint16_t* 
 // activation gradients (TRAIN).  256 threads = 4 waves x 16 samples.
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
-    __shared__ uint16_t sX[4][16 * LDX];
-    __shared__ uint16_t sH[4][3][16 * LDH];
-    __shared__ uint16_t sD[4][2][16 * LDH];
-    __shared__ uint16_t sG[4][16 * LDG];
+    __shared__ __align__(16) uint16_t sX[4][16 * LDX];
+    __shared__ __align__(16) uint16_t sH[4][3][16 * LDH];
+    __shared__ __align__(16) uint16_t sD[4][2][16 * LDH];
+    __shared__ __align__(16) uint16_t sG[4][16 * LDG];
     __shared__ float sW[4][16];
     __shared__ int32_t sT[4][16];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
@@ -178,9 +187,9 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     __syncthreads();
 
     // output layer, pass 1: per-row running max and sum of exponentials
-    bf16x4 h3[4];
+    bf16x8 h3[2];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) h3[kt] = ld4(H3 + r * LDH + kt * 16 + 4 * g);
+    for (int kt = 0; kt < 2; kt++) h3[kt] = ld8(H3 + r * LDH + kt * 32 + 8 * g);
     const uint16_t* W4 = WB + MW4;
     const float* b4 = P + OB4;
     float M[4], S[4];
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     for (int nt = 0; nt < MLP_OUT / 16; nt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) acc = mfma16(h3[kt], ld4(W4 + (nt * 16 + r) * 64 + kt * 16 + 4 * g), acc);
+        for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
         const float bc = b4[nt * 16 + r];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -223,7 +232,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         for (int nt = 0; nt < MLP_OUT / 16; nt++) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int kt = 0; kt < 4; kt++) acc = mfma16(h3[kt], ld4(W4 + (nt * 16 + r) * 64 + kt * 16 + 4 * g), acc);
+            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
             const float bc = b4[nt * 16 + r];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -248,27 +257,31 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         tg[i] = sT[w][4 * g + i];
     }
     float lpart = 0.0f;
-    for (int nt = 0; nt < MLP_OUT / 16; nt++) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int nt2 = 0; nt2 < MLP_OUT / 32; nt2++) {  // 32 output columns: one K step of dH3
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) acc = mfma16(h3[kt], ld4(W4 + (nt * 16 + r) * 64 + kt * 16 + 4 * g), acc);
-        const int col = nt * 16 + r;
-        const float bc = b4[col];
-        uint16_t gv[4];
+        for (int half = 0; half < 2; half++) {
+            const int nt = 2 * nt2 + half;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float z = acc[i] + bc;
-            const float p = __expf(z - M[i]) * invS[i];
-            const bool hit = col == tg[i];
-            if (hit) lpart += wt[i] * (M[i] + __logf(S[i]) - z);
-            gv[i] = to_bf16(wt[i] * (p - (hit ? 1.0f : 0.0f)));
-            G[(4 * g + i) * LDG + r] = gv[i];
+            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
+            const int col = nt * 16 + r;
+            const float bc = b4[col];
+            uint16_t gv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = acc[i] + bc;
+                const float p = __expf(z - M[i]) * invS[i];
+                const bool hit = col == tg[i];
+                if (hit) lpart += wt[i] * (M[i] + __logf(S[i]) - z);
+                gv[i] = to_bf16(wt[i] * (p - (hit ? 1.0f : 0.0f)));
+                G[(4 * g + i) * LDG + half * 16 + r] = gv[i];
+            }
+            st4(act, n, ACT_DZ + col, s0 + 4 * g, gv);
         }
-        st4(act, n, ACT_DZ + col, s0 + 4 * g, gv);
         __syncthreads();
-        const bf16x4 ga = ld4(G + r * LDG + 4 * g);
+        const bf16x8 ga = ld8(G + r * LDG + 8 * g);
 #pragma unroll
-        for (int ht = 0; ht < 4; ht++) dh[ht] = mfma16(ga, ld4(W4T + (ht * 16 + r) * MLP_OUT + nt * 16 + 4 * g), dh[ht]);
+        for (int ht = 0; ht < 4; ht++) dh[ht] = mfma32(ga, ld8(W4T + (ht * 16 + r) * MLP_OUT + nt2 * 32 + 8 * g), dh[ht]);
         __syncthreads();
     }
     if (a.loss) {
@@ -311,12 +324,11 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t* GT, const uint
     for (int t = 0; t < N / 16; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bs = 0.0f;
     const uint16_t* arow = GT + (int64_t)(mt * 16 + r) * B;
-    for (int64_t k = k0; k < k0 + chunk; k += 16) {
-        const bf16x4 av = ld4(arow + k + 4 * g);
-        bs += (from_bf16((uint16_t)av[0]) + from_bf16((uint16_t)av[1])) +
-              (from_bf16((uint16_t)av[2]) + from_bf16((uint16_t)av[3]));
+    for (int64_t k = k0; k < k0 + chunk; k += 32) {
+        const bf16x8 av = ld8(arow + k + 8 * g);
+        bs += bf16_sum8(av);
 #pragma unroll
-        for (int t = 0; t < N / 16; t++) acc[t] = mfma16(av, ld4(HT + (int64_t)(t * 16 + r) * B + k + 4 * g), acc[t]);
+        for (int t = 0; t < N / 16; t++) acc[t] = mfma32(av, ld8(HT + (int64_t)(t * 16 + r) * B + k + 8 * g), acc[t]);
     }
     float* Pp = part + (int64_t)ks * part_stride + off;
 #pragma unroll

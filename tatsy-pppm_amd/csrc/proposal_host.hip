@@ -167,7 +167,9 @@ int pppm_proposal_enable(void* ctx_raw, const pppm_proposal_params* p) {
         p->anchors > 32 || p->train_interval < 1 || p->train_steps < 1 || p->batch < 256 || p->batch % 256 != 0 ||
         p->batch > 65536 || p->ring_capacity < 1 || p->ring_capacity > (1LL << 28))
         return ctx->fail(PPPM_ERR_INVALID, "bad proposal parameters");
-    if (ctx->h_lights.size() != 1) return ctx->fail(PPPM_ERR_INVALID, "the learned proposal needs exactly one light");
+    if (ctx->h_lights.size() != 1 || ctx->has_env || ctx->rp.method != PPPM_METHOD_PROBABILISTIC)
+        return ctx->fail(PPPM_ERR_INVALID, "the learned proposal needs the probabilistic method, exactly one area "
+                                           "light and no environment map");
     (void)hipSetDevice(ctx->device);
     ctx->prop.pp = *p;
     std::vector<float> w = mlp_init_params(p->seed);
@@ -196,7 +198,9 @@ int pppm_proposal_set_table(void* ctx_raw, const float* prob, int32_t anchors, d
     if (!ctx->rendering) return ctx->fail(PPPM_ERR_STATE, "call pppm_render_begin first");
     if (!prob || anchors < 1 || anchors > 32 || !(lambda > 0.0 && lambda <= 1.0))
         return ctx->fail(PPPM_ERR_INVALID, "bad proposal table");
-    if (ctx->h_lights.size() != 1) return ctx->fail(PPPM_ERR_INVALID, "the learned proposal needs exactly one light");
+    if (ctx->h_lights.size() != 1 || ctx->has_env || ctx->rp.method != PPPM_METHOD_PROBABILISTIC)
+        return ctx->fail(PPPM_ERR_INVALID, "the learned proposal needs the probabilistic method, exactly one area "
+                                           "light and no environment map");
     (void)hipSetDevice(ctx->device);
     const size_t A = (size_t)anchors * anchors;
     TRY_HIP(ctx, ctx->prop_prob.ensure(sizeof(float) * PROP_CELLS * A));

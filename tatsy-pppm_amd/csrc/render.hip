@@ -96,6 +96,40 @@ __device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
 __device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, Rng& rng, v3f& o, v3f& dir,
                                    v3f& flux) {
     float ul = rng.at(0);
+    if (sc.env && (sc.n_lights == 0 || (double)ul * sc.pick_total >= sc.pick_area)) {
+        // environment: a direction w by the importance cells (envmap.cc:62-80;
+        // FP64, so the cell, the texel and the weight are the host's), a point
+        // on a disc of the bounding sphere's radius facing w, travelling along
+        // -w; flux = L(w) pi R^2 (4 pi / 4096) / (pdf(cell) N P_env)
+        const double r = (double)rng.at(3);
+        int lo = 0, hi = ENV_CELLS * ENV_CELLS;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sc.env_cdf[mid] < r) lo = mid + 1;
+            else hi = mid;
+        }
+        const int idx = lo < ENV_CELLS * ENV_CELLS ? lo : sc.env_last;
+        const double u = ((idx % ENV_CELLS) + (double)rng.at(4)) / ENV_CELLS;
+        const double v = ((idx / ENV_CELLS) + (double)rng.at(5)) / ENV_CELLS;
+        const double phi = u * 3.14159265358979323846 * 2.0;
+        const double y = (1.0 - v) * 2.0 - 1.0;
+        const double sy = sqrt(1.0 - y * y);
+        const double wx = sy * cos(phi), wz = sy * sin(phi);
+        const v3f L = env_lookup(sc, wx, y, wz);
+        const float k = (float)(sc.env_k / sc.env_pdf[idx]);
+        flux = L * k;
+        const v3f w = mk((float)wx, (float)y, (float)wz);
+        const v3f ta = fabsf(w.x) > 1.0e-12f ? normalize(cross(mk(0.0f, 1.0f, 0.0f), w))
+                                            : normalize(cross(mk(1.0f, 0.0f, 0.0f), w));
+        const v3f tb = cross(w, ta);
+        const float rad = sc.env_r * sqrtf(rng.at(1));
+        float sn, cs;
+        __sincosf(2.0f * PPPM_PI * rng.at(2), &sn, &cs);
+        const v3f pos = sc.env_c + w * (sc.env_r * 1.0001f) + ta * (rad * cs) + tb * (rad * sn);
+        dir = -w;
+        o = pos + dir * a.ray_eps;
+        return;
+    }
     int l = 0;
     while (l < sc.n_lights - 1 && sc.lights[l].cdf_hi <= ul) l++;
     const LightDev L = sc.lights[l];
@@ -287,10 +321,18 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         float4 hit_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
         for (int bounce = 0;; bounce++) {
-            if (bounce >= a.max_bounces) break;
+            // leaving the scene, or at the bounce limit: the environment, or
+            // black (ppm_probabilistic.cc:248-251)
+            if (bounce >= a.max_bounces) {
+                if (sc.env) direct = thr * env_lookup(sc, d.x, d.y, d.z);
+                break;
+            }
             Hit h = trace_closest<256, COUNT>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf, &tc);
             segs++;
-            if (h.tri < 0) break;
+            if (h.tri < 0) {
+                if (sc.env) direct = thr * env_lookup(sc, d.x, d.y, d.z);
+                break;
+            }
             const int dd = EYE_DIM_BOUNCE + 3 * bounce;
             const float r0 = rng.at(dd), r1 = rng.at(dd + 1), r2 = rng.at(dd + 2);
             (void)r2;
@@ -302,7 +344,7 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
                 break;
             }
             float roulette = m.rr_max;
-            if (bounce >= 3) {
+            if (a.roulette && bounce >= 3) {
                 if (r0 > roulette) break;
             } else {
                 roulette = 1.0f;
@@ -378,6 +420,46 @@ __global__ void k_direct_add(const float4* __restrict__ hp_w, int64_t np, float*
         kahan_add(img_sum, img_comp, 3 * p + 0, w.x);
         kahan_add(img_sum, img_comp, 3 * p + 1, w.y);
         kahan_add(img_sum, img_comp, 3 * p + 2, w.z);
+    }
+}
+
+// The image average of pppm_image_read ((sum - comp) + (dsum - dcomp)) / passes
+// in FP64, encoded per pixel: fmt 0 = 8-bit RGB after c^ginv (ginv <= 0: none)
+// clamped and truncated; fmt 1 = RGBE (frexp of the largest channel);
+// fmt 2 = FP32 RGB.
+__global__ void k_image_encode(const float* __restrict__ s, const float* __restrict__ c,
+                               const float* __restrict__ ds, const float* __restrict__ dc, int64_t npix, double inv,
+                               int fmt, double ginv, uint8_t* out8, float* outf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    double v[3];
+    for (int k = 0; k < 3; k++) {
+        const int64_t j = 3 * i + k;
+        v[k] = (((double)s[j] - (double)c[j]) + ((double)ds[j] - (double)dc[j])) * inv;
+    }
+    if (fmt == 0) {
+        for (int k = 0; k < 3; k++) {
+            double d = ginv > 0.0 ? pow(v[k], ginv) : v[k];
+            d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+            out8[3 * i + k] = (uint8_t)(255.0 * d);
+        }
+    } else if (fmt == 1) {
+        for (int k = 0; k < 3; k++) v[k] = v[k] > 0.0 ? v[k] : 0.0;
+        double d = fmax(v[0], fmax(v[1], v[2]));
+        uint8_t* o = out8 + 4 * i;
+        if (!(d > 1.0e-32)) {
+            o[0] = o[1] = o[2] = o[3] = 0;
+        } else {
+            int e;
+            const double m = frexp(d, &e);
+            d = m * 256.0 / d;
+            o[0] = (uint8_t)(v[0] * d);
+            o[1] = (uint8_t)(v[1] * d);
+            o[2] = (uint8_t)(v[2] * d);
+            o[3] = (uint8_t)(e + 128);
+        }
+    } else {
+        for (int k = 0; k < 3; k++) outf[3 * i + k] = (float)v[k];
     }
 }
 
@@ -479,6 +561,13 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp) {
     if (np <= 0) return;
     hipLaunchKernelGGL(k_direct_add, dim3(grid1(np, 256)), dim3(256), 0, s, hp_w, np, img_sum, img_comp);
+}
+
+void launch_image_encode(hipStream_t s, const float* sum, const float* comp, const float* dsum, const float* dcomp,
+                         int64_t npix, double inv, int fmt, double ginv, uint8_t* out8, float* outf) {
+    if (npix <= 0) return;
+    hipLaunchKernelGGL(k_image_encode, dim3(grid1(npix, 256)), dim3(256), 0, s, sum, comp, dsum, dcomp, npix, inv, fmt,
+                       ginv, out8, outf);
 }
 
 void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, float* sum, float* comp) {

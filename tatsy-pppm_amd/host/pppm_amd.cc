@@ -112,6 +112,45 @@ Vector3D Image::operator()(int x, int y) const {
     return Vector3D(_rgb[k], _rgb[k + 1], _rgb[k + 2]);
 }
 
+// ---- Envmap ----------------------------------------------------------------
+Envmap::Envmap(const std::string& filename) {
+    double* rgb = nullptr;
+    int32_t w = 0, h = 0;
+    char err[512] = {0};
+    const int rc = pppm_image_load(filename.c_str(), &rgb, &w, &h, err, (int32_t)sizeof(err));
+    if (rc != PPPM_OK) throw Error(rc, std::string("Envmap: ") + err);
+    _w = w;
+    _h = h;
+    _rgb.resize(3 * (size_t)w * h);
+    for (size_t i = 0; i < _rgb.size(); i++) _rgb[i] = (float)rgb[i];
+    pppm_image_free(rgb);
+}
+
+void Envmap::resize(int width, int height) {
+    _w = width;
+    _h = height;
+    _rgb.assign(3 * (size_t)width * height, 0.0f);
+}
+
+void Envmap::clearColor(const Vector3D& c) {
+    for (size_t i = 0; i < _rgb.size(); i += 3) {
+        _rgb[i] = (float)c.x();
+        _rgb[i + 1] = (float)c.y();
+        _rgb[i + 2] = (float)c.z();
+    }
+}
+
+Vector3D Envmap::sampleFromDir(const Vector3D& d) const {
+    if (empty()) return Vector3D();
+    const double theta = std::acos(d.y());
+    double phi = std::atan2(d.z(), d.x());
+    if (phi < 0.0) phi += 2.0 * M_PI;
+    const int ix = std::min(std::max((int)(phi / (2.0 * M_PI) * _w), 0), _w - 1);
+    const int iy = std::min(std::max((int)(theta / M_PI * _h), 0), _h - 1);
+    const float* p = &_rgb[3 * ((size_t)iy * _w + ix)];
+    return Vector3D(p[0], p[1], p[2]);
+}
+
 // ---- Device ----------------------------------------------------------------
 Device::Device(int device) {
     int rc = pppm_create(&_ctx, device);
@@ -448,6 +487,8 @@ void Scene::upload() const {
     _dev->check(pppm_scene_upload(_dev->handle(), v.data(), _bsdfIds.data(), (int64_t)_triangles.size(),
                                   table.data(), (int32_t)table.size(), _lights.data(), (int32_t)_lights.size()));
     _dev->check(pppm_bvh_build(_dev->handle(), nullptr));
+    if (!_envmap.empty())
+        _dev->check(pppm_envmap_set(_dev->handle(), _envmap.texels().data(), _envmap.width(), _envmap.height()));
 }
 
 void Scene::setAccelerator() {
@@ -480,11 +521,13 @@ BBox Scene::bounds() const {
 // ---- ProgressivePhotonMappingProb ------------------------------------------
 const double ProgressivePhotonMappingProb::ALPHA = 0.7;  // ppm_probabilistic.cc:9
 
-void ProgressivePhotonMappingProb::render(const Scene& scene, const Camera& camera,
-                                          const RenderParameters& params, RandomSamplerType randomSamplerType) {
-    if (randomSamplerType != RANDOM_SAMPLER_PSEUDO_RANDOM)
-        throw Error(PPPM_ERR_INVALID, "only RANDOM_SAMPLER_PSEUDO_RANDOM is provided: the GPU path draws counter-based "
-                                      "Philox numbers and has no permuted Halton sequence");
+namespace {
+
+// The render loop shared by both methods: params.spp() passes of
+// params.photons() photons each.
+void run_passes(const Scene& scene, const Camera& camera, const RenderParameters& params, const PPMSettings& set,
+                int32_t method, Image& result, std::vector<pppm_pass_stats>& stats,
+                const std::function<void(int, const Image&)>& onPass) {
     if (camera.width() <= 0 || camera.height() <= 0)
         throw Error(PPPM_ERR_INVALID, "the camera has no image: assign a PerspectiveCamera with a size");
     std::shared_ptr<Device> dev = scene.device();
@@ -496,31 +539,52 @@ void ProgressivePhotonMappingProb::render(const Scene& scene, const Camera& came
     std::memset(&p, 0, sizeof(p));
     p.photons = params.photons();
     p.passes = params.spp();
-    p.max_bounces = _settings.bounceLimit;
-    p.alpha = _settings.alpha;
-    p.radius1 = _settings.radiusFraction * (b.posMax() - b.posMin()).norm();
-    p.ray_eps = _settings.rayEps;
-    p.seed = _settings.seed;
-    p.scene_id = _settings.sceneId;
-    p.hash_size = _settings.hashSize;
+    p.max_bounces = set.bounceLimit;
+    p.alpha = set.alpha;
+    p.radius1 = set.radiusFraction * (b.posMax() - b.posMin()).norm();
+    p.ray_eps = set.rayEps;
+    p.seed = set.seed;
+    p.scene_id = set.sceneId;
+    p.hash_size = set.hashSize;
     p.capacity_factor = 8.0;
-    p.gather_mode = _settings.gatherMode;
-    dev->check(pppm_subsurface_configure(ctx, _settings.subsurfacePoints));
-    dev->check(pppm_camera_set(ctx, &camera.raw()));  // configure ends a running render
+    p.gather_mode = set.gatherMode;
+    p.method = method;
+    if (method == PPPM_METHOD_PROBABILISTIC) {
+        dev->check(pppm_subsurface_configure(ctx, set.subsurfacePoints));
+        dev->check(pppm_camera_set(ctx, &camera.raw()));  // configure ends a running render
+    }
     dev->check(pppm_render_begin(ctx, &p));
-    if (_settings.learnedProposal) dev->check(pppm_proposal_enable(ctx, &_settings.proposal));
-    _result = Image(camera.width(), camera.height());
-    _stats.clear();
+    if (method == PPPM_METHOD_PROBABILISTIC && set.learnedProposal) dev->check(pppm_proposal_enable(ctx, &set.proposal));
+    result = Image(camera.width(), camera.height());
+    stats.clear();
     for (int t = 1; t <= params.spp(); t++) {
         pppm_pass_stats st;
         dev->check(pppm_render_pass(ctx, &st));
-        _stats.push_back(st);
-        if (_onPass) {
-            dev->check(pppm_image_read(ctx, _result.data().data(), (int64_t)_result.data().size()));
-            _onPass(t, _result);
+        stats.push_back(st);
+        if (onPass) {
+            dev->check(pppm_image_read(ctx, result.data().data(), (int64_t)result.data().size()));
+            onPass(t, result);
         }
     }
-    dev->check(pppm_image_read(ctx, _result.data().data(), (int64_t)_result.data().size()));
+    dev->check(pppm_image_read(ctx, result.data().data(), (int64_t)result.data().size()));
+}
+
+}  // namespace
+
+void ProgressivePhotonMappingProb::render(const Scene& scene, const Camera& camera,
+                                          const RenderParameters& params, RandomSamplerType randomSamplerType) {
+    if (randomSamplerType != RANDOM_SAMPLER_PSEUDO_RANDOM)
+        throw Error(PPPM_ERR_INVALID, "only RANDOM_SAMPLER_PSEUDO_RANDOM is provided: the GPU path draws counter-based "
+                                      "Philox numbers and has no permuted Halton sequence");
+    run_passes(scene, camera, params, _settings, PPPM_METHOD_PROBABILISTIC, _result, _stats, _onPass);
+}
+
+const double ProgressivePhotonMapping::ALPHA = 0.7;  // progressive_photon_mapping.cc:17
+
+void ProgressivePhotonMapping::render(const Scene& scene, const Camera& camera, const RenderParameters& params) {
+    if (_settings.learnedProposal)
+        throw Error(PPPM_ERR_INVALID, "the learned proposal belongs to the probabilistic method");
+    run_passes(scene, camera, params, _settings, PPPM_METHOD_HACHISUKA, _result, _stats, _onPass);
 }
 
 }  // namespace pppm_amd

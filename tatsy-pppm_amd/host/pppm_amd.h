@@ -349,6 +349,29 @@ public:
     const std::vector<double>& data() const { return _rgb; }
 };
 
+// ---- sources/envmap.h:14-60 ------------------------------------------------
+// Lat-long environment (row 0 along +y).  A map of 64 x 64 texels or more
+// lights the scene it is set on (Scene::setEnvmap) and is what eye rays
+// leaving the scene see; the default (empty) map is black.
+class Envmap {
+    int _w = 0, _h = 0;
+    std::vector<float> _rgb;  // [h][w][3]
+
+public:
+    Envmap() {}
+    // .hdr (RGBE), .pfm or .bmp through pppm_image_load
+    explicit Envmap(const std::string& filename);
+    void resize(int width, int height);
+    void clearColor(const Vector3D& color);
+    // the texel of a direction (envmap.cc:47-60)
+    Vector3D sampleFromDir(const Vector3D& dir) const;
+    int width() const { return _w; }
+    int height() const { return _h; }
+    bool empty() const { return _rgb.empty(); }
+    const std::vector<float>& texels() const { return _rgb; }
+    std::vector<float>& texels() { return _rgb; }
+};
+
 // A device context shared by the objects that use the GPU.
 class Device {
     void* _ctx = nullptr;
@@ -407,6 +430,7 @@ class Scene {
     std::vector<int> _bsdfIds;
     std::vector<BSDF> _bsdfs;
     std::vector<pppm_light> _lights;
+    Envmap _envmap;
     HostBvh _host;
     bool _accel = false;
 
@@ -430,6 +454,9 @@ public:
     // load a procedural benchmark scene (include/pppm.h PPPM_SCENE_*)
     PerspectiveCamera loadProcedural(int kind, uint64_t seed, int frequency, int width, int height);
 
+    // sources/scene.h:60-61: the environment lights the scene beside its area lights
+    void setEnvmap(const Envmap& envmap) { _envmap = envmap; }
+    const Envmap& envmap() const { return _envmap; }
     const Triangle& getTriangle(int id) const { return _triangles.at(id); }
     const BSDF& getBsdf(int triId) const { return _bsdfs.at(_bsdfIds.at(triId)); }
     size_t numTriangles() const { return _triangles.size(); }
@@ -542,6 +569,29 @@ public:
     const std::vector<pppm_pass_stats>& passStats() const { return _stats; }
     // called after every pass with the running average (the reference saves a
     // PNG here, sources/ppm_probabilistic.cc:97-106)
+    void setPassCallback(std::function<void(int, const Image&)> f) { _onPass = std::move(f); }
+    PPMSettings& settings() { return _settings; }
+};
+
+// ---- sources/progressive_photon_mapping.h:25-108 ----------------------------
+// Progressive photon mapping with a radius per render point (Hachisuka et al.
+// 2008) on the same device path: pixel render points keep r^2, their photon
+// count and their flux across passes.  Takes no BSSRDF materials and no
+// learned proposal; settings().radiusFraction only serves a pass whose render
+// points have no extent.
+class ProgressivePhotonMapping {
+    Image _result;
+    PPMSettings _settings;
+    std::vector<pppm_pass_stats> _stats;
+    std::function<void(int, const Image&)> _onPass;
+
+public:
+    static const double ALPHA;
+    ProgressivePhotonMapping() {}
+    explicit ProgressivePhotonMapping(const PPMSettings& s) : _settings(s) {}
+    void render(const Scene& scene, const Camera& camera, const RenderParameters& params);
+    const Image& result() const { return _result; }
+    const std::vector<pppm_pass_stats>& passStats() const { return _stats; }
     void setPassCallback(std::function<void(int, const Image&)> f) { _onPass = std::move(f); }
     PPMSettings& settings() { return _settings; }
 };

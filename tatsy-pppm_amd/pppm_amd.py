@@ -16,6 +16,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
 GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE = 0, 1, 2, 3
+METHOD_PROBABILISTIC, METHOD_HACHISUKA = 0, 1
 PROPOSAL_CELLS = 1024
 MLP_PARAMS = 76992
 
@@ -45,7 +46,7 @@ class RenderParams(C.Structure):
     _fields_ = [("photons", C.c_int64), ("passes", C.c_int32), ("max_bounces", C.c_int32),
                 ("alpha", C.c_double), ("radius1", C.c_double), ("ray_eps", C.c_double),
                 ("seed", C.c_uint64), ("scene_id", C.c_uint32), ("hash_size", C.c_int32),
-                ("capacity_factor", C.c_double), ("gather_mode", C.c_int32), ("reserved", C.c_int32)]
+                ("capacity_factor", C.c_double), ("gather_mode", C.c_int32), ("method", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -108,6 +109,13 @@ _DECLS = {
                                     C.POINTER(C.c_int32)]),
     "pppm_mesh_load": (C.c_int, [C.c_char_p, C.POINTER(Mesh), C.c_char_p, C.c_int32]),
     "pppm_mesh_free": (None, [C.POINTER(Mesh)]),
+    "pppm_envmap_set": (C.c_int, [_P, _P, C.c_int32, C.c_int32]),
+    "pppm_envmap_importance": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "pppm_image_save": (C.c_int, [_P, C.c_char_p, C.c_double]),
+    "pppm_image_write": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_double]),
+    "pppm_image_load": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
+    "pppm_image_free": (None, [C.POINTER(C.c_double)]),
     "pppm_hashgrid_build": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
     "pppm_hashgrid_build_boxes": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P,
                                             C.c_int64, C.POINTER(C.c_int64), _P, _P]),
@@ -245,6 +253,40 @@ def load_mesh(path):
     return v, f
 
 
+def envmap_importance(rgb):
+    """pdf [4096] and running sums cdf [4096] of an environment map's importance cells."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    pdf, cdf = np.zeros(4096), np.zeros(4096)
+    rc = lib().pppm_envmap_importance(_ptr(rgb), rgb.shape[1], rgb.shape[0], _ptr(pdf), _ptr(cdf))
+    if rc != OK:
+        raise PPPMError(rc, "bad environment map")
+    return pdf, cdf
+
+
+def image_write(path, rgb, gamma=0.0):
+    """Write an FP64 [h][w][3] image (top row first) by extension: .png / .bmp
+    (8-bit after c^(1/gamma)), .hdr (RGBE), .pfm (FP32)."""
+    rgb = np.ascontiguousarray(rgb, np.float64)
+    h, w = rgb.shape[:2]
+    rc = lib().pppm_image_write(str(path).encode(), _ptr(rgb), w, h, gamma)
+    if rc != OK:
+        raise PPPMError(rc, f"cannot write {path}")
+
+
+def image_load(path):
+    """FP64 [h][w][3] image of a .hdr, .pfm or .bmp file."""
+    p = C.POINTER(C.c_double)()
+    w, h = C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(512)
+    rc = lib().pppm_image_load(str(path).encode(), C.byref(p), C.byref(w), C.byref(h), err, 512)
+    if rc != OK:
+        raise PPPMError(rc, err.value.decode())
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 3)).copy()
+    finally:
+        lib().pppm_image_free(p)
+
+
 def batch_specs(n_scenes, seed):
     """Scene specifications of a batch drawn like the batched-scenes workload:
     triangle counts log-uniform in [1k, 500k], square resolutions from
@@ -319,8 +361,9 @@ class Context:
     def upload(self, scene):
         tri = np.ascontiguousarray(scene.tri, np.float32)
         ids = np.ascontiguousarray(scene.bsdf_id, np.int32)
+        lights = scene.light_array() if scene.lights else None
         self._chk(self.L.pppm_scene_upload(self.h, _ptr(tri), _ptr(ids), len(ids), scene.bsdf_array(),
-                                           len(scene.bsdfs), scene.light_array(), len(scene.lights)))
+                                           len(scene.bsdfs), lights, len(scene.lights)))
 
     def build_bvh(self):
         info = BvhInfo()
@@ -367,15 +410,24 @@ class Context:
                                                    C.byref(npairs), _ptr(cs), _ptr(ce)))
         return items[:npairs.value], cs, ce
 
+    def set_envmap(self, rgb):
+        """Lat-long environment map [h][w][3] (None: remove)."""
+        if rgb is None:
+            self._chk(self.L.pppm_envmap_set(self.h, None, 0, 0))
+            return
+        self._env = np.ascontiguousarray(rgb, np.float32)
+        self._chk(self.L.pppm_envmap_set(self.h, _ptr(self._env), self._env.shape[1], self._env.shape[0]))
+
     def set_camera(self, cam):
         self._chk(self.L.pppm_camera_set(self.h, C.byref(cam)))
         self.camera = cam
 
     def begin(self, photons, passes, radius1, alpha=0.7, ray_eps=1e-4, seed=0x7A75, scene_id=0,
-              hash_size=0, max_bounces=64, capacity_factor=8.0, gather_mode=GATHER_AUTO):
+              hash_size=0, max_bounces=64, capacity_factor=8.0, gather_mode=GATHER_AUTO, method=METHOD_PROBABILISTIC):
         p = RenderParams(photons=photons, passes=passes, max_bounces=max_bounces, alpha=alpha,
                          radius1=radius1, ray_eps=ray_eps, seed=seed, scene_id=scene_id,
-                         hash_size=hash_size, capacity_factor=capacity_factor, gather_mode=gather_mode)
+                         hash_size=hash_size, capacity_factor=capacity_factor, gather_mode=gather_mode,
+                         method=method)
         self._chk(self.L.pppm_render_begin(self.h, C.byref(p)))
         self.params = p
 
@@ -397,6 +449,9 @@ class Context:
         out = np.zeros((h, w, 3), np.float64)
         self._chk(self.L.pppm_image_read(self.h, _ptr(out), out.size))
         return out
+
+    def image_save(self, path, gamma=2.2):
+        self._chk(self.L.pppm_image_save(self.h, str(path).encode(), gamma))
 
     def last_photons(self, cap=None):
         n = C.c_int64()

@@ -9,6 +9,7 @@
 //   appmain_shape [width] [height] [spp] [mesh.ply]
 // Writes the mesh file it loads (an icosphere) when none is given; exits 0 and
 // prints OK when the render is finite, non-negative and lit.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -132,16 +133,28 @@ void setScene(Scene* scene, Camera* camera, int imageWidth, int imageHeight) {
 
 static int check(const ProgressivePhotonMappingProb& ppm, const Camera& camera) {
     const Image& img = ppm.result();
-    double sum = 0.0;
-    bool ok = img.width() == camera.imagesize().width() && img.height() == camera.imagesize().height();
+    int fails = 0;
+    auto expect = [&](bool cond, const char* what) {
+        std::printf("%s: %s\n", what, cond ? "ok" : "FAILED");
+        fails += cond ? 0 : 1;
+    };
+    expect(img.width() == camera.imagesize().width() && img.height() == camera.imagesize().height(),
+           "image size is the camera's");
+    double sum = 0.0, lo = 0.0;
+    bool finite = true;
     for (double v : img.data()) {
-        ok = ok && std::isfinite(v) && v >= 0.0;
+        finite = finite && std::isfinite(v);
+        lo = std::min(lo, v);
         sum += v;
     }
-    ok = ok && sum > 0.0 && !ppm.passStats().empty();
+    expect(finite, "image finite");
+    // compensated sums of non-negative terms: at worst a rounding below zero
+    expect(lo > -1e-6 * (sum / img.data().size()), "image non-negative");
+    expect(sum > 0.0 && !ppm.passStats().empty(), "image lit");
     // the camera ray through the image centre is the view direction
     const Ray r = camera.getRay(0.5 * camera.imagesize().width(), 0.5 * camera.imagesize().height());
-    ok = ok && std::fabs(Vector3D::dot(r.direction(), camera.direction().normalized()) - 1.0) < 1e-12;
+    expect(std::fabs(Vector3D::dot(r.direction(), camera.direction().normalized()) - 1.0) < 1e-9,
+           "centre ray along the view direction");
     // a quasi-Monte Carlo sampler is refused with an error, not a silent substitute
     bool refused = false;
     try {
@@ -151,7 +164,8 @@ static int check(const ProgressivePhotonMappingProb& ppm, const Camera& camera) 
     } catch (const Error& e) {
         refused = e.code == PPPM_ERR_INVALID;
     }
-    ok = ok && refused;
-    std::printf("mean %.6g over %dx%d: %s\n", sum / img.data().size(), img.width(), img.height(), ok ? "OK" : "FAIL");
-    return ok ? 0 : 1;
+    expect(refused, "quasi-Monte Carlo sampler refused");
+    std::printf("mean %.6g (min %.3g) over %dx%d: %s\n", sum / img.data().size(), lo, img.width(), img.height(),
+                fails ? "FAIL" : "OK");
+    return fails ? 1 : 0;
 }

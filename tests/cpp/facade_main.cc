@@ -237,6 +237,31 @@ int main(int argc, char** argv) {
         }
         std::printf("proposal: loss %.4f after pass 2\n", ppm.passStats()[1].train_loss);
     }
+    // --- per-point radii (sources/progressive_photon_mapping.h:92-98) -----------
+    if (argc > 4) {
+        Scene s5;
+        PerspectiveCamera cam = s5.loadProcedural(PPPM_SCENE_BOX, 1, 0, 40, 32);
+        ProgressivePhotonMapping ppm;
+        CHECK(ProgressivePhotonMapping::ALPHA == 0.7, "ALPHA");
+        ppm.render(s5, cam, RenderParameters(30000, 3, 0, 0.0));
+        CHECK(ppm.passStats().size() == 3 && ppm.passStats()[2].radius > 0.0, "three passes with a radius");
+        FILE* f = std::fopen(argv[4], "wb");
+        if (f) {
+            std::fwrite(ppm.result().data().data(), sizeof(double), ppm.result().data().size(), f);
+            std::fclose(f);
+        }
+        bool threw = false;
+        try {
+            PPMSettings set;
+            set.learnedProposal = true;
+            ProgressivePhotonMapping bad(set);
+            bad.render(s5, cam, RenderParameters(1000, 1, 0, 0.0));
+        } catch (const Error&) {
+            threw = true;
+        }
+        CHECK(threw, "the learned proposal is refused by the per-point method");
+        std::printf("hachisuka: initial radius %.6f in pass 3\n", ppm.passStats()[2].radius);
+    }
     std::printf(failures ? "FAILED\n" : "OK\n");
     return failures ? 1 : 0;
 }

@@ -36,7 +36,7 @@ def test_cpp_interface_matches_c_abi(gpu_ctx, tmp_path):
 
 def test_cpp_translucent_and_learned_renders_match_c_abi(gpu_ctx, tmp_path):
     print("[case] C++ interface: BSSRDF and learned proposal", flush=True)
-    out = [tmp_path / n for n in ("a.bin", "sss.bin", "prop.bin")]
+    out = [tmp_path / n for n in ("a.bin", "sss.bin", "prop.bin", "ppm.bin")]
     r = subprocess.run([BIN] + [str(o) for o in out], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                        text=True, timeout=240)
     print(r.stdout, flush=True)
@@ -61,6 +61,15 @@ def test_cpp_translucent_and_learned_renders_match_c_abi(gpu_ctx, tmp_path):
     for _ in range(4):
         gpu_ctx.render_pass()
     assert np.array_equal(np.fromfile(out[2], np.float64).reshape(48, 48, 3), gpu_ctx.image())
+    # ProgressivePhotonMapping: the C ABI with method = PPPM_METHOD_HACHISUKA
+    s = P.scenegen(P.SCENE_BOX, 1, 0, 40, 32)
+    gpu_ctx.upload(s)
+    gpu_ctx.build_bvh()
+    gpu_ctx.set_camera(s.camera)
+    gpu_ctx.begin(30000, 3, 0.01 * s.diagonal(), ray_eps=1e-4, seed=0x7A75, method=P.METHOD_HACHISUKA)
+    for _ in range(3):
+        gpu_ctx.render_pass()
+    assert np.array_equal(np.fromfile(out[3], np.float64).reshape(32, 40, 3), gpu_ctx.image())
 
 
 def test_reference_application_shape_runs(tmp_path):

@@ -426,8 +426,9 @@ def test_kahan_accumulation_matches_fp64(gpu_ctx, kind):
     got = s.astype(np.float64) - c.astype(np.float64)
     # compensated: within a few FP32 ulps of the total
     assert np.all(np.abs(got - ref) <= 4 * np.spacing(np.float32(ref))), (got - ref).max()
-    # the plain FP32 running sum of the same addends is far off (the test can tell)
+    # the plain FP32 running sum of a fifth of the addends already misses that
+    # bound (the test can tell compensated from plain accumulation)
     naive = np.float32(0.0)
     for x in v[:200_000]:
         naive = np.float32(naive + x)
-    assert abs(float(naive) - v[:200_000].astype(np.float64).sum()) > 100 * np.spacing(np.float32(ref))
+    assert abs(float(naive) - v[:200_000].astype(np.float64).sum()) > 4 * np.spacing(np.float32(ref))

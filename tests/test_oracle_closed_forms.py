@@ -208,3 +208,27 @@ def test_render_agrees_with_independent_path_tracer_on_box():
     # 8 x 8 pixel blocks: the photon map's own block noise is ~1.4 % (two seeds)
     assert np.sqrt((rel ** 2).mean()) < 0.025
     assert np.abs(rel).max() < 0.08
+
+
+# ---- progressive photon mapping with per-point radii (Hachisuka) -------------
+def test_hachisuka_render_sphere_furnace_closed_form():
+    """The per-point-radius method converges to the same radiance: inside the
+    furnace sphere every render point sees L = rho Phi / (pi A (1 - rho))."""
+    rho, le = (0.3, 0.5, 0.7), 4.0
+    sc, sph = S.sphere_furnace(rho=rho, le=(le,) * 3, res=24)
+    A = S.tri_area(sph).sum()
+    phi = math.pi * f32(le) * 0.1 * 0.1
+    expect = f32(rho) * phi / (math.pi * A * (1.0 - f32(rho)))
+    img, st = O.OracleScene(sc).render_hachisuka(sc.camera, 100000, 6)
+    assert st[0] > 0
+    np.testing.assert_allclose(img.reshape(-1, 3).mean(0) / expect, 1.0, atol=0.03)
+
+
+def test_hachisuka_and_probabilistic_agree_on_box():
+    """Both consistent estimators of the same image: their box-scene renders
+    agree within noise and radius bias (and with the path tracer)."""
+    s = P.scenegen(P.SCENE_BOX, 1, 0, 32, 32)
+    os_ = O.OracleScene(s)
+    h, _ = os_.render_hachisuka(s.camera, 100000, 8)
+    pt = os_.pathtrace(s.camera, 128)
+    assert abs(h.mean() / pt.mean() - 1) < 0.03
