@@ -209,6 +209,21 @@ int pppm_bvh_download(void* ctx, pppm_bvh_node* nodes, int64_t node_cap, float* 
 int pppm_hashgrid_build(void* ctx, const float* pos /* [n][3] */, int64_t n, int32_t hash_size,
                         double hash_scale, const double bbox_min[3], int32_t* sorted_idx,
                         int32_t* cell_start, int32_t* cell_end);
+/* The point-insertion grid of pppm_hashgrid_build without any allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  pos, sorted_idx, cell_start and cell_end are DEVICE
+ * buffers the caller owns (same layouts as pppm_hashgrid_build); bbox_min is
+ * host memory.  Intermediates live in one fixed array per GPU (about 21 MB,
+ * held until the process ends), so n <= PPPM_NOALLOC_MAX_POINTS and
+ * hash_size <= PPPM_NOALLOC_MAX_BUCKETS (PPPM_ERR_CAPACITY otherwise).
+ * Calls that overlap on one GPU race: every call on a GPU shares its array.
+ * Synchronous on the context's stream. */
+#define PPPM_NOALLOC_MAX_POINTS (1 << 20)
+#define PPPM_NOALLOC_MAX_BUCKETS (1 << 20)
+int pppm_hashgrid_build_noalloc(void* ctx, const float* pos, int64_t n, int32_t hash_size, double hash_scale,
+                                const double bbox_min[3], int32_t* sorted_idx, int32_t* cell_start,
+                                int32_t* cell_end);
 /* Box insertion add(i, box_min_i, box_max_i): every (item, cell) pair in the
  * reference's z, y, x loop order.  *n_pairs returns the pair count; fails
  * with PPPM_ERR_CAPACITY if it exceeds cap. */

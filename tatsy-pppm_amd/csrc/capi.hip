@@ -168,6 +168,7 @@ int pppm_create(void** out, int32_t device) {
         c->cu = prop.multiProcessorCount;
         c->hbm = prop.totalGlobalMem;
     }
+    c->noalloc_err = noalloc_setup(device, &c->noalloc_scratch);
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         delete c;

@@ -148,6 +148,10 @@ struct Ctx {
     double env_k = 0.0, pick_total = 0.0, pick_area = 0.0;
     // render points of the Hachisuka method: radius^2, photon count, flux
     DBuf ppm_r2, ppm_n, ppm_flux, ppm_bounds, ppm_image;
+    // device address of the no-allocation scratch array on this device
+    // (pppm_hashgrid_build_noalloc), or null with the lookup's error
+    void* noalloc_scratch = nullptr;
+    hipError_t noalloc_err = hipSuccess;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -229,6 +233,9 @@ double dipole_fdr(double eta);
 uint32_t morton_bits(uint32_t v);
 hipError_t subsurface_setup(Ctx* c, const float* tri_h /* may be null: read back */);
 hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius);
+// no-allocation scratch (noalloc.hip): its device address on `device` (the
+// current device), looked up once per device
+hipError_t noalloc_setup(int device, void** addr);
 // environment map (envmap_host.hip): the importance cells of Envmap::
 // createImportanceMap (FP64, bit for bit), and the emission constants of a
 // render with N photons per pass (area-light power total in, pick and disc

@@ -117,6 +117,7 @@ _DECLS = {
                                   C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
     "pppm_image_free": (None, [C.POINTER(C.c_double)]),
     "pppm_hashgrid_build": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
+    "pppm_hashgrid_build_noalloc": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, _P, _P]),
     "pppm_hashgrid_build_boxes": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P,
                                             C.c_int64, C.POINTER(C.c_int64), _P, _P]),
     "pppm_radius_schedule": (C.c_int, [C.c_double, C.c_double, C.c_int32, C.c_int32, _P]),
