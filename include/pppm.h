@@ -337,6 +337,13 @@ int pppm_issue_probe(void* ctx, double* valu_cycles, double* trans_cycles, doubl
 /* Surface points of later renders (0: the reference's density, at most 2^20);
  * takes effect at pppm_render_begin. */
 int pppm_subsurface_configure(void* ctx, int64_t points);
+/* The sum over the surface points: max_error 0 (default) sums every point;
+ * max_error > 0 walks the reference's octree cut, taking a node whole when
+ * its area / d^2 < max_error and x lies outside its box
+ * (sources/subsurface_integrator.cc:69-167; the reference passes 0.05 at
+ * ppm_probabilistic.cc:84).  Takes effect at
+ * pppm_render_begin, and for pppm_dipole_eval at once. */
+int pppm_subsurface_cut(void* ctx, double max_error);
 /* Points of the current render [n][3], their normals, and the area per point. */
 int pppm_subsurface_points(void* ctx, float* pos, float* normal, int64_t cap, int64_t* n, double* area);
 /* Irradiance E_j [n][3] of the last pass. */

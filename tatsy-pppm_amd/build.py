@@ -27,7 +27,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "i
 
 HIP_SOURCES = ["radix.hip", "hash_grid.hip", "bvh.hip", "render.hip", "gather.hip", "proposal.hip", "subsurface.hip",
                "capi.hip", "pass.hip", "sss_host.hip", "proposal_host.hip", "batch.hip", "noalloc.hip",
-               "envmap_host.hip", "hachisuka.hip"]
+               "envmap_host.hip", "hachisuka.hip", "dipole_cut.hip"]
 CXX_SOURCES = ["scenegen.cc", "pppm_amd.cc", "mesh_io.cc", "image_io.cc"]
 FACADE_TEST = os.path.join(ROOT, "tests", "cpp", "facade_main")
 # C++ programs over the reference-shaped interface, built beside their sources

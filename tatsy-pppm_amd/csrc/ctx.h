@@ -47,6 +47,13 @@ struct DBuf {
     }
 };
 
+// the octree cut of the dipole sum on the device (dipole_cut.hip)
+struct CutBufs {
+    DBuf a, b, lo, hi, perm, levels;
+    std::vector<int64_t> level_off;  // nodes of depth d: levels[level_off[d] .. level_off[d + 1])
+    int32_t n_nodes = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -115,6 +122,8 @@ struct Ctx {
     DBuf dips, sss_pts, sss_ptw, sss_E, irr_pos, irr_flux, sss_pos, sss_w;
     std::vector<int32_t> h_sss_index;  // material -> dipole index (-1: none)
     int64_t sss_points_req = 0;        // 0: the default count
+    double sss_cut = 0.0;              // 0: the exact sum; > 0: the octree cut's maximum error
+    CutBufs cut;                       // the cut over the surface points
     int64_t sss_n = 0;                 // surface points of the current render
     double sss_area = 0.0;             // area per point
     bool sss_present = false;
@@ -233,6 +242,17 @@ double dipole_fdr(double eta);
 uint32_t morton_bits(uint32_t v);
 hipError_t subsurface_setup(Ctx* c, const float* tri_h /* may be null: read back */);
 hipError_t subsurface_irradiance(Ctx* c, int64_t n_irr, double radius);
+// the octree cut of the dipole sum (dipole_cut.hip): the preorder tree built
+// on the host, its upload, its device view and the per-pass node values
+struct CutHost {
+    std::vector<float4> lo, hi, a;
+    std::vector<uint32_t> esc, perm;
+    std::vector<int32_t> depth;
+};
+void cut_build(const float4* pts, int64_t n, double area, CutHost* out);
+hipError_t cut_upload(CutBufs* t, const CutHost& h);
+CutDev cut_dev(const CutBufs& t, double max_error);
+void cut_refresh(const CutBufs& t, hipStream_t s, const float* E, double max_error);
 // no-allocation scratch (noalloc.hip): its device address on `device` (the
 // current device), looked up once per device
 hipError_t noalloc_setup(int device, void** addr);

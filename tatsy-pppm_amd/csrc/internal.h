@@ -374,6 +374,20 @@ struct DipoleArgs {
 };
 // L = W (1 - F_dr) / pi * area * sum_j R_d(|x - x_j|^2) E_j for every hit
 void launch_dipole(hipStream_t s, const DipoleArgs& a);
+// The octree cut of the same sum (dipole_cut.hip): nodes in preorder
+struct CutDev {
+    float4* a;              // position xyz (per pass), area
+    float4* b;              // irradiance rgb (per pass), escape index | leaf bit 31
+    const float4* lo;       // box min, first leaf point in perm (bits)
+    const float4* hi;       // box max, leaf point count (bits; 0: interior node)
+    const uint32_t* perm;   // points of the leaves
+    int32_t n_nodes;
+    float max_error;        // a node is taken whole when area / d^2 < max_error, x outside its box
+};
+// node values of `count` nodes of one level from E [point][3] and the deeper levels
+void launch_cut_update(hipStream_t s, const CutDev& t, const int32_t* nodes, int64_t count, const float* E);
+// launch_dipole's output with the sum walked over the cut
+void launch_dipole_cut(hipStream_t s, const DipoleArgs& a, const CutDev& t);
 // issue-rate probes (which 0: v_fma_f32 chains, 1: v_exp_f32 + v_fma_f32
 // chains) over blocks x 256 threads writing out[blocks * 256]
 void launch_issue_probe(hipStream_t s, float* out, int blocks, int which);

@@ -376,7 +376,12 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         da.img_sum = ctx->img_sum.as<float>();
         da.img_comp = ctx->img_comp.as<float>();
         da.out_rgb = nullptr;
-        launch_dipole(s, da);
+        if (ctx->sss_cut > 0.0) {
+            cut_refresh(ctx->cut, s, ctx->sss_E.as<float>(), ctx->sss_cut);
+            launch_dipole_cut(s, da, cut_dev(ctx->cut, ctx->sss_cut));
+        } else {
+            launch_dipole(s, da);
+        }
         TRY_HIP(ctx, hipGetLastError());
     }
     TRY_HIP(ctx, hipEventRecord(ctx->ev[10], s));

@@ -136,6 +136,12 @@ hipError_t subsurface_setup(Ctx* c, const float* tri_h /* may be null: read back
         return e;
     c->sss_n = n;
     c->sss_area = total / (double)n;
+    c->cut.n_nodes = 0;
+    if (c->sss_cut > 0.0) {  // the octree over the points, built once per render
+        CutHost h;
+        cut_build(sorted.data(), n, c->sss_area, &h);
+        if ((e = cut_upload(&c->cut, h)) != hipSuccess) return e;
+    }
     c->sss_present = true;
     return hipSuccess;
 }
@@ -190,6 +196,15 @@ int pppm_subsurface_configure(void* ctx_raw, int64_t points) {
     CHECK_CTX(ctx);
     if (points < 0 || points > (1LL << 26)) return ctx->fail(PPPM_ERR_INVALID, "points: 0 (default) .. 2^26");
     ctx->sss_points_req = points;
+    ctx->rendering = false;
+    return PPPM_OK;
+}
+
+int pppm_subsurface_cut(void* ctx_raw, double max_error) {
+    CHECK_CTX(ctx);
+    if (!(max_error >= 0.0) || !std::isfinite(max_error))
+        return ctx->fail(PPPM_ERR_INVALID, "max_error: 0 (the exact sum) or a positive bound on area / d^2");
+    ctx->sss_cut = max_error;
     ctx->rendering = false;
     return PPPM_OK;
 }
@@ -293,6 +308,7 @@ int pppm_dipole_eval(void* ctx_raw, const pppm_bsdf* material, const float* pts,
         std::memcpy(&idf, &id, 4);
         w4[(size_t)i] = make_float4(1.0f, 1.0f, 1.0f, idf);
     }
+    CutBufs cut;  // freed on return
     void* buf = nullptr;
     const size_t bytes = sizeof(DipoleDev) + sizeof(float4) * (p4.size() + 2 * q4.size()) +
                          sizeof(float) * 3 * (size_t)(std::max<int64_t>(n_pts, 1) + n_query);
@@ -323,7 +339,15 @@ int pppm_dipole_eval(void* ctx_raw, const pppm_bsdf* material, const float* pts,
         a.img_sum = nullptr;
         a.img_comp = nullptr;
         a.out_rgb = dout;
-        if (n_pts > 0) {
+        if (n_pts > 0 && ctx->sss_cut > 0.0) {  // the cut over these points, built for this call
+            CutHost h;
+            cut_build(p4.data(), n_pts, area, &h);
+            e = cut_upload(&cut, h);
+            if (e == hipSuccess) {
+                cut_refresh(cut, ctx->stream, dE, ctx->sss_cut);
+                launch_dipole_cut(ctx->stream, a, cut_dev(cut, ctx->sss_cut));
+            }
+        } else if (n_pts > 0) {
             launch_dipole(ctx->stream, a);
         } else {
             launch_zero_u32(ctx->stream, dout, 3 * n_query);

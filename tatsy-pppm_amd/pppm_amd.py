@@ -136,6 +136,7 @@ _DECLS = {
     "pppm_kahan_probe": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
     "pppm_issue_probe": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pppm_subsurface_configure": (C.c_int, [_P, C.c_int64]),
+    "pppm_subsurface_cut": (C.c_int, [_P, C.c_double]),
     "pppm_subsurface_points": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "pppm_subsurface_irradiance": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pppm_last_irradiance_photons": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
@@ -564,6 +565,10 @@ class Context:
     # ---- subsurface scattering ----
     def subsurface_configure(self, points):
         self._chk(self.L.pppm_subsurface_configure(self.h, points))
+
+    def subsurface_cut(self, max_error):
+        """0: the exact dipole sum; > 0: the octree cut with this error bound."""
+        self._chk(self.L.pppm_subsurface_cut(self.h, max_error))
 
     def subsurface_points(self):
         n = C.c_int64()

@@ -8,7 +8,8 @@ QBVHAccel construct/intersect (qbvh_accel.cc), Triangle::intersect
 (triangle.cc), sampler::onHemisphere (sampler.cc), checkTotalReflection
 (reflectance.h), BSDF::sample of the Lambertian / specular / refraction BSDFs
 (brdf.cc), PerspectiveCamera::getRay (perspective_camera.cc) and
-DipoleBSSRDF::operator() / Fdr (bssrdf.cc).
+DipoleBSSRDF::operator() / Fdr (bssrdf.cc) and SubsurfaceIntegrator's octree
+cut of the dipole sum (subsurface_integrator.cc:69-167, 234-238).
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -214,6 +215,32 @@ def main():
             dip_vals.append([*o3, fdr.value])
     out["dip_params"] = np.array(dip_params)
     out["dip_vals"] = np.array(dip_vals)
+
+    # --- SubsurfaceIntegrator's octree cut (its own generator, so the
+    # arrays above do not move) ------------------------------------------------
+    orng = np.random.default_rng(0x0C7)
+    for name, n, eps in (("sphere", 3000, 0.05), ("plane", 2500, 0.05), ("sphere_tight", 3000, 0.01)):
+        if name.startswith("sphere"):
+            v = orng.normal(size=(n, 3))
+            pts = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+            qv = orng.normal(size=(200, 3))
+            q = (qv / np.linalg.norm(qv, axis=1, keepdims=True) * orng.uniform(0.97, 1.03, (200, 1)))
+        else:
+            pts = np.column_stack([orng.uniform(-1.5, 1.5, n), np.zeros(n), orng.uniform(-1.5, 1.5, n)])
+            q = np.column_stack([orng.uniform(-2.0, 2.0, 200), orng.uniform(-0.05, 0.05, 200),
+                                 orng.uniform(-2.0, 2.0, 200)])
+        pts = np.ascontiguousarray(pts, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        E = np.ascontiguousarray(orng.uniform(0.0, 2.0, (n, 3)) * (orng.random((n, 1)) < 0.9), np.float32)
+        sa, ss, eta = orng.uniform(0.01, 0.2, 3), orng.uniform(0.5, 3.0, 3), 1.3
+        area = 4.0 * np.pi / n if name.startswith("sphere") else 9.0 / n
+        res = np.zeros((len(q), 3))
+        R.ref_dipole_octree(P(sa), P(ss), eta, P(pts), P(E), n, area, P(q), len(q), eps, P(res))
+        out[f"oct_{name}_pts"] = pts
+        out[f"oct_{name}_E"] = E
+        out[f"oct_{name}_q"] = q
+        out[f"oct_{name}_params"] = np.array([*sa, *ss, eta, area, eps])
+        out[f"oct_{name}_out"] = res
 
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")

@@ -136,3 +136,65 @@ def test_sss_render_is_finite_and_deterministic(gpu_ctx, sss_scene):
     assert all(t["sss_hits"] > 0 for t in sts)
     assert np.array_equal(imgs[0], imgs[1])
     assert np.all(np.isfinite(imgs[0])) and imgs[0].min() >= 0.0
+
+
+# ---- the octree cut (pppm_subsurface_cut) ------------------------------------
+@pytest.mark.parametrize("name", ["sphere", "plane", "sphere_tight"])
+def test_dipole_cut_matches_oracle_octree(gpu_ctx, golden, name):
+    """The GPU's walk of the cut against the oracle's FP64 octree (itself
+    pinned to the reference's Octree) on the same points and queries; the
+    FP32 error test can take or open a node the FP64 one does not, which moves
+    a query by a fraction of that node's share."""
+    say(f"dipole octree cut, {name}")
+    pts, E, q = golden[f"oct_{name}_pts"], golden[f"oct_{name}_E"], golden[f"oct_{name}_q"]
+    prm = golden[f"oct_{name}_params"]
+    sa, ss, eta, area, eps = prm[0:3], prm[3:6], prm[6], prm[7], prm[8]
+    try:
+        gpu_ctx.subsurface_cut(eps)
+        g = gpu_ctx.dipole_eval(_material(sa, ss, eta), pts, E, area, q)
+    finally:
+        gpu_ctx.subsurface_cut(0.0)
+    o, _ = O.dipole_octree(sa, ss, eta, pts, E, area, q, max_error=eps)
+    big = o.sum(1) > 1e-3 * o.sum(1).max()
+    rel = np.abs(g[big] / o[big] - 1)
+    say(f"median {np.median(rel):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) < 2e-5 and rel.max() < 2e-3
+
+
+def test_dipole_cut_without_error_is_the_exact_sum(gpu_ctx):
+    say("dipole cut at max_error 1e-30")
+    rng = np.random.default_rng(4)
+    sa, ss = (0.01, 0.02, 0.04), (1.0, 1.5, 0.7)
+    pts = rng.uniform(-1.5, 1.5, (3000, 3)).astype(np.float32)
+    pts[100:110] = pts[5]  # coincident points: one leaf
+    E = rng.random((3000, 3)).astype(np.float32)
+    q = rng.uniform(-1.5, 1.5, (500, 3)).astype(np.float32)
+    try:
+        gpu_ctx.subsurface_cut(1e-30)
+        g = gpu_ctx.dipole_eval(_material(sa, ss), pts, E, 0.003, q)
+    finally:
+        gpu_ctx.subsurface_cut(0.0)
+    o = O.dipole_sum(sa, ss, 1.3, pts, E, 0.003, q)
+    np.testing.assert_allclose(g, o, rtol=2e-5, atol=1e-12)
+    with pytest.raises(P.PPPMError):
+        gpu_ctx.subsurface_cut(-1.0)
+
+
+def test_sss_render_with_cut_is_close_to_the_exact_sum(gpu_ctx, sss_scene):
+    """The whole render with the reference's cut (0.05) against the exact sum:
+    same photons and eye paths, so the images differ only by the cut."""
+    s = sss_scene
+    say("sss render, cut vs exact")
+    imgs = []
+    for eps in (0.0, 0.05):
+        gpu_ctx.subsurface_cut(eps)
+        _begin(gpu_ctx, s, 200000, 2)
+        sts = [gpu_ctx.render_pass() for _ in range(2)]
+        imgs.append(gpu_ctx.image())
+    gpu_ctx.subsurface_cut(0.0)
+    assert sts[-1]["sss_hits"] > 0
+    exact, cut = imgs
+    d = np.abs(cut - exact).sum() / exact.sum()
+    say(f"relative L1 difference {d:.2e}")
+    assert np.all(np.isfinite(cut)) and cut.min() >= 0.0
+    assert d < 5e-3

@@ -91,3 +91,59 @@ def test_dipole_sum_is_linear_in_irradiance():
     d2 = float(np.sum((q[0].astype(np.float64) - pts[0]) ** 2))
     np.testing.assert_allclose(one, (1 - O.orc().orc_dipole_fdr(1.3)) / math.pi * 0.01 * O.dipole(*MARBLE, 1.3, d2) * E[0],
                                rtol=1e-12)
+
+
+# ---- the octree cut (subsurface_integrator.cc:69-167) -----------------------
+@pytest.mark.parametrize("name", ["sphere", "plane", "sphere_tight"])
+def test_dipole_octree_matches_reference_golden(golden, name):
+    """The oracle's octree cut against the reference's own Octree (construct +
+    iradSubsurface + irradiance's factor), both with the reference's exp."""
+    pts, E, q = golden[f"oct_{name}_pts"], golden[f"oct_{name}_E"], golden[f"oct_{name}_q"]
+    prm, ref = golden[f"oct_{name}_params"], golden[f"oct_{name}_out"]
+    out, vis = O.dipole_octree(prm[0:3], prm[3:6], prm[6], pts, E, prm[7], q, max_error=prm[8], fast=True)
+    np.testing.assert_allclose(out, ref, rtol=1e-11, atol=1e-300)
+    if prm[8] >= 0.05:
+        assert vis.mean() < 0.5 * len(pts)  # the cut is a cut
+
+
+def _cloud(n=1500, seed=5):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    pts = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+    E = rng.uniform(0.0, 2.0, (n, 3)).astype(np.float32)
+    qv = rng.normal(size=(100, 3))
+    q = (qv / np.linalg.norm(qv, axis=1, keepdims=True)).astype(np.float32)
+    return pts, E, q, 4 * np.pi / n
+
+
+def test_dipole_octree_without_cut_is_the_exact_sum():
+    """max_error = 0 accepts no interior node: the walk reaches every leaf."""
+    pts, E, q, area = _cloud()
+    sa, ss = MARBLE
+    out, vis = O.dipole_octree(sa, ss, 1.3, pts, E, area, q, max_error=0.0)
+    exact = O.dipole_sum(sa, ss, 1.3, pts, E, area, q)
+    np.testing.assert_allclose(out, exact, rtol=1e-12)
+    assert vis.min() >= len(pts)
+
+
+def test_dipole_octree_coincident_points_share_a_leaf():
+    """Points at one position (the reference would split them forever) form
+    one leaf standing for all of them."""
+    pts, E, q, area = _cloud(400)
+    pts = np.concatenate([pts, np.repeat(pts[:3], 5, axis=0)])
+    E = np.concatenate([E, np.full((15, 3), 0.7, np.float32)])
+    sa, ss = MARBLE
+    out, _ = O.dipole_octree(sa, ss, 1.3, pts, E, area, q, max_error=0.0)
+    np.testing.assert_allclose(out, O.dipole_sum(sa, ss, 1.3, pts, E, area, q), rtol=1e-12)
+
+
+def test_dipole_octree_cut_error_is_small():
+    """With the reference's bound 0.05 the cut stays within 2 % of the exact
+    sum on a uniformly lit sphere, and the error falls with the bound."""
+    pts, E, q, area = _cloud(4000, seed=9)
+    E[:] = 1.0
+    sa, ss = (0.05, 0.08, 0.12), (1.0, 1.2, 1.5)
+    exact = O.dipole_sum(sa, ss, 1.3, pts, E, area, q)
+    e1 = np.abs(O.dipole_octree(sa, ss, 1.3, pts, E, area, q, max_error=0.05)[0] / exact - 1).max()
+    e2 = np.abs(O.dipole_octree(sa, ss, 1.3, pts, E, area, q, max_error=0.005)[0] / exact - 1).max()
+    assert e1 < 0.02 and e2 < e1
