@@ -353,7 +353,7 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
         solo_ms += 1e3 * (time.perf_counter() - t1)
     cu = max(1, ctx.device_info()["cu"])
     ctx.close()
-    modes = {P.GATHER_LANES: "lanes", P.GATHER_TILED: "tiled", P.GATHER_WAVE: "wave"}
+    modes = {P.GATHER_LANES: "lanes", P.GATHER_TILED: "tiled", P.GATHER_WAVE: "wave", P.GATHER_BLOCKS: "blocks"}
     return {
         "workload": (f"{n_scenes} scenes, triangles log-uniform in [1k, 500k], resolutions from "
                      "{128,256,512,1024}^2, 8 photons per pixel"),
@@ -385,7 +385,7 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--gather-mode", choices=["auto", "lanes", "tiled", "wave"], default="auto")
+    ap.add_argument("--gather-mode", choices=["auto", "lanes", "tiled", "wave", "blocks"], default="auto")
     ap.add_argument("--no-rmse", action="store_true", help="skip the plain-vs-learned time-to-error leg")
     ap.add_argument("--rmse-passes", type=int, default=256)
     ap.add_argument("--rmse-ref-passes", type=int, default=1024)
@@ -421,7 +421,7 @@ def main():
     ctx.set_camera(scene.camera)
     total = args.warmup + args.steps
     mode = {"auto": P.GATHER_AUTO, "lanes": P.GATHER_LANES, "tiled": P.GATHER_TILED,
-            "wave": P.GATHER_WAVE}[args.gather_mode]
+            "wave": P.GATHER_WAVE, "blocks": P.GATHER_BLOCKS}[args.gather_mode]
     ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed, gather_mode=mode)
     for _ in range(args.warmup):
         ctx.render_pass()

@@ -106,7 +106,7 @@ typedef struct {
     uint32_t scene_id;     /* RNG stream id of this scene */
     int32_t hash_size;     /* 0: the number of stored photons of each pass */
     double capacity_factor;/* photon buffer = photons * factor records (0: 8) */
-    int32_t gather_mode;   /* PPPM_GATHER_AUTO (0, the default: resolves to LANES), LANES, TILED or WAVE */
+    int32_t gather_mode;   /* PPPM_GATHER_AUTO (0, the default: resolves to LANES), LANES, TILED, WAVE or BLOCKS */
     int32_t method;        /* PPPM_METHOD_PROBABILISTIC (0) or PPPM_METHOD_HACHISUKA */
 } pppm_render_params;
 
@@ -126,6 +126,7 @@ typedef struct {
 #define PPPM_GATHER_LANES 1 /* one lane per Morton-sorted hit point over its own buckets */
 #define PPPM_GATHER_TILED 2 /* LDS-tiled: a wave's union of buckets staged in 64-photon tiles */
 #define PPPM_GATHER_WAVE 3  /* wave-uniform bucket walk, photons through scalar loads */
+#define PPPM_GATHER_BLOCKS 4 /* wave-uniform walk of the 32-record blocks some lane's ball meets, staged in LDS */
 
 /* Timings and counters of one pass. */
 typedef struct {
@@ -161,6 +162,7 @@ typedef struct {
     int64_t photon_nodes;      /* inner BVH nodes visited by the photon pass */
     int64_t photon_tris;       /* triangles tested by the photon pass */
     int64_t eye_nodes, eye_tris;
+    int64_t gather_staged;     /* records staged in LDS, once per wave (blocks kernel): x 64 lane tests */
 } pppm_pass_stats;
 
 /* ---- context ---- */

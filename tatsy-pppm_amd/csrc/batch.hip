@@ -78,7 +78,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
                                         &dc->photons_stored, nullptr, true);
     launch_gather_layout(s, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, cap,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
-                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
+                         gather_compact(ctx->rp.gather_mode) ? nullptr : ctx->ph_c.as<float4>(),
                          g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr, 2);
     EyePassArgs ea = eye_args(ctx, 0);
     ea.pd = pd;
@@ -241,7 +241,7 @@ int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid) {
         Ctx* c = sc.ctx;
         const pppm_render_params base = c->rp;
         std::vector<int> modes;
-        if (tune_gather) modes = {GATHER_LANES, GATHER_TILED, GATHER_WAVE};
+        if (tune_gather) modes = {GATHER_LANES, GATHER_BLOCKS, GATHER_TILED, GATHER_WAVE};
         else modes = {base.gather_mode};
         std::vector<int> grids;
         const int cu = c->cu > 0 ? c->cu : 256;

@@ -78,7 +78,7 @@ int grid_bucket_bits(uint32_t size, bool sentinel) {
 // The bucket bits always count a sentinel value, so single passes and
 // enqueued (batched) passes order a bucket's records the same way.
 int render_sub_bits(const Ctx* c, uint32_t size) {
-    if (c->rp.gather_mode != GATHER_LANES) return 0;
+    if (!gather_compact(c->rp.gather_mode)) return 0;
     return std::max(0, std::min(3, (32 - grid_bucket_bits(size, true)) / 3));
 }
 
@@ -696,7 +696,7 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
                 int64_t n_hit, double radius, int32_t hash_size, const double bbox_min[3],
                 int32_t gather_mode, float* out_rgb, pppm_pass_stats* st) {
     CHECK_CTX(ctx);
-    if (gather_mode < PPPM_GATHER_AUTO || gather_mode > PPPM_GATHER_WAVE)
+    if (gather_mode < PPPM_GATHER_AUTO || gather_mode > PPPM_GATHER_BLOCKS)
         return ctx->fail(PPPM_ERR_INVALID, "unknown gather mode");
     if (gather_mode == PPPM_GATHER_AUTO) gather_mode = GATHER_AUTO_CHOICE;
     if (n_ph < 0 || n_hit < 0 || hash_size <= 0 || !(radius > 0.0) || !bbox_min ||
@@ -707,9 +707,10 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     hipStream_t s = ctx->stream;
     const int64_t np = n_ph > 0 ? n_ph : 1, nh = n_hit > 0 ? n_hit : 1;
     // staging: photons [3 x 3 floats] + hit points [3 x 3 floats] + outputs
-    // (the lanes kernel's reads past the sorted positions p4s land in f4s)
+    // (the lanes kernel's reads past the sorted positions p4s land in f4s and
+    // what follows; GATHER_PAD records at the end keep them inside)
     size_t bytes = sizeof(float) * 9 * (size_t)(np + nh) + sizeof(float4) * 5 * (size_t)np +
-                   sizeof(float4) * 2 * (size_t)nh + sizeof(float) * 3 * (size_t)nh;
+                   sizeof(float4) * 2 * (size_t)nh + sizeof(float) * 3 * (size_t)nh + sizeof(float4) * GATHER_PAD;
     TRY_HIP(ctx, ctx->scratch.ensure(bytes));
     float* f = ctx->scratch.as<float>();
     float* dpp = f; float* dpf = dpp + 3 * np; float* dpn = dpf + 3 * np;
@@ -743,10 +744,10 @@ int pppm_gather(void* ctx_raw, const float* ph_pos, const float* ph_flux, const 
     g.scale = 1.0 / (2.0 * radius);
     g.size = (uint32_t)hash_size;
     // the lanes kernel as in a pass: Morton order inside buckets, block boxes
-    if (gather_mode == GATHER_LANES)
+    if (gather_compact(gather_mode))
         g.sub_bits = std::max(0, std::min(3, (32 - grid_bucket_bits(g.size, true)) / 3));
     uint32_t* sorted = build_grid(ctx, p4, nullptr, nullptr, n_ph, g, -1);
-    launch_gather_layout(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s, gather_mode == GATHER_LANES ? nullptr : c4s);
+    launch_gather_layout(s, p4, f4, nullptr, sorted, n_ph, p4s, f4s, gather_compact(gather_mode) ? nullptr : c4s);
     if (g.sub_bits > 0) {
         TRY_HIP(ctx, ctx->boxes.ensure(sizeof(float4) * 2 * (size_t)gather_box_count(n_ph)));
         launch_gather_boxes(s, p4s, n_ph, ctx->boxes.as<float4>());

@@ -175,39 +175,39 @@ __device__ __forceinline__ void flush_counters(const GatherArgs& a, bool valid, 
     }
 }
 
-// The records [j0, j1) of one bucket: four 16-byte records in flight per wait
-// (up to three records past the range read and masked out), flux loads issued
-// before the sums, accumulation in record order.
+// The records [j0, j1) of one bucket: LANES_INFLIGHT 16-byte records in
+// flight per wait (records past the range read and masked out), flux loads
+// issued before the sums, accumulation in record order.
 __device__ __forceinline__ void lanes_range(const GatherArgs& a, uint32_t j0, uint32_t j1, v3f x, v3f nx, float r2,
                                             float inv_kr, bool test_normals, float& ax, float& ay, float& az,
                                             uint32_t& ni) {
     const float4* __restrict__ pa = a.ph_a;
     const float4* __restrict__ pb = a.ph_b;
-    for (uint32_t j = j0; j < j1; j += 4) {
-        // up to three records past the range are read and masked out: the
-        // layout array holds GATHER_PAD records beyond its last one
+    for (uint32_t j = j0; j < j1; j += LANES_INFLIGHT) {
+        // up to LANES_INFLIGHT - 1 records past the range are read and masked
+        // out: the layout array holds GATHER_PAD records beyond its last one
         const float4* __restrict__ pj = pa + j;
-        float4 A[4];
+        float4 A[LANES_INFLIGHT];
 #pragma unroll
-        for (int q = 0; q < 4; q++) A[q] = pj[q];
+        for (int q = 0; q < LANES_INFLIGHT; q++) A[q] = pj[q];
         // the normal word is read with the position (one 16-byte load)
 #pragma unroll
-        for (int q = 0; q < 4; q++) asm volatile("" ::"v"(A[q].w));
-        float d2[4];
-        bool in[4];
+        for (int q = 0; q < LANES_INFLIGHT; q++) asm volatile("" ::"v"(A[q].w));
+        float d2[LANES_INFLIGHT];
+        bool in[LANES_INFLIGHT];
         // the normal test is skipped where the block's normals agree as one
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < LANES_INFLIGHT; q++) {
             d2[q] = dist2(x, A[q]);
             in[q] = j + q < j1 && d2[q] <= r2 && (!test_normals || normal_agrees(__float_as_uint(A[q].w), nx));
         }
-        float4 F[4];
+        float4 F[LANES_INFLIGHT];
         const float4* __restrict__ pbj = pb + j;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < LANES_INFLIGHT; q++)
             if (in[q]) F[q] = pbj[q];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < LANES_INFLIGHT; q++) {
             if (in[q]) {
                 const float wt = cone_weight(d2[q], inv_kr);
                 ax += wt * F[q].x;
@@ -330,7 +330,8 @@ __device__ __forceinline__ void lane_hit(const GatherArgs& a, int64_t n, int64_t
 // lane's pending set; false when the union is exhausted.
 // The union is walked in ascending bucket id: the wave minimum of the lanes'
 // smallest pending buckets.
-__device__ __forceinline__ bool next_union_bucket(const LaneHit& h, uint32_t& pending, uint32_t& bucket) {
+template <class Hit>
+__device__ __forceinline__ bool next_union_bucket(const Hit& h, uint32_t& pending, uint32_t& bucket) {
     if (__ballot(pending != 0u) == 0) return false;
     uint32_t v = min_bucket(h.mb, pending);
     for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
@@ -443,6 +444,188 @@ __global__ __launch_bounds__(64) void k_gather_tiled(GatherArgs a) {
     }
     if (h.valid) write_result(a, (int64_t)__float_as_uint(h.hw.w), xyz(h.hw), acc.x, acc.y, acc.z);
     flush_counters(a, h.valid, h.nb, h.nc, acc.n, h.nc, 0);
+}
+
+// Orders a wave's LDS writes before its later LDS reads (the LDS serves one
+// wave's operations in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define BLOCKS_UNROLL 4
+// The m (<= GATHER_BLOCK) records staged at rec (positions, then fluxes at
+// rec + GATHER_BLOCK) for one lane: BLOCKS_UNROLL records per step, read
+// before any is used; slots past m hold stale records and are masked out (weight and flux
+// zero).  TN: test each record's normal; MARK: used[t] = 1 for accepted t.
+template <bool TN, bool MARK>
+__device__ __forceinline__ void blocks_walk(const float4* rec, uint32_t m, bool need, v3f x, v3f nx, float r2,
+                                            float inv_kr, uint32_t* used, float& ax, float& ay, float& az,
+                                            uint32_t& ni) {
+    for (uint32_t t = 0; t < m; t += BLOCKS_UNROLL) {
+        float4 A[BLOCKS_UNROLL], F[BLOCKS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < BLOCKS_UNROLL; u++) A[u] = rec[t + u];
+#pragma unroll
+        for (int u = 0; u < BLOCKS_UNROLL; u++) F[u] = rec[GATHER_BLOCK + t + u];
+#pragma unroll
+        for (int u = 0; u < BLOCKS_UNROLL; u++) {
+            const float d2 = dist2(x, A[u]);
+            bool in = need && t + u < m && d2 <= r2;
+            if (TN) in = in && normal_agrees(__float_as_uint(A[u].w), nx);
+            const float wt = in ? cone_weight(d2, inv_kr) : 0.0f;
+            ax = fmaf(wt, in ? F[u].x : 0.0f, ax);
+            ay = fmaf(wt, in ? F[u].y : 0.0f, ay);
+            az = fmaf(wt, in ? F[u].z : 0.0f, az);
+            ni += in ? 1u : 0u;
+            if (MARK && in) used[t + u] = 1u;
+        }
+    }
+}
+
+// One wave per 64 Morton-adjacent hit points, four independent waves per
+// workgroup.  The wave walks the union of its lanes' buckets block by block
+// (GATHER_BLOCK records): a lane needs a block when the block's bucket is one
+// of its own, the block's box meets its ball and, for a block of one normal
+// code, that code agrees with its normal -- the lanes kernel's decisions, so
+// each lane accepts the same records.  A block no lane needs is skipped by
+// the whole wave; otherwise its records are staged in LDS (position words by
+// lanes 0-31, flux words by lanes 32-63, one coalesced load) and every lane
+// walks them in step, reading each record by broadcast.  Per lane the sum
+// runs over its accepted records in union order (ascending bucket id).
+// Squared distance between the boxes (alo, ahi) and (lo, hi), zero where they
+// meet: a lower bound, in FP32 too, of box_dist2(x, lo, hi) for x in the first.
+__device__ __forceinline__ float box_box_dist2(const float alo[3], const float ahi[3], float4 lo, float4 hi) {
+    const float dx = fmaxf(fmaxf(lo.x - ahi[0], alo[0] - hi.x), 0.0f);
+    const float dy = fmaxf(fmaxf(lo.y - ahi[1], alo[1] - hi.y), 0.0f);
+    const float dz = fmaxf(fmaxf(lo.z - ahi[2], alo[2] - hi.z), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// A lane's hit point in the blocks kernel: the fields next_union_bucket reads
+struct BlockHit {
+    bool valid;
+    v3f x, nx;
+    uint32_t mb[8];
+    uint32_t vm;  // valid, de-duplicated slots of mb
+};
+
+// Stages the m records [j0, j0 + m) of one block (positions by lanes 0-31,
+// fluxes by lanes 32-63, one coalesced load) and walks them for this lane.
+template <bool MARK>
+__device__ __forceinline__ void blocks_stage_walk(const GatherArgs& a, float4* rec, int lane, uint32_t j0, uint32_t m,
+                                                  bool need, bool test_normals, const BlockHit& h, float r2,
+                                                  float inv_kr, float& ax, float& ay, float& az, uint32_t& ni) {
+    const uint32_t q = (uint32_t)lane & (GATHER_BLOCK - 1);
+    if (q < m) rec[lane] = lane < GATHER_BLOCK ? a.ph_a[j0 + q] : a.ph_b[j0 + q];
+    wave_lds_sync();
+    if (test_normals)
+        blocks_walk<true, MARK>(rec, m, need, h.x, h.nx, r2, inv_kr, a.used + j0, ax, ay, az, ni);
+    else
+        blocks_walk<false, MARK>(rec, m, need, h.x, h.nx, r2, inv_kr, a.used + j0, ax, ay, az, ni);
+    wave_lds_sync();  // every lane has read the block before the next one is staged
+}
+
+template <bool MARK>
+__global__ __launch_bounds__(256) void k_gather_blocks(GatherArgs a) {
+    __shared__ float4 s_rec[4][2 * GATHER_BLOCK];
+    __shared__ float4 s_box[4][2 * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t n = a.hp_count_src ? (int64_t)a.hp_count_src->hitpoints : a.n_hit;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (((int64_t)blockIdx.x * 256 + (threadIdx.x & ~63)) >= n) return;  // idle wave (uniform)
+    float4* rec = s_rec[threadIdx.x >> 6];
+    float4* sbox = s_box[threadIdx.x >> 6];
+    // the lane's hit point (its weight is read at the end)
+    BlockHit h;
+    h.valid = i < n;
+    const float4 hp = h.valid ? a.hp_pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    h.x = xyz(hp);
+    h.nx = h.valid ? oct_decode(__float_as_uint(hp.w)) : mk(0.0f, 0.0f, 0.0f);
+    {
+        const uint32_t vm = own_buckets(a, hp, h.mb);
+        h.vm = h.valid ? vm : 0u;
+    }
+    uint32_t nb = (uint32_t)__popc(h.vm), nc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((h.vm >> k) & 1u) nc += a.off[h.mb[k] + 1] - a.off[h.mb[k]];
+    uint32_t pending = h.vm;
+    const float r2 = (float)(gather_radius(a) * gather_radius(a));
+    const float r2_box = r2 * (1.0f + 1.0f / 4096.0f);
+    const float inv_kr = (float)(1.0 / (1.1 * gather_radius(a)));
+    // the box of the wave's hit points: a block farther from it than the
+    // radius is needed by no lane
+    float wl[3] = {h.valid ? h.x.x : 3.0e38f, h.valid ? h.x.y : 3.0e38f, h.valid ? h.x.z : 3.0e38f};
+    float wh[3] = {h.valid ? h.x.x : -3.0e38f, h.valid ? h.x.y : -3.0e38f, h.valid ? h.x.z : -3.0e38f};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            wl[d] = fminf(wl[d], __shfl_xor(wl[d], off));
+            wh[d] = fmaxf(wh[d], __shfl_xor(wh[d], off));
+        }
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    uint32_t ni = 0, nt = 0, nk = 0, staged = 0;
+    uint32_t b;
+    while (next_union_bucket(h, pending, b)) {
+        bool own = false;
+#pragma unroll
+        for (int c = 0; c < 8; c++) own |= ((h.vm >> c) & 1u) && h.mb[c] == b;
+        const uint32_t b0 = a.off[b], b1 = a.off[b + 1];
+        if (b1 <= b0) continue;
+        const uint32_t k0 = b0 / GATHER_BLOCK, k1 = (b1 - 1) / GATHER_BLOCK;
+        if (!a.boxes) {  // no block boxes: every block of the bucket, no culling
+            for (uint32_t k = k0; k <= k1; k++) {
+                const uint32_t j0 = max(b0, k * GATHER_BLOCK), m = min(b1, (k + 1) * GATHER_BLOCK) - j0;
+                staged += m;
+                nt += own ? m : 0u;
+                blocks_stage_walk<MARK>(a, rec, lane, j0, m, own, true, h, r2, inv_kr, ax, ay, az, ni);
+            }
+            continue;
+        }
+        nk += own ? k1 - k0 + 1 : 0u;
+        // 64 block boxes per step, one per lane: a coalesced load into LDS
+        // and the test against the wave's box; then the candidates one by one
+        for (uint32_t kc = k0; kc <= k1; kc += 64) {
+            const uint32_t kk = kc + (uint32_t)lane;
+            bool cand = false;
+            if (kk <= k1) {
+                const float4 lo = a.boxes[2 * kk], hi = a.boxes[2 * kk + 1];
+                sbox[2 * lane] = lo;
+                sbox[2 * lane + 1] = hi;
+                cand = box_box_dist2(wl, wh, lo, hi) <= r2_box;
+            }
+            unsigned long long cm = __ballot(cand);
+            wave_lds_sync();
+            while (cm) {
+                const int t = __builtin_ctzll(cm);
+                cm &= cm - 1;
+                const float4 lo = sbox[2 * t];
+                bool need = own && box_dist2(h.x, lo, sbox[2 * t + 1]) <= r2_box;
+                bool test_normals = true;
+                const uint32_t eb = __float_as_uint(lo.w);
+                if (eb != MIXED_NORMALS) {  // one normal code for the whole block
+                    need = need && normal_agrees(eb, h.nx);
+                    test_normals = false;
+                }
+                if (__ballot(need) == 0) continue;  // no lane's ball meets the block
+                const uint32_t k = kc + (uint32_t)t;
+                const uint32_t j0 = max(b0, k * GATHER_BLOCK), m = min(b1, (k + 1) * GATHER_BLOCK) - j0;
+                staged += m;
+                nt += need ? m : 0u;
+                blocks_stage_walk<MARK>(a, rec, lane, j0, m, need, test_normals, h, r2, inv_kr, ax, ay, az, ni);
+            }
+            wave_lds_sync();  // the boxes are read before the next step overwrites them
+        }
+    }
+    if (h.valid) {
+        const float4 hw = a.hp_w[i];
+        write_result(a, (int64_t)__float_as_uint(hw.w), xyz(hw), ax, ay, az);
+    }
+    flush_counters(a, h.valid, nb, nc, ni, nt, nk);
+    if (lane == 0 && a.counters) atomicAdd(&a.counters->gather_staged, (unsigned long long)staged);
 }
 
 // The box and shared normal code of the block of GATHER_BLOCK records that
@@ -565,7 +748,7 @@ __global__ __launch_bounds__(256) void k_mark_used(GatherArgs a, const uint32_t*
     const v3f x = xyz(hp);
     const v3f nx = oct_decode(__float_as_uint(hp.w));
     const float r2 = (float)(gather_radius(a) * gather_radius(a));
-    const bool compact = a.mode == GATHER_LANES;
+    const bool compact = gather_compact(a.mode);
     uint32_t mb[8];
     const uint32_t todo = own_buckets(a, hp, mb);
     for (int c = 0; c < 8; c++) {
@@ -612,6 +795,10 @@ void launch_gather(hipStream_t s, const GatherArgs& a) {
     if (a.n_hit <= 0) return;
     if (a.mode == GATHER_LANES)
         hipLaunchKernelGGL(k_gather_lanes, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
+    else if (a.mode == GATHER_BLOCKS && a.used)
+        hipLaunchKernelGGL(k_gather_blocks<true>, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
+    else if (a.mode == GATHER_BLOCKS)
+        hipLaunchKernelGGL(k_gather_blocks<false>, dim3(grid1(a.n_hit, 256)), dim3(256), 0, s, a);
     else if (a.mode == GATHER_TILED)
         hipLaunchKernelGGL(k_gather_tiled, dim3(grid1(a.n_hit, 64)), dim3(64), 0, s, a);
     else

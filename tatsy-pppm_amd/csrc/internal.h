@@ -115,6 +115,7 @@ struct Counters {
     unsigned long long gather_blocks;  // block boxes tested by the lanes gather
     // traversal work, counted by the counting kernels only
     unsigned long long photon_nodes, photon_tris, eye_nodes, eye_tris;
+    unsigned long long gather_staged;  // records staged per wave by the blocks gather
 };
 
 // Persistent photon-pass blocks per CU: at least the blocks one CU holds at
@@ -192,6 +193,9 @@ void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, flo
 #define GATHER_TILED PPPM_GATHER_TILED
 #define GATHER_LANES PPPM_GATHER_LANES
 #define GATHER_WAVE PPPM_GATHER_WAVE
+#define GATHER_BLOCKS PPPM_GATHER_BLOCKS
+// the lanes and blocks kernels read the compact two-word layout with block boxes
+inline __host__ __device__ bool gather_compact(int32_t mode) { return mode == GATHER_LANES || mode == GATHER_BLOCKS; }
 // PPPM_GATHER_AUTO resolves to the kernel measured fastest on MI355X
 #define GATHER_AUTO_CHOICE GATHER_LANES
 
@@ -199,7 +203,7 @@ void launch_kahan_probe(hipStream_t s, const float* v, int64_t n, int lanes, flo
 #define GATHER_BLOCK 32
 
 struct GatherArgs {
-    int32_t mode;           // GATHER_WAVE, GATHER_TILED or GATHER_LANES
+    int32_t mode;           // GATHER_LANES, GATHER_BLOCKS, GATHER_TILED or GATHER_WAVE
     // photons in bucket order.  Lanes: (x, y, z, oct normal) (Phi, key); the
     // others: (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g) (Phi_b, key, -, -)
     const float4* ph_a;
@@ -243,9 +247,12 @@ void launch_hit_keys(hipStream_t s, const float4* hp_pos, const float4* hp_w, in
 void launch_gather_boxes(hipStream_t s, const float4* a, int64_t n, float4* boxes);
 #define MIXED_NORMALS 0x80008000u
 inline int64_t gather_box_count(int64_t n) { return (n + GATHER_BLOCK - 1) / GATHER_BLOCK; }
-// Records a lanes-layout position array holds past its capacity: the lanes
-// kernel reads positions four at a time and masks out those past a bucket.
-#define GATHER_PAD 4
+// Records the lanes kernel reads per wait, and the records a lanes-layout
+// position array holds past its capacity (reads past a bucket are masked out).
+#ifndef LANES_INFLIGHT
+#define LANES_INFLIGHT 8
+#endif
+#define GATHER_PAD 8
 void launch_gather_layout(hipStream_t s, const float4* pos, const float4* flux, const uint32_t* perm,
                           const uint32_t* sorted, int64_t n, float4* A, float4* B, float4* C,
                           float4* boxes = nullptr, int stride = 1);

@@ -85,7 +85,7 @@ int pppm_render_begin(void* ctx_raw, const pppm_render_params* p) {
     if (!ctx->has_cam) return ctx->fail(PPPM_ERR_STATE, "no camera set");
     if (!p || p->photons <= 0 || p->passes <= 0 || !(p->alpha > 0.0 && p->alpha < 1.0) ||
         !(p->radius1 > 0.0) || p->max_bounces <= 0 || p->max_bounces > 64 || p->hash_size < 0 ||
-        (p->gather_mode < PPPM_GATHER_AUTO || p->gather_mode > PPPM_GATHER_WAVE) ||
+        (p->gather_mode < PPPM_GATHER_AUTO || p->gather_mode > PPPM_GATHER_BLOCKS) ||
         (p->method != PPPM_METHOD_PROBABILISTIC && p->method != PPPM_METHOD_HACHISUKA))
         return ctx->fail(PPPM_ERR_INVALID, "bad render parameters");
     if (p->method == PPPM_METHOD_HACHISUKA && ctx->n_dips > 0)
@@ -300,7 +300,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     uint32_t* sorted = build_grid(ctx, nullptr, nullptr, perm1, n, g, keep, nullptr, nullptr, true);
     launch_gather_layout(s, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, n,
                          ctx->ph_a.as<float4>(), ctx->ph_b.as<float4>(),
-                         ctx->rp.gather_mode == GATHER_LANES ? nullptr : ctx->ph_c.as<float4>(),
+                         gather_compact(ctx->rp.gather_mode) ? nullptr : ctx->ph_c.as<float4>(),
                          g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr, 2);
     TRY_HIP(ctx, hipGetLastError());
     TRY_HIP(ctx, hipEventRecord(ctx->ev[3], s));
@@ -326,7 +326,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
     // contribution records need the photons the gather accepts: the lanes
     // kernel marks them as it goes, the others through a separate pass
     ga.boxes = g.sub_bits > 0 ? ctx->boxes.as<float4>() : nullptr;
-    const bool mark_in_gather = ctx->prop.enabled && n > 0 && ga.mode == GATHER_LANES;
+    const bool mark_in_gather = ctx->prop.enabled && n > 0 && gather_compact(ga.mode);
     ga.used = mark_in_gather ? ctx->prop_used.as<uint32_t>() : nullptr;
     if (mark_in_gather) launch_zero_u32(s, ctx->prop_used.p, n);
     launch_gather(s, ga);
@@ -436,6 +436,7 @@ int pppm_render_pass(void* ctx_raw, pppm_pass_stats* st) {
         st->photon_tris = (int64_t)hc.photon_tris;
         st->eye_nodes = (int64_t)hc.eye_nodes;
         st->eye_tris = (int64_t)hc.eye_tris;
+        st->gather_staged = (int64_t)hc.gather_staged;
         st->photon_ms = t[0];
         st->order_ms = t[1];
         st->grid_ms = t[2];
@@ -648,7 +649,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         ctx->stream = keep_stream;
         launch_gather_layout(A, ctx->ph_rec.as<float4>(), ctx->ph_rec.as<float4>() + 1, nullptr, sorted, n,
                              S.ph_a->as<float4>(), S.ph_b->as<float4>(),
-                             ctx->rp.gather_mode == GATHER_LANES ? nullptr : S.ph_c->as<float4>(),
+                             gather_compact(ctx->rp.gather_mode) ? nullptr : S.ph_c->as<float4>(),
                              g.sub_bits > 0 ? S.boxes->as<float4>() : nullptr, 2);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[ODONE], A));
@@ -740,6 +741,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
             st.photon_tris = (int64_t)c.photon_tris;
             st.eye_nodes = (int64_t)c.eye_nodes;
             st.eye_tris = (int64_t)c.eye_tris;
+            st.gather_staged = (int64_t)c.gather_staged;
             st.photon_ms = t[0];
             st.order_ms = t[1];
             st.grid_ms = 0.0;

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpppm_hip.so")
 OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
-GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE = 0, 1, 2, 3
+GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE, GATHER_BLOCKS = 0, 1, 2, 3, 4
 METHOD_PROBABILISTIC, METHOD_HACHISUKA = 0, 1
 PROPOSAL_CELLS = 1024
 MLP_PARAMS = 76992
@@ -63,7 +63,7 @@ class PassStats(C.Structure):
                 ("sss_hits", C.c_int64), ("sss_points", C.c_int64),
                 ("gather_tested", C.c_int64), ("gather_blocks", C.c_int64),
                 ("photon_nodes", C.c_int64), ("photon_tris", C.c_int64), ("eye_nodes", C.c_int64),
-                ("eye_tris", C.c_int64)]
+                ("eye_tris", C.c_int64), ("gather_staged", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -157,7 +157,8 @@ def test_batch_of_64_scenes_from_the_batched_workload(gpu_ctx):
         assert img.shape == (s.camera.height, s.camera.width, 3)
         assert np.all(np.isfinite(img)) and img.min() >= 0.0 and img.max() > 0.0, k
         info = b.info(k)
-        assert 0 <= info["branch"] < 4 and info["gather_mode"] in (P.GATHER_LANES, P.GATHER_TILED, P.GATHER_WAVE)
+        assert 0 <= info["branch"] < 4 and info["gather_mode"] in (P.GATHER_LANES, P.GATHER_TILED, P.GATHER_WAVE,
+                                                                    P.GATHER_BLOCKS)
     # the largest scene equals its solo render bit for bit
     k = int(np.argmax([s.n_tris * s.camera.width for s, _ in scenes]))
     s, n = scenes[k]

@@ -230,7 +230,7 @@ def test_eye_pass_matches_oracle(gpu_ctx, c1_small):
     assert np.mean(np.sum(hn[sel] * on[p], axis=1) > 0.9999) > 0.999
 
 
-@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES])
+@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES, P.GATHER_BLOCKS])
 def test_gather_matches_oracle(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
     say(f"gather vs oracle mode={mode}")
@@ -259,7 +259,7 @@ def rel_rmse(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.mean(b))
 
 
-@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES])
+@pytest.mark.parametrize("mode", [P.GATHER_WAVE, P.GATHER_TILED, P.GATHER_LANES, P.GATHER_BLOCKS])
 def test_render_matches_oracle_equal_photons(gpu_ctx, c1_small, mode):
     s, os_ = c1_small
     say(f"render vs oracle (4 passes) gather mode={mode}")
@@ -318,16 +318,20 @@ def test_full_size_pass_properties(gpu_ctx):
     st_l = _render_one_pass(gpu_ctx, s, 4_000_000, r1, gather_mode=P.GATHER_LANES)[0]
     img_lanes = gpu_ctx.image()
     den = np.maximum(np.abs(img_lanes), 1e-6 * img_lanes.max())
-    for mode in (P.GATHER_TILED, P.GATHER_WAVE):
+    for mode in (P.GATHER_TILED, P.GATHER_WAVE, P.GATHER_BLOCKS):
         # the gather kernels sum the same photon set in different orders
         st = _render_one_pass(gpu_ctx, s, 4_000_000, r1, gather_mode=mode)[0]
         say(f"mode {mode} stats {st}")
         img = gpu_ctx.image()
         assert st_l["gather_inradius"] == st["gather_inradius"]
         assert st_l["gather_candidates"] == st["gather_candidates"]
-        # the whole-bucket kernels test every candidate; the lanes kernel skips
-        # the blocks whose box misses the ball, and accepts the same photons
-        assert st["gather_tested"] == st["gather_candidates"] and st["gather_blocks"] == 0
+        if mode == P.GATHER_BLOCKS:
+            # per lane the lanes kernel's block decisions
+            assert st["gather_tested"] == st_l["gather_tested"] and st["gather_blocks"] == st_l["gather_blocks"]
+        else:
+            # the whole-bucket kernels test every candidate; the lanes kernel
+            # skips the blocks whose box misses the ball, and accepts the same photons
+            assert st["gather_tested"] == st["gather_candidates"] and st["gather_blocks"] == 0
         assert 0 < st_l["gather_tested"] < 0.6 * st_l["gather_candidates"] and st_l["gather_blocks"] > 0
         assert np.quantile(np.abs(img - img_lanes) / den, 0.999) < 1e-4
     n = st["photons_stored"]
