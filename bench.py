@@ -215,6 +215,87 @@ def error_study(P, np, photons, width, passes, ref_passes, lam, probe_every=8):
     }
 
 
+def oracle_error_study(P, np, lam, probe_every=8):
+    """Time to error against the FP64 oracle image of the occluded-light scene
+    at 64 x 64 (tests/golden/occluded_reference_64.npz: the oracle's P-KZ, four
+    seeds x 256 passes x 1 M photons, written by make_occluded_reference.py),
+    plain against learned emission at the same configuration (1 M photons per
+    pass, up to 256 passes, GPU seed 0x7A75).
+
+    Error = sqrt(mean((GPU - oracle)^2) - mean(oracle stderr^2)) / mean(oracle):
+    the RMSE on linear RGB, the oracle's own noise removed, relative to its
+    mean pixel value.  Reports each method's time to relative RMSE 1e-2 (null
+    when not reached in 256 passes), and the learned run's time to plain's
+    error after 256 passes (speedup = plain time / learned time)."""
+    path = os.path.join(ROOT, "tests", "golden", "occluded_reference_64.npz")
+    if not os.path.exists(path):
+        return {"skipped": f"{path} missing (tests/golden/make_occluded_reference.py writes it)"}
+    ref = np.load(path, allow_pickle=False)
+    oracle, stderr, cfg = ref["image"], ref["stderr"], ref["config"]
+    res, photons, passes = int(cfg[0]), int(cfg[1]), int(cfg[2])
+    scene = P.scenegen(P.SCENE_TWO_ROOMS, 1, 100, res, res)
+    r1 = 0.01 * scene.diagonal()
+    ctx = P.Context(0)
+    ctx.upload(scene)
+    ctx.build_bvh()
+    ctx.set_camera(scene.camera)
+    mean_o = float(oracle.mean())
+    floor = float(np.mean(stderr ** 2))
+
+    def rel(img):
+        return float(np.sqrt(max(float(np.mean((img - oracle) ** 2)) - floor, 0.0))) / mean_o
+
+    def run(learned):
+        ctx.begin(photons, passes, r1, seed=0x7A75)
+        if learned:
+            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=64, batch=16384,
+                                ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
+        t, curve = 0.0, []
+        for i in range(1, passes + 1):
+            t0 = time.perf_counter()
+            ctx.render_pass()
+            t += time.perf_counter() - t0
+            if i % probe_every == 0 or i == 1:
+                curve.append((i, t, rel(ctx.image())))
+        return t, curve
+
+    def time_to(curve, target):
+        prev = None
+        for i, t, e in curve:
+            if e <= target:
+                if prev is None:
+                    return t
+                t0, e0 = prev
+                return t0 + (t - t0) * (e0 - target) / max(e0 - e, 1e-30)
+            prev = (t, e)
+        return None
+
+    t_plain, c_plain = run(False)
+    t_learned, c_learned = run(True)
+    ctx.close()
+    target = c_plain[-1][2]
+    reach = time_to(c_learned, target)
+    return {
+        "scene": "occluded light (two 6^3 rooms, 0.2 x 2 slit), 200k-triangle displaced sphere in room B",
+        "resolution": [res, res],
+        "photons_per_pass": photons,
+        "reference": "FP64 oracle P-KZ, 4 seeds x 256 passes x 1 M photons (tests/golden/occluded_reference_64.npz)",
+        "error": "sqrt(MSE - mean oracle stderr^2) / mean(oracle), linear RGB",
+        "oracle_rel_noise": float(np.sqrt(floor)) / mean_o,
+        "plain_time_to_1e-2_s": time_to(c_plain, 1e-2),
+        "learned_time_to_1e-2_s": time_to(c_learned, 1e-2),
+        "plain_rel_at_256": target,
+        "learned_rel_at_256": c_learned[-1][2],
+        "plain_s": t_plain,
+        "learned_s_to_plain_error": reach,
+        "speedup": (t_plain / reach) if reach else None,
+        "learned_total_s": t_learned,
+        "lambda": lam,
+        "curve_plain": [(i, round(t, 4), round(e, 5)) for i, t, e in c_plain],
+        "curve_learned": [(i, round(t, 4), round(e, 5)) for i, t, e in c_learned],
+    }
+
+
 # Dipole sum issue model (gfx950, wave64): the inner loop of k_dipole issues
 # DIPOLE_VALU full-rate and DIPOLE_TRANS transcendental (v_rsq_f32, v_exp_f32)
 # VALU instructions per pair (counted in its AMD assembly); their cycle costs
@@ -541,6 +622,7 @@ def main():
         "photons_stored_per_pass": sum(s["photons_stored"] for s in stats) / args.steps,
         "hitpoints_per_pass": sum(s["hitpoints"] for s in stats) / args.steps,
         "time_to_rmse": None,
+        "time_to_rmse_vs_fp64_oracle": None,
     }
     if args.copy_calibration > 0:
         copy_ms = ctx.stream_copy(args.copy_calibration)
@@ -550,6 +632,7 @@ def main():
     if not args.no_rmse:
         out["time_to_rmse"] = error_study(P, np, args.photons, args.width, args.rmse_passes,
                                           args.rmse_ref_passes, args.rmse_lambda)
+        out["time_to_rmse_vs_fp64_oracle"] = oracle_error_study(P, np, args.rmse_lambda)
     if not args.no_sss:
         out["subsurface"] = subsurface_study(P, np, args.width, args.sss_photons, args.sss_passes,
                                              args.sss_points, args.sss_frequency)

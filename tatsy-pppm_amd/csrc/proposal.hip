@@ -13,7 +13,8 @@
 //    photon order (a prefix scan, no atomics) into a ring.
 //  * MLP: Fourier encoding of (u, v) (32) -> 64 -> 64 -> 64 (ReLU) -> 1024
 //    logits, BF16 weights and activations on gfx950's v_mfma_f32_16x16x32_bf16
-//    with FP32 accumulation; training minimises the contribution-weighted cross
+//    with FP32 accumulation, the output layer split across the four waves of
+//    a workgroup; training minimises the contribution-weighted cross
 //    entropy of the record's cell (the KL divergence to the contribution
 //    density); Adam in FP32.  Weight gradients are split-K partial sums
 //    reduced in a fixed order, so training is deterministic.
@@ -55,6 +56,14 @@ __device__ __forceinline__ void st4(uint16_t* base, int64_t n, int row, int64_t 
     w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
     w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
     *(uint2*)(base + (int64_t)row * n + s) = w;
+}
+
+// Orders a wave's LDS writes before its later LDS reads (one wave's LDS
+// operations are served in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 #define LDX 40  // LDS row strides in bf16: 16-byte aligned rows, padded against bank conflicts
@@ -119,25 +128,37 @@ __device__ __forceinline__ void dense_back(const uint16_t* Din, const uint16_t* 
 }
 
 // Forward (INFER: softmax probabilities) or forward + backward to the
-// activation gradients (TRAIN).  256 threads = 4 waves x 16 samples.
+// activation gradients (TRAIN) of 16 samples per workgroup of four waves.
+// Wave 0 encodes the input and runs the hidden layers (and, for TRAIN, the
+// backward through them); the 1024-way output layer is split by column, wave
+// w taking columns [256 w, 256 w + 256): the softmax statistics combine the
+// four waves' partial maxima and sums in wave order, and dH3 the four
+// partial products in wave order, so the result does not depend on timing.
+// A batch of B samples is B / 16 workgroups: four waves per SIMD at the
+// training batch of 16384.
+#define MLP_COLS_PER_WAVE (MLP_OUT / 4)
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
-    __shared__ __align__(16) uint16_t sX[4][16 * LDX];
-    __shared__ __align__(16) uint16_t sH[4][3][16 * LDH];
-    __shared__ __align__(16) uint16_t sD[4][2][16 * LDH];
+    __shared__ __align__(16) uint16_t sX[16 * LDX];
+    __shared__ __align__(16) uint16_t sH[3][16 * LDH];
+    __shared__ __align__(16) uint16_t sD[2][16 * LDH];
     __shared__ __align__(16) uint16_t sG[4][16 * LDG];
-    __shared__ float sW[4][16];
-    __shared__ int32_t sT[4][16];
+    __shared__ float sW[16];
+    __shared__ int32_t sT[16];
+    __shared__ float sM[4][16], sS[4][16];
+    __shared__ float sDH[4][16][MLP_HID];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
-    const int64_t s0 = ((int64_t)blockIdx.x * 4 + w) * 16;
+    const int64_t s0 = (int64_t)blockIdx.x * 16;
     const int64_t n = a.n;
     const float* P = a.params;
     const uint16_t* WB = a.wb;
-    uint16_t* X = sX[w];
     uint16_t* act = TRAIN ? a.act : nullptr;
+    uint16_t* H1 = sH[0];
+    uint16_t* H2 = sH[1];
+    uint16_t* H3 = sH[2];
 
-    // input encoding: lane (r, g) -> sample s0 + r, features 8g .. 8g + 7
-    {
+    if (w == 0) {
+        // input encoding: lane (r, g) -> sample s0 + r, features 8g .. 8g + 7
         const int64_t s = s0 + r;
         float u = 0.0f, v = 0.0f;
         if (TRAIN) {
@@ -146,8 +167,8 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
             u = rc.x;
             v = rc.y;
             if (g == 0) {
-                sW[w][r] = rc.w * a.inv_wsum[0];
-                sT[w][r] = (int32_t)__float_as_uint(rc.z);
+                sW[r] = rc.w * a.inv_wsum[0];
+                sT[r] = (int32_t)__float_as_uint(rc.z);
             }
         } else if (s < n) {
             const float2 t = a.uv[s];
@@ -159,46 +180,45 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         for (int q = 0; q < 4; q++) {
             float sn, cs;
             fourier(x, (g & 1) * 4 + q, &sn, &cs);
-            X[r * LDX + 8 * g + 2 * q] = to_bf16(sn);
-            X[r * LDX + 8 * g + 2 * q + 1] = to_bf16(cs);
+            sX[r * LDX + 8 * g + 2 * q] = to_bf16(sn);
+            sX[r * LDX + 8 * g + 2 * q + 1] = to_bf16(cs);
         }
-    }
-    __syncthreads();
-    if (TRAIN) {  // X0^T: lane -> feature l / 2, samples 8 (l & 1) .. + 7
-        const int f = l >> 1, h = l & 1;
-        uint16_t xv[8];
+        wave_sync();
+        if (TRAIN) {  // X0^T: lane -> feature l / 2, samples 8 (l & 1) .. + 7
+            const int f = l >> 1, h = l & 1;
+            uint16_t xv[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) xv[t] = X[(8 * h + t) * LDX + f];
-        uint4 pk;
-        pk.x = (uint32_t)xv[0] | ((uint32_t)xv[1] << 16);
-        pk.y = (uint32_t)xv[2] | ((uint32_t)xv[3] << 16);
-        pk.z = (uint32_t)xv[4] | ((uint32_t)xv[5] << 16);
-        pk.w = (uint32_t)xv[6] | ((uint32_t)xv[7] << 16);
-        *(uint4*)(act + (int64_t)(ACT_X0 + f) * n + s0 + 8 * h) = pk;
+            for (int t = 0; t < 8; t++) xv[t] = sX[(8 * h + t) * LDX + f];
+            uint4 pk;
+            pk.x = (uint32_t)xv[0] | ((uint32_t)xv[1] << 16);
+            pk.y = (uint32_t)xv[2] | ((uint32_t)xv[3] << 16);
+            pk.z = (uint32_t)xv[4] | ((uint32_t)xv[5] << 16);
+            pk.w = (uint32_t)xv[6] | ((uint32_t)xv[7] << 16);
+            *(uint4*)(act + (int64_t)(ACT_X0 + f) * n + s0 + 8 * h) = pk;
+        }
+        dense_relu<MLP_IN>(sX, LDX, WB + MW1, P + OB1, H1, act, n, ACT_H1, s0);
+        wave_sync();
+        dense_relu<MLP_HID>(H1, LDH, WB + MW2, P + OB2, H2, act, n, ACT_H2, s0);
+        wave_sync();
+        dense_relu<MLP_HID>(H2, LDH, WB + MW3, P + OB3, H3, act, n, ACT_H3, s0);
     }
-    uint16_t* H1 = sH[w][0];
-    uint16_t* H2 = sH[w][1];
-    uint16_t* H3 = sH[w][2];
-    dense_relu<MLP_IN>(X, LDX, WB + MW1, P + OB1, H1, act, n, ACT_H1, s0);
-    __syncthreads();
-    dense_relu<MLP_HID>(H1, LDH, WB + MW2, P + OB2, H2, act, n, ACT_H2, s0);
-    __syncthreads();
-    dense_relu<MLP_HID>(H2, LDH, WB + MW3, P + OB3, H3, act, n, ACT_H3, s0);
     __syncthreads();
 
-    // output layer, pass 1: per-row running max and sum of exponentials
+    // output layer over this wave's columns, pass 1: per-row running max and
+    // sum of exponentials
     bf16x8 h3[2];
 #pragma unroll
     for (int kt = 0; kt < 2; kt++) h3[kt] = ld8(H3 + r * LDH + kt * 32 + 8 * g);
     const uint16_t* W4 = WB + MW4;
     const float* b4 = P + OB4;
+    const int nt0 = w * (MLP_COLS_PER_WAVE / 16), nt1 = nt0 + MLP_COLS_PER_WAVE / 16;
     float M[4], S[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         M[i] = -INFINITY;
         S[i] = 0.0f;
     }
-    for (int nt = 0; nt < MLP_OUT / 16; nt++) {
+    for (int nt = nt0; nt < nt1; nt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
@@ -224,12 +244,33 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
             M[i] = mm;
         }
     }
+    if (r == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            sM[w][4 * g + i] = M[i];
+            sS[w][4 * g + i] = S[i];
+        }
+    __syncthreads();
+    // the four waves' statistics of each row, combined in wave order
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float mm = sM[0][4 * g + i], ss = sS[0][4 * g + i];
+#pragma unroll
+        for (int v = 1; v < 4; v++) {
+            const float m2 = sM[v][4 * g + i], s2 = sS[v][4 * g + i];
+            const float mx = fmaxf(mm, m2);
+            ss = ss * __expf(mm - mx) + s2 * __expf(m2 - mx);
+            mm = mx;
+        }
+        M[i] = mm;
+        S[i] = ss;
+    }
     float invS[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) invS[i] = 1.0f / S[i];
 
     if (!TRAIN) {
-        for (int nt = 0; nt < MLP_OUT / 16; nt++) {
+        for (int nt = nt0; nt < nt1; nt++) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
@@ -243,7 +284,8 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         return;
     }
 
-    // pass 2: dZ = w (softmax - onehot), dH3 = dZ W4
+    // pass 2: dZ = w (softmax - onehot) over this wave's columns, and its
+    // share of dH3 = dZ W4
     const uint16_t* W4T = WB + MW4T;
     uint16_t* G = sG[w];
     f32x4 dh[4];
@@ -253,11 +295,11 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     int32_t tg[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        wt[i] = sW[w][4 * g + i];
-        tg[i] = sT[w][4 * g + i];
+        wt[i] = sW[4 * g + i];
+        tg[i] = sT[4 * g + i];
     }
     float lpart = 0.0f;
-    for (int nt2 = 0; nt2 < MLP_OUT / 32; nt2++) {  // 32 output columns: one K step of dH3
+    for (int nt2 = nt0 / 2; nt2 < nt1 / 2; nt2++) {  // 32 output columns: one K step of dH3
 #pragma unroll
         for (int half = 0; half < 2; half++) {
             const int nt = 2 * nt2 + half;
@@ -278,34 +320,42 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
             }
             st4(act, n, ACT_DZ + col, s0 + 4 * g, gv);
         }
-        __syncthreads();
+        wave_sync();
         const bf16x8 ga = ld8(G + r * LDG + 8 * g);
 #pragma unroll
         for (int ht = 0; ht < 4; ht++) dh[ht] = mfma32(ga, ld8(W4T + (ht * 16 + r) * MLP_OUT + nt2 * 32 + 8 * g), dh[ht]);
-        __syncthreads();
+        wave_sync();
     }
     if (a.loss) {
         for (int off = 32; off > 0; off >>= 1) lpart += __shfl_xor(lpart, off);
         if (l == 0) atomicAdd(a.loss, lpart);
     }
-    // dA3 = dH3 * [H3 > 0]
-    uint16_t* D0 = sD[w][0];
-    uint16_t* D1 = sD[w][1];
+#pragma unroll
+    for (int ht = 0; ht < 4; ht++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sDH[w][4 * g + i][ht * 16 + r] = dh[ht][i];
+    __syncthreads();
+    if (w != 0) return;
+    // dA3 = (sum of the four waves' dH3, in wave order) * [H3 > 0]
+    uint16_t* D0 = sD[0];
+    uint16_t* D1 = sD[1];
 #pragma unroll
     for (int ht = 0; ht < 4; ht++) {
         const int col = ht * 16 + r;
         uint16_t dv[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float hv = from_bf16(H3[(4 * g + i) * LDH + col]);
-            dv[i] = to_bf16(hv > 0.0f ? dh[ht][i] : 0.0f);
-            D0[(4 * g + i) * LDH + col] = dv[i];
+            const int row = 4 * g + i;
+            const float d = ((sDH[0][row][col] + sDH[1][row][col]) + sDH[2][row][col]) + sDH[3][row][col];
+            const float hv = from_bf16(H3[row * LDH + col]);
+            dv[i] = to_bf16(hv > 0.0f ? d : 0.0f);
+            D0[row * LDH + col] = dv[i];
         }
         st4(act, n, ACT_D3 + col, s0 + 4 * g, dv);
     }
-    __syncthreads();
+    wave_sync();
     dense_back(D0, WB + MW3T, H2, D1, act, n, ACT_D2, s0);
-    __syncthreads();
+    wave_sync();
     dense_back(D1, WB + MW2T, H1, nullptr, act, n, ACT_D1, s0);
 }
 
@@ -488,7 +538,7 @@ static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) /
 
 void launch_mlp(hipStream_t s, const MlpArgs& a, bool train) {
     if (a.n <= 0) return;
-    const unsigned blocks = grid1(a.n, 64);
+    const unsigned blocks = grid1(a.n, 16);
     if (train) hipLaunchKernelGGL(k_mlp<true>, dim3(blocks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_mlp<false>, dim3(blocks), dim3(256), 0, s, a);
 }
