@@ -478,7 +478,7 @@ def main():
     ap.add_argument("--sss-frequency", type=int, default=316)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
-    ap.add_argument("--batch-passes", type=int, default=16)
+    ap.add_argument("--batch-passes", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")

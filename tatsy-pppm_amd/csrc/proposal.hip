@@ -218,19 +218,30 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         M[i] = -INFINITY;
         S[i] = 0.0f;
     }
-    for (int nt = nt0; nt < nt1; nt++) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    // four tiles per step, their weights and biases loaded before any is used
+    for (int nb = nt0; nb < nt1; nb += 4) {
+        bf16x8 wv[4][2];
+        float bcv[4];
 #pragma unroll
-        for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
-        const float bc = b4[nt * 16 + r];
+        for (int q = 0; q < 4; q++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float z = acc[i] + bc;
-            if (z > M[i]) {
-                S[i] = S[i] * __expf(M[i] - z) + 1.0f;
-                M[i] = z;
-            } else {
-                S[i] += __expf(z - M[i]);
+            for (int kt = 0; kt < 2; kt++) wv[q][kt] = ld8(W4 + ((nb + q) * 16 + r) * 64 + kt * 32 + 8 * g);
+            bcv[q] = b4[(nb + q) * 16 + r];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], wv[q][kt], acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = acc[i] + bcv[q];
+                if (z > M[i]) {
+                    S[i] = S[i] * __expf(M[i] - z) + 1.0f;
+                    M[i] = z;
+                } else {
+                    S[i] += __expf(z - M[i]);
+                }
             }
         }
     }
@@ -270,15 +281,25 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     for (int i = 0; i < 4; i++) invS[i] = 1.0f / S[i];
 
     if (!TRAIN) {
-        for (int nt = nt0; nt < nt1; nt++) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int nb = nt0; nb < nt1; nb += 4) {
+            bf16x8 wv[4][2];
+            float bcv[4];
 #pragma unroll
-            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
-            const float bc = b4[nt * 16 + r];
+            for (int q = 0; q < 4; q++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int64_t s = s0 + 4 * g + i;
-                if (s < n) a.prob[s * MLP_OUT + nt * 16 + r] = __expf(acc[i] + bc - M[i]) * invS[i];
+                for (int kt = 0; kt < 2; kt++) wv[q][kt] = ld8(W4 + ((nb + q) * 16 + r) * 64 + kt * 32 + 8 * g);
+                bcv[q] = b4[(nb + q) * 16 + r];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], wv[q][kt], acc);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int64_t s = s0 + 4 * g + i;
+                    if (s < n) a.prob[s * MLP_OUT + (nb + q) * 16 + r] = __expf(acc[i] + bcv[q] - M[i]) * invS[i];
+                }
             }
         }
         return;
@@ -300,14 +321,27 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     }
     float lpart = 0.0f;
     for (int nt2 = nt0 / 2; nt2 < nt1 / 2; nt2++) {  // 32 output columns: one K step of dH3
+        // this step's weights (both halves), biases and transposed weights,
+        // loaded before any is used
+        bf16x8 wv[2][2], wt4[4];
+        float bcv[2];
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++)
+                wv[half][kt] = ld8(W4 + ((2 * nt2 + half) * 16 + r) * 64 + kt * 32 + 8 * g);
+            bcv[half] = b4[(2 * nt2 + half) * 16 + r];
+        }
+#pragma unroll
+        for (int ht = 0; ht < 4; ht++) wt4[ht] = ld8(W4T + (ht * 16 + r) * MLP_OUT + nt2 * 32 + 8 * g);
 #pragma unroll
         for (int half = 0; half < 2; half++) {
             const int nt = 2 * nt2 + half;
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], ld8(W4 + (nt * 16 + r) * 64 + kt * 32 + 8 * g), acc);
+            for (int kt = 0; kt < 2; kt++) acc = mfma32(h3[kt], wv[half][kt], acc);
             const int col = nt * 16 + r;
-            const float bc = b4[col];
+            const float bc = bcv[half];
             uint16_t gv[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -323,7 +357,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         wave_sync();
         const bf16x8 ga = ld8(G + r * LDG + 8 * g);
 #pragma unroll
-        for (int ht = 0; ht < 4; ht++) dh[ht] = mfma32(ga, ld8(W4T + (ht * 16 + r) * MLP_OUT + nt2 * 32 + 8 * g), dh[ht]);
+        for (int ht = 0; ht < 4; ht++) dh[ht] = mfma32(ga, wt4[ht], dh[ht]);
         wave_sync();
     }
     if (a.loss) {
@@ -374,7 +408,26 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t* GT, const uint
     for (int t = 0; t < N / 16; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bs = 0.0f;
     const uint16_t* arow = GT + (int64_t)(mt * 16 + r) * B;
-    for (int64_t k = k0; k < k0 + chunk; k += 32) {
+    // two K steps per iteration, every load issued before the products; the
+    // chunk (a multiple of 32) may end with one single step
+    const int64_t k1 = k0 + chunk;
+    for (int64_t k = k0; k + 64 <= k1; k += 64) {
+        bf16x8 av[2], hv[2][N / 16];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            av[u] = ld8(arow + k + 32 * u + 8 * g);
+#pragma unroll
+            for (int t = 0; t < N / 16; t++) hv[u][t] = ld8(HT + (int64_t)(t * 16 + r) * B + k + 32 * u + 8 * g);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            bs += bf16_sum8(av[u]);
+#pragma unroll
+            for (int t = 0; t < N / 16; t++) acc[t] = mfma32(av[u], hv[u][t], acc[t]);
+        }
+    }
+    if ((chunk & 63) != 0) {
+        const int64_t k = k1 - 32;
         const bf16x8 av = ld8(arow + k + 8 * g);
         bs += bf16_sum8(av);
 #pragma unroll
