@@ -452,7 +452,10 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
         "tune_s": tune_s,
         "capture_and_first_pass_s": capture_s,
         "tuned_gather": {v: sum(1 for i in infos if i["gather_mode"] == k) for k, v in modes.items()},
-        "tuned_photon_blocks_per_cu": sorted({i["photon_grid"] // cu for i in infos}),
+        "tuned_photon_blocks_per_cu": {str(g): sum(1 for i in infos if i["photon_grid"] // cu == g)
+                                       for g in sorted({i["photon_grid"] // cu for i in infos})},
+        "tuned_trace_keep": {str(k): sum(1 for i in infos if i["trace_keep"] == k)
+                             for k in sorted({i["trace_keep"] for i in infos})},
     }
 
 

@@ -425,14 +425,19 @@ int pppm_batch_add(void* batch, const float* tri_xyz, const int32_t* bsdf_id, in
                    int32_t nb, const pppm_light* lights, int32_t nl, const pppm_camera* camera,
                    const pppm_render_params* params, int32_t* index);
 /* Autotuner: per scene, times one pass of each gather kernel (tune_gather)
- * and each photon-kernel grid of 16, 32, 64 and 128 waves per CU (tune_grid)
- * and keeps the fastest; the renders restart. */
+ * with each photon-kernel grid of 1, 2, 4 and 8 workgroups of 256 lanes per
+ * CU (tune_grid), then, at the fastest of those, the photon traversal's lanes
+ * kept in flight before a wave regenerates paths (8, 16, 24, 32, 48;
+ * tune_grid); keeps the fastest.  None changes the image.  The renders
+ * restart. */
 int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid);
 /* Runs `passes` more passes of every scene (graph replays); *ms = GPU time. */
 int pppm_batch_run(void* batch, int32_t passes, double* ms);
 int pppm_batch_image(void* batch, int32_t index, double* rgb, int64_t n_values);
 int pppm_batch_scene_info(void* batch, int32_t index, int32_t* gather_mode, int32_t* photon_grid, int32_t* branch,
                           double* cost_ms);
+/* The photon traversal's lanes in flight the tuner chose for a scene. */
+int pppm_batch_scene_trace_keep(void* batch, int32_t index, int32_t* trace_keep);
 
 /* ---- mesh files: replace Trimesh::load (sources/trimesh.cc:125-259) ---- */
 typedef struct {

@@ -34,7 +34,7 @@ hipError_t prepare_enqueued(Ctx* ctx) {
 // sentinel keys behind the records, and the pass index and radius come from
 // the device pass state (advanced first).  The image equals pppm_render_pass's
 // bit for bit for the same hash size.
-void enqueue_pass(Ctx* ctx, int photon_grid) {
+void enqueue_pass(Ctx* ctx, int photon_grid, int trace_keep) {
     hipStream_t s = ctx->stream;
     PassDev* pd = ctx->pdev.as<PassDev>();
     Counters* dc = ctx->counters.as<Counters>();
@@ -64,6 +64,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
     pa.irr_cap = 0;
     pa.rec_rank = ctx->rec_rank.as<uint8_t>();
     pa.rec_count = ctx->rec_count.as<uint32_t>();
+    pa.trace_keep = trace_keep;
     launch_photon_pass(s, pa, photon_grid);
     launch_capacity_check(s, pd, dc, cap);
     GridParams g;
@@ -107,6 +108,7 @@ void enqueue_pass(Ctx* ctx, int photon_grid) {
 struct BatchScene {
     Ctx* ctx = nullptr;
     int photon_grid = 0;     // photon kernel grid (autotuned)
+    int trace_keep = PHOTON_TRACE_KEEP;  // photon traversal's lanes in flight before regeneration (autotuned)
     double cost_ms = 0.0;    // measured pass time of the chosen configuration
     int branch = 0;
 };
@@ -248,29 +250,47 @@ int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid) {
         if (tune_grid) grids = {cu, cu * 2, cu * 4, cu * 8};
         else grids = {sc.photon_grid};
         double best = 1e30;
-        int best_mode = base.gather_mode, best_grid = sc.photon_grid;
-        for (int m : modes) {
+        int best_mode = base.gather_mode, best_grid = sc.photon_grid, best_keep = sc.trace_keep;
+        // one timed pass per candidate after a warm-up pass; returns false on error
+        auto time_passes = [&](int mode, const std::vector<std::pair<int, int>>& shapes, bool& ok) {
+            ok = false;
             pppm_render_params p = base;
-            p.gather_mode = m;
-            p.passes = (int32_t)grids.size() + 1;
-            int rc = pppm_render_begin(c, &p);
-            if (rc != PPPM_OK) return b->fail(rc, c->err);
-            if (prepare_enqueued(c) != hipSuccess) return b->fail(PPPM_ERR_HIP, "tuning set-up");
-            // first pass warms the caches; then one pass per grid
-            enqueue_pass(c, grids[0]);
-            for (int gsz : grids) {
+            p.gather_mode = mode;
+            p.passes = (int32_t)shapes.size() + 1;
+            if (pppm_render_begin(c, &p) != PPPM_OK) return;
+            if (prepare_enqueued(c) != hipSuccess) return;
+            enqueue_pass(c, shapes[0].first, shapes[0].second);
+            for (const auto& sh : shapes) {
                 (void)hipEventRecord(c->ev[0], c->stream);
-                enqueue_pass(c, gsz);
+                enqueue_pass(c, sh.first, sh.second);
                 (void)hipEventRecord(c->ev[1], c->stream);
-                if (hipEventSynchronize(c->ev[1]) != hipSuccess) return b->fail(PPPM_ERR_HIP, "tuning pass");
+                if (hipEventSynchronize(c->ev[1]) != hipSuccess) return;
                 float ms = 0.0f;
                 (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
                 if (ms < best) {
                     best = ms;
-                    best_mode = m;
-                    best_grid = gsz;
+                    best_mode = mode;
+                    best_grid = sh.first;
+                    best_keep = sh.second;
                 }
             }
+            ok = true;
+        };
+        // gather kernel x photon grid, then the traversal's lanes in flight
+        // at the best of those (coordinate descent)
+        for (int m : modes) {
+            std::vector<std::pair<int, int>> shapes;
+            for (int gsz : grids) shapes.push_back({gsz, sc.trace_keep});
+            bool ok;
+            time_passes(m, shapes, ok);
+            if (!ok) return b->fail(c->err.empty() ? PPPM_ERR_HIP : PPPM_ERR_STATE, "tuning pass: " + c->err);
+        }
+        if (tune_grid) {
+            std::vector<std::pair<int, int>> shapes;
+            for (int k : {8, 24, 32, 48}) shapes.push_back({best_grid, k});
+            bool ok;
+            time_passes(best_mode, shapes, ok);
+            if (!ok) return b->fail(c->err.empty() ? PPPM_ERR_HIP : PPPM_ERR_STATE, "tuning pass: " + c->err);
         }
         pppm_render_params p = base;
         p.gather_mode = best_mode;
@@ -278,6 +298,7 @@ int pppm_batch_tune(void* batch, int32_t tune_gather, int32_t tune_grid) {
         if (rc != PPPM_OK) return b->fail(rc, c->err);
         if (prepare_enqueued(c) != hipSuccess) return b->fail(PPPM_ERR_HIP, "tuning reset");
         sc.photon_grid = best_grid;
+        sc.trace_keep = best_keep;
         sc.cost_ms = best;
     }
     b->drop_graph();
@@ -316,7 +337,7 @@ int pppm_batch_run(void* batch, int32_t passes, double* ms) {
             BatchScene& sc = b->scenes[i];
             hipStream_t own = sc.ctx->stream;
             sc.ctx->stream = b->br[(size_t)sc.branch];
-            enqueue_pass(sc.ctx, sc.photon_grid);
+            enqueue_pass(sc.ctx, sc.photon_grid, sc.trace_keep);
             sc.ctx->stream = own;
         }
         for (int k = 0; k < b->branches; k++) {
@@ -362,6 +383,14 @@ int pppm_batch_image(void* batch, int32_t index, double* rgb, int64_t n_values) 
     int rc = pppm_image_read(b->scenes[(size_t)index].ctx, rgb, n_values);
     if (rc != PPPM_OK) b->fail(rc, b->scenes[(size_t)index].ctx->err);
     return rc;
+}
+
+int pppm_batch_scene_trace_keep(void* batch, int32_t index, int32_t* trace_keep) {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return PPPM_ERR_INVALID;
+    if (index < 0 || index >= (int32_t)b->scenes.size() || !trace_keep) return b->fail(PPPM_ERR_INVALID, "scene index");
+    *trace_keep = b->scenes[(size_t)index].trace_keep;
+    return PPPM_OK;
 }
 
 int pppm_batch_scene_info(void* batch, int32_t index, int32_t* gather_mode, int32_t* photon_grid, int32_t* branch,

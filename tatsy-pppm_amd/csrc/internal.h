@@ -125,6 +125,9 @@ struct Counters {
 // 4096 blocks, which the hardware hands out as earlier blocks finish.
 #define EYE_BLOCKS_PER_CU 64
 
+#ifndef PHOTON_TRACE_KEEP
+#define PHOTON_TRACE_KEEP 16
+#endif
 struct PhotonPassArgs {
     SceneDev sc;
     int64_t n_photons;
@@ -150,6 +153,10 @@ struct PhotonPassArgs {
     // records, rec_count[photon index] = that photon's stored records
     uint8_t* rec_rank = nullptr;
     uint32_t* rec_count = nullptr;
+    // the wave traces until at most this many of its lanes still have a ray
+    // in flight, then regenerates paths (a launch-shape knob: the records do
+    // not depend on it)
+    int32_t trace_keep = PHOTON_TRACE_KEEP;
 };
 void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit);
 

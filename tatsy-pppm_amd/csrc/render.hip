@@ -87,9 +87,6 @@ __device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
 #define TRACE_TMAX 1.0e30f
 
 // Unfinished rays a photon wave leaves in flight when it stops to shade and refill.
-#ifndef PHOTON_TRACE_KEEP
-#define PHOTON_TRACE_KEEP 16
-#endif
 
 // Emission of photon i: light by power, uniform point, cosine direction (or
 // the learned proposal's mixture); returns the ray origin, direction, flux.
@@ -150,8 +147,8 @@ __device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, 
 }
 
 // One lane per photon path, with path regeneration at the level of the
-// traversal: the wave traces until at most PHOTON_TRACE_KEEP of its lanes
-// still have a ray in flight, then the lanes whose ray is finished shade their
+// traversal: the wave traces until at most a.trace_keep (PHOTON_TRACE_KEEP by
+// default) of its lanes still have a ray in flight, then the lanes whose ray is finished shade their
 // hit and start their next segment, lanes whose path has ended take the next
 // photon index from a pass-wide counter (one atomic per wave), and the
 // traversal resumes with the unfinished rays where they stopped.  Lanes thus
@@ -268,7 +265,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             if (exhausted) break;
             continue;
         }
-        trace_resume<256, COUNT>(sc, ray, ts, stk, ovf_stack, &ovf, &tc, exhausted ? 0 : PHOTON_TRACE_KEEP);
+        trace_resume<256, COUNT>(sc, ray, ts, stk, ovf_stack, &ovf, &tc, exhausted ? 0 : a.trace_keep);
     }
     segs = wave_sum(segs);
     uint64_t any_ovf = __ballot(ovf);

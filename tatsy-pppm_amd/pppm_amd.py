@@ -159,6 +159,7 @@ _DECLS = {
     "pppm_batch_image": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "pppm_batch_scene_info": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "pppm_batch_scene_trace_keep": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "pppm_scenegen": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SceneDesc)]),
     "pppm_scenegen_free": (None, [C.POINTER(SceneDesc)]),
 }
@@ -667,4 +668,7 @@ class Batch:
         m, g, b = C.c_int32(), C.c_int32(), C.c_int32()
         c = C.c_double()
         self._chk(self.L.pppm_batch_scene_info(self.h, index, C.byref(m), C.byref(g), C.byref(b), C.byref(c)))
-        return {"gather_mode": m.value, "photon_grid": g.value, "branch": b.value, "cost_ms": c.value}
+        k = C.c_int32()
+        self._chk(self.L.pppm_batch_scene_trace_keep(self.h, index, C.byref(k)))
+        return {"gather_mode": m.value, "photon_grid": g.value, "branch": b.value, "cost_ms": c.value,
+                "trace_keep": k.value}
