@@ -31,7 +31,7 @@ HIP_SOURCES = ["radix.hip", "hash_grid.hip", "bvh.hip", "render.hip", "gather.hi
 CXX_SOURCES = ["scenegen.cc", "pppm_amd.cc", "mesh_io.cc", "image_io.cc"]
 FACADE_TEST = os.path.join(ROOT, "tests", "cpp", "facade_main")
 # C++ programs over the reference-shaped interface, built beside their sources
-CPP_TESTS = ["facade_main", "appmain_shape", "mesh_main"]
+CPP_TESTS = ["facade_main", "appmain_shape", "mesh_main", "image_main"]
 
 
 def _run(cmd):

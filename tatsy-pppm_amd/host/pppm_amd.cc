@@ -2,6 +2,7 @@
 #include "pppm_amd.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -110,6 +111,96 @@ PerspectiveCamera::PerspectiveCamera(const Vector3D& center, const Vector3D& dir
 Vector3D Image::operator()(int x, int y) const {
     size_t k = 3 * ((size_t)y * _w + x);
     return Vector3D(_rgb[k], _rgb[k + 1], _rgb[k + 2]);
+}
+
+void Image::setPixel(int x, int y, const Vector3D& c) {
+    const size_t k = 3 * ((size_t)y * _w + x);
+    _rgb[k] = c.x();
+    _rgb[k + 1] = c.y();
+    _rgb[k + 2] = c.z();
+}
+
+void Image::resize(int w, int h) {
+    _w = w;
+    _h = h;
+    _rgb.assign(3 * (size_t)w * h, 0.0);
+}
+
+void Image::fill(const Vector3D& c) {
+    for (size_t i = 0; i < _rgb.size(); i += 3) {
+        _rgb[i] = c.x();
+        _rgb[i + 1] = c.y();
+        _rgb[i + 2] = c.z();
+    }
+}
+
+void Image::gamma(double gam, bool inv) {
+    const double e = inv ? 1.0 / gam : gam;
+    for (double& v : _rgb) v = std::pow(v, e);
+}
+
+void Image::tonemap() {
+    const size_t n = (size_t)_w * _h;
+    if (n == 0) return;
+    const double key = 0.18, tiny = 1.0e-8;
+    double log_sum = 0.0, white = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const double lum = 0.2126 * _rgb[3 * i] + 0.7152 * _rgb[3 * i + 1] + 0.0722 * _rgb[3 * i + 2];
+        log_sum += std::log(lum + tiny);
+        white = std::max(white, lum);
+    }
+    const double mean_log = std::exp(log_sum / (double)n);
+    const double white2 = white * white;
+    for (double& v : _rgb) {
+        const double s = v * key / mean_log;
+        v = s * (1.0 + s / white2) / (1.0 + s);
+    }
+}
+
+namespace {
+std::string lower_ext(const std::string& f) {
+    const size_t dot = f.find_last_of('.');
+    std::string e = dot == std::string::npos ? "" : f.substr(dot);
+    for (char& c : e) c = (char)std::tolower((unsigned char)c);
+    return e;
+}
+}  // namespace
+
+void Image::load(const std::string& filename) {
+    double* rgb = nullptr;
+    int32_t w = 0, h = 0;
+    char err[512] = {0};
+    const int rc = pppm_image_load(filename.c_str(), &rgb, &w, &h, err, (int32_t)sizeof(err));
+    if (rc != PPPM_OK) throw Error(rc, std::string("Image::load: ") + err);
+    _w = w;
+    _h = h;
+    _rgb.assign(rgb, rgb + 3 * (size_t)w * h);
+    pppm_image_free(rgb);
+}
+
+void Image::save(const std::string& filename) const {
+    const int rc = pppm_image_write(filename.c_str(), _rgb.data(), _w, _h, 0.0);
+    if (rc != PPPM_OK) throw Error(rc, "Image::save: cannot write " + filename);
+}
+
+void Image::loadBmp(const std::string& filename) {
+    if (lower_ext(filename) != ".bmp") throw Error(PPPM_ERR_INVALID, "loadBmp: not a .bmp file");
+    load(filename);
+}
+
+void Image::saveBmp(const std::string& filename) const {
+    if (lower_ext(filename) != ".bmp") throw Error(PPPM_ERR_INVALID, "saveBmp: not a .bmp file");
+    save(filename);
+}
+
+void Image::loadHdr(const std::string& filename) {
+    if (lower_ext(filename) != ".hdr") throw Error(PPPM_ERR_INVALID, "loadHdr: not a .hdr file");
+    load(filename);
+}
+
+void Image::saveHdr(const std::string& filename) const {
+    if (lower_ext(filename) != ".hdr") throw Error(PPPM_ERR_INVALID, "saveHdr: not a .hdr file");
+    save(filename);
 }
 
 // ---- Envmap ----------------------------------------------------------------

@@ -345,8 +345,24 @@ public:
     int width() const { return _w; }
     int height() const { return _h; }
     Vector3D operator()(int x, int y) const;
+    void setPixel(int x, int y, const Vector3D& c);
     std::vector<double>& data() { return _rgb; }
     const std::vector<double>& data() const { return _rgb; }
+    void resize(int w, int h);
+    void fill(const Vector3D& color);
+    // every channel raised to gam (inv: to 1 / gam), image.cc:202-213
+    void gamma(double gam, bool inv = false);
+    // Reinhard's global operator with key 0.18 and the brightest luminance
+    // as white, image.cc:498-524
+    void tonemap();
+    // by extension: .bmp, .png, .hdr (RGBE) and .pfm; the values as they are
+    // (apply gamma() first), image.cc:215-240; errors throw
+    void load(const std::string& filename);
+    void save(const std::string& filename) const;
+    void loadBmp(const std::string& filename);
+    void saveBmp(const std::string& filename) const;
+    void loadHdr(const std::string& filename);
+    void saveHdr(const std::string& filename) const;
 };
 
 // ---- sources/envmap.h:14-60 ------------------------------------------------

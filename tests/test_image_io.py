@@ -10,6 +10,7 @@
   runs, flat scanlines, and bad files rejected.
 """
 import math
+import os
 import struct
 import zlib
 
@@ -18,6 +19,8 @@ import pytest
 
 import oracle as O
 import pppm_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def decode_png(path):
@@ -165,3 +168,12 @@ def test_hdr_files_decode_alike_in_both_loaders(tmp_path):
     np.testing.assert_array_equal(ref_by_ref, ref_by_us)
     # the same RGBE bytes: HDRPixel's encoding, so the decoded images agree too
     np.testing.assert_array_equal(ours_by_us, ref_by_us)
+
+
+def test_cpp_image_class(tmp_path):
+    """Image::gamma / tonemap / save / load of the C++ interface (tests/cpp/image_main.cc)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "image_main")
+    r = subprocess.run([exe, str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0 and "OK" in r.stdout
