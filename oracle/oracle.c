@@ -37,6 +37,13 @@
  * emit_direction /              no reference counterpart (the learned      uniform-table and one-hot-table closed
  *   orc_trace_photons_prop        mixture replacing sampler.cc:17-32)        forms; lambda = 1 equals on_hemisphere;
  *                                                                          E[factor] = 1 (unbiasedness)
+ * orc_dipole_octree             sources/subsurface_integrator.cc:69-167    compiled reference Octree (_ref golden, three
+ *                                                                          point sets); max_error 0 = the exact sum
+ * orc_scene_set_envmap /        sources/envmap.cc:47-163                   compiled reference Envmap (_ref): importance
+ *   orc_envmap_lookup /                                                    cells and running sums bit for bit, lookups,
+ *   orc_envmap_sample_dir                                                  photon directions; env-lit floor closed form
+ * orc_render_hachisuka          sources/progressive_photon_mapping.cc:     sphere furnace closed form; agreement with
+ *                               22-287                                     orc_render and orc_pathtrace on the box
  *
  * Deliberate differences from the literal reference (each is a quirk the
  * reference cannot run with): fixed-radius gather instead of the swapped kNN,
