@@ -30,6 +30,11 @@ BUCKET_RANGE_BYTES = 8      # cell start + end
 PHOTON_BYTES = 24           # position 12 + flux 12
 
 
+
+# Proposal retraining: Adam steps per retrain and samples per step (set from
+# the command line)
+TRAIN = [8, 65536]
+
 def gather_bytes(st):
     """Algorithmic bytes of one gather launch (reference cell convention)."""
     return (st["hitpoints"] * (HIT_RECORD_BYTES + ACCUM_BYTES)
@@ -139,7 +144,7 @@ def error_study(P, np, photons, width, passes, ref_passes, lam, probe_every=8):
     def run(seed, n, learned, ref=None):
         ctx.begin(photons, n, r1, seed=seed)
         if learned:
-            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=64, batch=16384,
+            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=TRAIN[0], batch=TRAIN[1],
                                 ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
         t, curve, extra = 0.0, [], 0.0
         for i in range(1, n + 1):
@@ -215,6 +220,47 @@ def error_study(P, np, photons, width, passes, ref_passes, lam, probe_every=8):
     }
 
 
+def c1_parity_run(P, np):
+    """The reference's CPU-runnable case (C1: closed box, two f = 8 spheres,
+    256^2, 200k photons per pass, 32 passes, alpha 0.7, r1 = 1 % of the
+    diagonal) in full, on the GPU and with the FP64 oracle on the host cores,
+    same seed: the CPU path timed beside the GPU, and the relative RMSE between
+    the two images at an equal photon count (both use the same random numbers,
+    so it measures only FP32 against FP64 arithmetic)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    scene = P.scenegen(P.SCENE_BOX, 1, 8, 256, 256)
+    photons, passes, seed = 200000, 32, 0x7A75
+    r1 = 0.01 * scene.diagonal()
+    ctx = P.Context(0)
+    ctx.upload(scene)
+    ctx.build_bvh()
+    ctx.set_camera(scene.camera)
+    ctx.begin(photons, passes, r1, seed=seed)
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        ctx.render_pass()
+    gpu_s = time.perf_counter() - t0
+    g = ctx.image()
+    ctx.close()
+    os_ = O.OracleScene(scene)
+    t0 = time.perf_counter()
+    o, _ = os_.render(scene.camera, photons, passes, r1, ray_eps=1e-4, seed=seed)
+    cpu_s = time.perf_counter() - t0
+    return {
+        "workload": "closed box, two f=8 spheres, 256x256, 200k photons/pass, 32 passes",
+        "photons": photons * passes,
+        "gpu_s": gpu_s,
+        "gpu_photons_per_s": photons * passes / gpu_s,
+        "cpu_s": cpu_s,
+        "cpu_photons_per_s": photons * passes / cpu_s,
+        "cpu_threads": int(O.orc().orc_num_threads()),
+        "rel_rmse_gpu_vs_fp64_oracle": float(np.sqrt(np.mean((g - o) ** 2)) / o.mean()),
+        "note": "single passes (pppm_render_pass) on the GPU; the oracle is FP64 OpenMP",
+    }
+
+
 def oracle_error_study(P, np, lam, probe_every=8):
     """Time to error against the FP64 oracle image of the occluded-light scene
     at 64 x 64 (tests/golden/occluded_reference_64.npz: the oracle's P-KZ, four
@@ -248,7 +294,7 @@ def oracle_error_study(P, np, lam, probe_every=8):
     def run(learned):
         ctx.begin(photons, passes, r1, seed=0x7A75)
         if learned:
-            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=64, batch=16384,
+            ctx.proposal_enable(lam=lam, anchors=8, train_interval=8, train_steps=TRAIN[0], batch=TRAIN[1],
                                 ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
         t, curve = 0.0, []
         for i in range(1, passes + 1):
@@ -331,7 +377,7 @@ def subsurface_study(P, np, width, photons, passes, points, frequency):
         t0 = time.perf_counter()
         ctx.begin(photons, passes + 1, r1, seed=0x7A75)
         setup = time.perf_counter() - t0
-        ctx.proposal_enable(lam=0.5, anchors=8, train_interval=8, train_steps=64, batch=16384,
+        ctx.proposal_enable(lam=0.5, anchors=8, train_interval=8, train_steps=TRAIN[0], batch=TRAIN[1],
                             ring_capacity=16 << 20, learning_rate=2e-3, seed=0x5EED)
         ctx.render_pass()  # warm-up
         out = []
@@ -474,6 +520,8 @@ def main():
     ap.add_argument("--rmse-passes", type=int, default=256)
     ap.add_argument("--rmse-ref-passes", type=int, default=1024)
     ap.add_argument("--rmse-lambda", type=float, default=0.5)
+    ap.add_argument("--train-steps", type=int, default=8, help="Adam steps per retrain of the proposal MLP")
+    ap.add_argument("--train-batch", type=int, default=65536, help="samples per step (a multiple of 256)")
     ap.add_argument("--no-sss", action="store_true", help="skip the subsurface (dipole BSSRDF) leg")
     ap.add_argument("--sss-passes", type=int, default=2)
     ap.add_argument("--sss-photons", type=int, default=8_000_000)
@@ -486,6 +534,7 @@ def main():
                     help="after the passes, copy BYTES device-to-device once (a plain copy kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    TRAIN[0], TRAIN[1] = args.train_steps, args.train_batch
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
               "Passes shard across GPUs as independent photon streams; see the design notes.",
@@ -650,6 +699,7 @@ def main():
         out["batch"] = None
     if not args.no_cpu:
         cb, _ = cpu_baseline(scene, args.photons, r1, ray_eps, seed)
+        cb["c1_full_run"] = c1_parity_run(P, np)
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
