@@ -7,14 +7,18 @@
 // L = W * sum / (pi r^2 (1 - 2 / 3.3)).
 //
 // Photons are read in bucket (cell) order from a gather layout: for the lanes
-// kernel the records as stored, (x, y, z, octahedral normal) (Phi, key); for
-// the others three 16-byte words (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g)
-// (Phi_b, key, -, -) with the normal decoded once.  Hit points arrive sorted
-// along a Morton curve of half cells (edge r), so neighbouring lanes share
-// their eight buckets.  Three kernels compute the same sum:
+// and blocks kernels the records as stored, (x, y, z, octahedral normal)
+// (Phi, key), with a box per 32-record block; for the others three 16-byte
+// words (x, y, z, n_x) (n_y, n_z, Phi_r, Phi_g) (Phi_b, key, -, -) with the
+// normal decoded once.  Hit points arrive sorted along a Morton curve of half
+// cells (edge r), so neighbouring lanes share their eight buckets.  Four
+// kernels compute the same sum:
 //   k_gather_lanes (default): one lane per hit point walking its own buckets;
 //     inside a bucket the records follow a cell-local Morton order and blocks
 //     of 32 records whose bounding box misses the ball are skipped.
+//   k_gather_blocks: a wave walks the union of its lanes' buckets block by
+//     block, skips the blocks no lane needs (the lanes kernel's decisions) and
+//     stages the others in LDS for all 64 lanes.
 //   k_gather_tiled: the union of a wave's buckets is staged through LDS in
 //     coalesced 64-photon chunks and tested by all 64 lanes.
 //   k_gather_wave: the union is walked with a wave-uniform photon index, so
