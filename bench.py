@@ -516,6 +516,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--gather-mode", choices=["auto", "lanes", "tiled", "wave", "blocks"], default="auto")
+    ap.add_argument("--bvh", choices=["default", "lbvh", "ploc"], default="default",
+                    help="binary tree the 4-wide BVH of the headline scene is collapsed from")
     ap.add_argument("--no-rmse", action="store_true", help="skip the plain-vs-learned time-to-error leg")
     ap.add_argument("--rmse-passes", type=int, default=256)
     ap.add_argument("--rmse-ref-passes", type=int, default=1024)
@@ -550,7 +552,7 @@ def main():
     seed = 0x7A75
     ctx = P.Context(0)
     ctx.upload(scene)
-    bvh = ctx.build_bvh()
+    bvh = ctx.build_bvh({"default": None, "lbvh": P.BVH_LBVH, "ploc": P.BVH_PLOC}[args.bvh])
     ctx.set_camera(scene.camera)
     total = args.warmup + args.steps
     mode = {"auto": P.GATHER_AUTO, "lanes": P.GATHER_LANES, "tiled": P.GATHER_TILED,
@@ -616,7 +618,8 @@ def main():
             "photons_per_pass": args.photons,
             "alpha": 0.7,
             "radius1": r1,
-            "bvh": {k: bvh[k] for k in ("nodes", "leaves", "depth", "build_ms")},
+            "bvh": {**{k: bvh[k] for k in ("nodes", "leaves", "depth", "build_ms")},
+                    "builder": {P.BVH_LBVH: "lbvh", P.BVH_PLOC: "ploc"}.get(bvh["builder"], "?")},
         },
         "roofline": {
             "bound": "hbm",

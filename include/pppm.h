@@ -90,6 +90,8 @@ typedef struct {
     int32_t depth;     /* root = 1 */
     int32_t max_leaf;  /* triangles per leaf limit */
     double build_ms;
+    int32_t builder;   /* PPPM_BVH_LBVH or PPPM_BVH_PLOC: the binary tree that was collapsed */
+    int32_t reserved;
 } pppm_bvh_info;
 
 /* Per-render settings (RenderParameters, sources/render_parameters.h:4-44,
@@ -178,8 +180,14 @@ int pppm_device_info(void* ctx, char* name, int32_t name_len, int32_t* cu_count,
 int pppm_scene_upload(void* ctx, const float* tri_xyz /* [n][9] */, const int32_t* bsdf_id,
                       int64_t n, const pppm_bsdf* table, int32_t nb, const pppm_light* lights,
                       int32_t nl);
-/* GPU BVH build: LBVH over Morton codes collapsed to 4-wide 128-byte nodes. */
+/* GPU BVH build: a binary tree over the Morton-sorted triangles collapsed to
+ * 4-wide 128-byte nodes; pppm_bvh_build uses PPPM_BVH_DEFAULT. */
+#define PPPM_BVH_DEFAULT 0 /* the builder whose trees traced fastest on gfx950 */
+#define PPPM_BVH_LBVH 1    /* radix tree over the Morton codes (Karras 2012) */
+#define PPPM_BVH_PLOC 2    /* locally-ordered clustering on surface area (Meister and Bittner 2018);
+                            * falls back to LBVH when its tree is deeper than the traversal stack */
 int pppm_bvh_build(void* ctx, pppm_bvh_info* info);
+int pppm_bvh_build_with(void* ctx, int32_t builder, pppm_bvh_info* info);
 /* Batch closest hit: replaces Scene::intersect (sources/scene.cc:40-47) and
  * QBVHAccel::intersect (sources/qbvh_accel.cc:205-285). */
 int pppm_intersect(void* ctx, const pppm_ray* rays, int64_t n, pppm_hit* hits);

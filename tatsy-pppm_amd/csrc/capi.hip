@@ -329,9 +329,14 @@ int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_i
     return PPPM_OK;
 }
 
-int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
+int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) { return pppm_bvh_build_with(ctx_raw, PPPM_BVH_DEFAULT, info); }
+
+int pppm_bvh_build_with(void* ctx_raw, int32_t builder, pppm_bvh_info* info) {
     CHECK_CTX(ctx);
     if (!ctx->has_scene) return ctx->fail(PPPM_ERR_STATE, "no scene uploaded");
+    if (builder != PPPM_BVH_DEFAULT && builder != PPPM_BVH_LBVH && builder != PPPM_BVH_PLOC)
+        return ctx->fail(PPPM_ERR_INVALID, "unknown BVH builder");
+    if (builder == PPPM_BVH_DEFAULT) builder = BVH_BUILDER_DEFAULT;
     (void)hipSetDevice(ctx->device);
     (void)hipFree(ctx->nodes);
     (void)hipFree(ctx->tris4);
@@ -340,8 +345,19 @@ int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
     ctx->has_bvh = false;
     TRY_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     BvhBuild b;
-    std::string e = bvh_build(ctx->stream, ctx->tri9.as<float>(), ctx->n_tris, &b);
+    std::string e = bvh_build(ctx->stream, ctx->tri9.as<float>(), ctx->n_tris,
+                              builder == PPPM_BVH_PLOC ? BVH_BUILD_PLOC : BVH_BUILD_LBVH, &b);
     if (!e.empty()) return ctx->fail(PPPM_ERR_HIP, "BVH build: " + e);
+    // the traversal stack holds at most 3 entries per level below the root:
+    // a clustered tree deeper than that is replaced by the radix tree, whose
+    // depth is bounded by the 30 Morton bits
+    if (builder == PPPM_BVH_PLOC && 3 * (int64_t)b.depth + 1 > STACK_MAX) {
+        (void)hipFree(b.nodes);
+        (void)hipFree(b.tris4);
+        builder = PPPM_BVH_LBVH;
+        e = bvh_build(ctx->stream, ctx->tri9.as<float>(), ctx->n_tris, BVH_BUILD_LBVH, &b);
+        if (!e.empty()) return ctx->fail(PPPM_ERR_HIP, "BVH build: " + e);
+    }
     TRY_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     TRY_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
     float ms = 0.0f;
@@ -349,7 +365,6 @@ int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
     ctx->nodes = b.nodes;
     ctx->tris4 = b.tris4;
     ctx->root = b.root;
-    // the traversal stack holds at most 3 entries per level below the root
     if (3 * (int64_t)b.depth + 1 > STACK_MAX)
         return ctx->fail(PPPM_ERR_INVALID, "BVH deeper than the traversal stack allows");
     // node fetches address the node array with 32-bit byte offsets
@@ -360,6 +375,8 @@ int pppm_bvh_build(void* ctx_raw, pppm_bvh_info* info) {
     ctx->binfo.depth = b.depth;
     ctx->binfo.max_leaf = LEAF_MAX;
     ctx->binfo.build_ms = ms;
+    ctx->binfo.builder = builder;
+    ctx->binfo.reserved = 0;
     ctx->has_bvh = true;
     if (info) *info = ctx->binfo;
     return PPPM_OK;

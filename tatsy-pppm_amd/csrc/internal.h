@@ -89,10 +89,17 @@ struct BvhBuild {
     int64_t n_nodes, n_leaves;
     int32_t depth;
 };
-// Builds the 4-wide BVH of n triangles (tri9: [n][9] floats on the device).
-// Allocates nodes / tris4 with hipMalloc (caller frees).  Returns an error
-// message, empty on success.
-std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, BvhBuild* out);
+// Binary trees the 4-wide BVH is collapsed from.
+#define BVH_BUILD_LBVH 1  // radix tree over the Morton codes (Karras 2012)
+#define BVH_BUILD_PLOC 2  // locally-ordered clustering on surface area (Meister and Bittner 2018)
+// What PPPM_BVH_DEFAULT resolves to.
+#ifndef BVH_BUILDER_DEFAULT
+#define BVH_BUILDER_DEFAULT PPPM_BVH_PLOC
+#endif
+// Builds the 4-wide BVH of n triangles (tri9: [n][9] floats on the device)
+// from the binary tree `builder` names.  Allocates nodes / tris4 with
+// hipMalloc (caller frees).  Returns an error message, empty on success.
+std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, int builder, BvhBuild* out);
 void launch_intersect(hipStream_t s, SceneDev sc, const pppm_ray* rays, int64_t n, pppm_hit* hits);
 
 // ---------------------------------------------------------------------------

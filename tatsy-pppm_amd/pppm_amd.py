@@ -16,6 +16,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_NOMEM = range(6)
 LAMBERTIAN, SPECULAR, PHONG, REFRACTION, BSSRDF, EMITTER = 0x01, 0x02, 0x04, 0x08, 0x10, 0x40
 SCENE_BOX, SCENE_BOX_CAUSTIC, SCENE_RANDOM, SCENE_TWO_ROOMS, SCENE_TWO_ROOMS_SSS = 1, 2, 3, 4, 5
 GATHER_AUTO, GATHER_LANES, GATHER_TILED, GATHER_WAVE, GATHER_BLOCKS = 0, 1, 2, 3, 4
+BVH_DEFAULT, BVH_LBVH, BVH_PLOC = 0, 1, 2
 METHOD_PROBABILISTIC, METHOD_HACHISUKA = 0, 1
 PROPOSAL_CELLS = 1024
 MLP_PARAMS = 76992
@@ -39,7 +40,8 @@ class Camera(C.Structure):
 
 class BvhInfo(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("leaves", C.c_int64), ("depth", C.c_int32),
-                ("max_leaf", C.c_int32), ("build_ms", C.c_double)]
+                ("max_leaf", C.c_int32), ("build_ms", C.c_double), ("builder", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class RenderParams(C.Structure):
@@ -104,6 +106,7 @@ _DECLS = {
     "pppm_device_info": (C.c_int, [_P, C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "pppm_scene_upload": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(Bsdf), C.c_int32, C.POINTER(Light), C.c_int32]),
     "pppm_bvh_build": (C.c_int, [_P, C.POINTER(BvhInfo)]),
+    "pppm_bvh_build_with": (C.c_int, [_P, C.c_int32, C.POINTER(BvhInfo)]),
     "pppm_intersect": (C.c_int, [_P, _P, C.c_int64, _P]),
     "pppm_bvh_download": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int32)]),
@@ -368,10 +371,14 @@ class Context:
         self._chk(self.L.pppm_scene_upload(self.h, _ptr(tri), _ptr(ids), len(ids), scene.bsdf_array(),
                                            len(scene.bsdfs), lights, len(scene.lights)))
 
-    def build_bvh(self):
+    def build_bvh(self, builder=None):
+        """4-wide BVH of the uploaded scene; builder: BVH_LBVH, BVH_PLOC or None (BVH_DEFAULT)."""
         info = BvhInfo()
-        self._chk(self.L.pppm_bvh_build(self.h, C.byref(info)))
-        return {f: getattr(info, f) for f, _ in info._fields_}
+        if builder is None:
+            self._chk(self.L.pppm_bvh_build(self.h, C.byref(info)))
+        else:
+            self._chk(self.L.pppm_bvh_build_with(self.h, int(builder), C.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_ if f != "reserved"}
 
     def intersect(self, rays):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

@@ -39,13 +39,18 @@ def random_rays(rng, n, lo, hi):
 # ---------------------------------------------------------------------------
 # BVH + traversal
 # ---------------------------------------------------------------------------
+BUILDERS = [P.BVH_LBVH, P.BVH_PLOC]
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("kind,freq", [(P.SCENE_BOX, 8), (P.SCENE_BOX_CAUSTIC, 24), (P.SCENE_TWO_ROOMS, 40)])
-def test_intersect_matches_brute_force(gpu_ctx, kind, freq):
-    say(f"intersect kind={kind} f={freq}")
+def test_intersect_matches_brute_force(gpu_ctx, kind, freq, builder):
+    say(f"intersect kind={kind} f={freq} builder={builder}")
     s = P.scenegen(kind, 1, freq, 32, 32)
     gpu_ctx.upload(s)
-    info = gpu_ctx.build_bvh()
+    info = gpu_ctx.build_bvh(builder)
     say(f"bvh {info} tris={s.n_tris}")
+    assert info["builder"] == builder
     assert info["nodes"] >= 1 and 3 * info["depth"] + 1 <= 128
     rng = np.random.default_rng(kind * 100 + freq)
     lo, hi = np.array(s.bbox_min) + 0.05, np.array(s.bbox_max) - 0.05
@@ -62,14 +67,15 @@ def test_intersect_matches_brute_force(gpu_ctx, kind, freq):
     np.testing.assert_allclose(hits["t"][both], t_ref[both], rtol=2e-5, atol=2e-6)
 
 
-def test_watertight_sphere_never_leaks(gpu_ctx):
-    say("watertight icosphere")
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_watertight_sphere_never_leaks(gpu_ctx, builder):
+    say(f"watertight icosphere builder={builder}")
     s = P.scenegen(P.SCENE_BOX, 3, 64, 8, 8)
     keep = s.bsdf_id == s.bsdf_id.max()  # the second sphere (centre 3.5, 1.0, 3.0)
     sub = P.Scene(s.tri[keep], np.zeros(keep.sum(), np.int32), [s.bsdfs[int(s.bsdf_id.max())]], s.lights,
                   s.camera, s.bbox_min, s.bbox_max)
     gpu_ctx.upload(sub)
-    gpu_ctx.build_bvh()
+    gpu_ctx.build_bvh(builder)
     rng = np.random.default_rng(9)
     n = 200000
     rays = np.zeros(n, P.RAY_DTYPE)
@@ -84,19 +90,75 @@ def test_watertight_sphere_never_leaks(gpu_ctx):
     assert np.all(hits["tri"] >= 0), int(np.sum(hits["tri"] < 0))
 
 
-def test_intersect_empty_and_tiny_scenes(gpu_ctx):
-    say("tiny scenes")
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_intersect_empty_and_tiny_scenes(gpu_ctx, builder):
+    say(f"tiny scenes builder={builder}")
     base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
     for n in (1, 2, 3, 4, 5, 9):
         sub = P.Scene(base.tri[:n], np.zeros(n, np.int32), [base.bsdfs[0]], base.lights, base.camera,
                       base.bbox_min, base.bbox_max)
         gpu_ctx.upload(sub)
-        gpu_ctx.build_bvh()
+        gpu_ctx.build_bvh(builder)
         rng = np.random.default_rng(n)
         rays = random_rays(rng, 4000, np.array([0.1] * 3), np.array([4.9] * 3))
         hits = gpu_ctx.intersect(rays)
         t_ref, tri_ref = O.OracleScene(sub).intersect(rays.view(np.float32).reshape(-1, 8), brute=True)
         assert np.array_equal(hits["tri"] >= 0, tri_ref >= 0)
+
+
+def _check_bvh(nodes, tris, root, tri_xyz, alone):
+    """Every triangle in exactly one leaf, every child box holding what lies
+    below it, and the triangles in `alone` (original ids) in leaves of their own."""
+    n = len(tri_xyz)
+    assert len(tris) == n
+    orig = tris[:, 3].copy().view(np.int32)
+    assert np.array_equal(np.sort(orig), np.arange(n))  # a permutation of the input
+    v = tris.reshape(n, 3, 4)[:, :, :3]
+    np.testing.assert_array_equal(v, tri_xyz.reshape(n, 3, 3)[orig])
+    lo_t, hi_t = v.min(axis=1), v.max(axis=1)
+    alone_pos = np.isin(orig, alone)
+    seen = np.zeros(n, np.int32)
+    stack = [(root, None)]
+    while stack:
+        ref, box = stack.pop()
+        if ref < 0:
+            u = ~ref & 0xFFFFFFFF
+            first, cnt = u >> 3, (u & 7) + 1
+            seen[first:first + cnt] += 1
+            if box is not None:
+                assert np.all(lo_t[first:first + cnt] >= box[0]) and np.all(hi_t[first:first + cnt] <= box[1])
+            if np.any(alone_pos[first:first + cnt]):
+                assert cnt == 1
+            continue
+        nd = nodes[ref]
+        for c in range(4):
+            ch = nd.child[c]
+            if ch == 0x7FFFFFFF:
+                continue
+            cb = (np.array([nd.lox[c], nd.loy[c], nd.loz[c]]), np.array([nd.hix[c], nd.hiy[c], nd.hiz[c]]))
+            assert np.all(cb[0] <= cb[1])
+            if box is not None:
+                assert np.all(cb[0] >= box[0]) and np.all(cb[1] <= box[1])
+            stack.append((ch, cb))
+    assert np.all(seen == 1), (int(np.sum(seen == 0)), int(np.sum(seen > 1)))
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
+@pytest.mark.parametrize("kind,freq", [(P.SCENE_BOX, 8), (P.SCENE_TWO_ROOMS, 24), (P.SCENE_BOX_CAUSTIC, 48)])
+def test_bvh_structure(gpu_ctx, kind, freq, builder):
+    say(f"bvh structure kind={kind} f={freq} builder={builder}")
+    s = P.scenegen(kind, 1, freq, 32, 32)
+    gpu_ctx.upload(s)
+    info = gpu_ctx.build_bvh(builder)
+    nodes, tris, root = gpu_ctx.bvh_download()
+    assert info["nodes"] == len(nodes)
+    # the room walls: a hundred times the median triangle's area and more,
+    # far above the build's big-triangle threshold, so one per leaf
+    tri = np.asarray(s.tri, np.float32).reshape(-1, 9)
+    e = tri.reshape(-1, 3, 3).max(axis=1) - tri.reshape(-1, 3, 3).min(axis=1)
+    area = e[:, 0] * e[:, 1] + e[:, 1] * e[:, 2] + e[:, 2] * e[:, 0]
+    alone = np.nonzero(area > 100.0 * np.median(area))[0]
+    _check_bvh(nodes, tris, root, tri, alone)
 
 
 # ---------------------------------------------------------------------------
