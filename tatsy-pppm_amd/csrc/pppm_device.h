@@ -127,7 +127,10 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
-#define LEAF_MAX 4
+// Triangles per BVH leaf at most (the leaf encoding holds up to 8).  On the
+// caustic box, 1 and 2 trace equally fast and 4, 6, 8 slower (4.44, 4.45,
+// 4.73, 4.97, 5.49 ms per photon pass); 2 keeps half the nodes of 1.
+#define LEAF_MAX 2
 #define EMPTY_REF 0x7FFFFFFF
 
 __host__ __device__ inline int32_t leaf_ref(int32_t first, int32_t count) {
