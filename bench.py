@@ -65,6 +65,47 @@ def read_traffic(name="gather_pmc_caustic_box_1024.txt"):
     return None
 
 
+def read_sq(kernel, counter, name="sq_caustic_box_1024.txt"):
+    """Per-launch value of an SQ counter for one kernel from a committed
+    rocprofv3 PMC summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    current = None
+    for line in open(path):
+        if line.startswith("kernel "):
+            current = line[len("kernel "):].split(":", 1)[0]
+        elif current == kernel and line.strip().startswith(counter + ":"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def issue_bound(kernel, probe, cus, iso_ms, pipelined_ms):
+    """The kernel against the VALU issue roofline: its VALU wave instructions
+    per launch (PMC summary) at the measured cycles per full-rate wave
+    instruction, over four SIMDs per CU at the reported clock, against its
+    launch time alone and in the pipeline.  lane_utilization = active lanes per
+    VALU instruction / 64 (SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64))."""
+    valu = read_sq(kernel, "SQ_INSTS_VALU_per_launch")
+    if valu is None or not iso_ms:
+        return None
+    issue_ms = 1e3 * valu * probe["valu_cycles"] / (4.0 * cus * probe["clock_hz"])
+    return {
+        "bound": "valu",
+        "kernel": kernel,
+        "valu_instr_per_launch": valu,
+        "cycles_per_instr": probe["valu_cycles"],
+        "issue_ms": issue_ms,
+        "frac_isolated": issue_ms / iso_ms,
+        "frac_pipelined": issue_ms / pipelined_ms if pipelined_ms else None,
+        "lane_utilization": read_sq(kernel, "lane_utilization"),
+        "source": "profiles/sq_caustic_box_1024.txt (VALU count, lane utilisation); pppm_issue_probe (cycles, clock)",
+    }
+
+
 def cpu_model():
     """Model name, family / model / stepping, and the logical CPUs of the host."""
     info = {}
@@ -643,7 +684,10 @@ def main():
     ctx.begin(args.photons, 3, r1, ray_eps=ray_eps, seed=seed + 1, gather_mode=mode)
     iso = [ctx.render_pass() for _ in range(3)][1:]
     ctx.count_traversal(False)
+    probe = ctx.issue_probe()
+    cus = max(1, ctx.device_info()["cu"])
     iso_ms = sum(s["gather_ms"] for s in iso) / len(iso)
+    iso_photon_ms = sum(s["photon_ms"] for s in iso) / len(iso)
     iso_bytes = sum(gather_bytes(s) for s in iso) / len(iso)
     iso_useful = sum(useful_bytes(s) for s in iso) / len(iso)
     rays_c = sum(s["photon_segments"] for s in iso)
@@ -713,6 +757,7 @@ def main():
                 "isolated_rate": iso_bytes / (iso_ms * 1e-3) / 1e9,
                 "note": "H (40 + 48) + 8 per bucket + 24 per record of the buckets of the reference's 2r "
                         "cells, whether read or not; a work-equivalent rate, not bandwidth"},
+            "issue": issue_bound("k_gather_lanes", probe, cus, iso_ms, gather_ms),
             "records_tested_per_launch": tested,
             "records_in_buckets_per_launch": sum(s["gather_candidates"] for s in stats) / args.steps,
             "records_accepted_per_launch": sum(s["gather_inradius"] for s in stats) / args.steps,
@@ -731,6 +776,7 @@ def main():
             "unit": "GB/s",
             "frac": trav_gbps / HBM_PEAK_GBPS,
             "rays_per_s": photon_rays / (photon_kernel_ms * 1e-3),
+            "issue": issue_bound("k_photon<true>", probe, cus, iso_photon_ms, None),
             "note": "bytes = rays (32 + 16) + 128 per node + 36 per triangle; node and triangle counts from "
                     "the counting kernels on untimed passes; the BVH (about 50 MB) lives in the 256 MB "
                     "last-level cache",

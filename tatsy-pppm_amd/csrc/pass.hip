@@ -585,9 +585,6 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     TRY_HIP(ctx, enqueue_eye(0));
     if (count > 1) TRY_HIP(ctx, enqueue_eye(1));
     for (int32_t k = 0; k < count; k++) {
-        // the event set of pass k + 2 was last used by pass k - 4
-        if (k >= 3) read_times(k - 3);
-        if (k + 2 < count) TRY_HIP(ctx, enqueue_eye(k + 2));
         const PassSet& S = set[k % 3];
         hipEvent_t* E = ctx->pev[k % NEV];
         const int32_t pass = first + k;
@@ -621,10 +618,14 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
-        // the stored count sizes the sorts; meanwhile the eye passes and the
-        // previous gather run
         Counters& hc = hc_pass[(size_t)k];
         TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, A));
+        // while the photons run: the eye side of pass k + 2 (its event set was
+        // last used by pass k - 4, whose times were read in the last iteration)
+        if (k >= 3) read_times(k - 3);
+        if (k + 2 < count) TRY_HIP(ctx, enqueue_eye(k + 2));
+        // the stored count sizes the sorts; meanwhile the eye passes and the
+        // previous gather run
         TRY_HIP(ctx, hipStreamSynchronize(A));
         const int64_t n = (int64_t)hc.photons_stored;
         if (n > ctx->cap) {
