@@ -107,11 +107,16 @@ uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32
 // their grid keys, from the ranks and counts the photon pass wrote: perm1
 // (origin position -> record slot) in u[4], keys and values (record slots) in
 // u[0], u[1] for build_grid(..., keep = 4, keys_ready).  Returns perm1.
-uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count) {
-    const int bits = grid_bucket_bits(g.size, count != nullptr) + 3 * g.sub_bits;
-    const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
+void origin_starts(Ctx* c, hipStream_t s) {
     launch_origin_starts(s, c->rec_count.as<uint32_t>(), c->rp.photons, c->rec_tiles.as<uint32_t>(),
                          c->rec_start.as<uint32_t>());
+}
+
+uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count,
+                       bool starts_done) {
+    const int bits = grid_bucket_bits(g.size, count != nullptr) + 3 * g.sub_bits;
+    const uint32_t sentinel = bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << bits) - 1);
+    if (!starts_done) origin_starts(c, s);
     launch_origin_point_keys(s, c->ph_rec.as<float4>(), c->ph_rec.as<float4>() + 1, c->rec_rank.as<uint8_t>(),
                              c->rec_start.as<uint32_t>(), n, count, g, sentinel, c->u[0].as<uint32_t>(),
                              c->u[1].as<uint32_t>(), c->u[4].as<uint32_t>(), 2);
@@ -213,6 +218,7 @@ int pppm_destroy(void* ctx_raw) {
     (void)hipStreamDestroy(c->stream3);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
+    (void)hipHostFree(c->pinned_counters);
     delete c;
     return PPPM_OK;
 }

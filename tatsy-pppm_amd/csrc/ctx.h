@@ -106,6 +106,10 @@ struct Ctx {
     uint32_t* last_sorted = nullptr;
     int32_t last_sub_bits = 0;  // the last pass ordered each bucket by a cell-local Morton code
     int32_t count_traversal = 0;  // passes run the node / triangle counting kernels
+    // pinned host copies of the pipelined passes' counters, so that their
+    // copy does not hold the host while the photon kernel runs
+    Counters* pinned_counters = nullptr;
+    int64_t pinned_count = 0;
     hipEvent_t ev[13] = {};
     hipStream_t stream2 = nullptr;  // the eye pass runs here, concurrent with the photon pass
     hipStream_t stream3 = nullptr;  // the gather of pipelined passes (pppm_render_passes)
@@ -228,7 +232,12 @@ int render_sub_bits(const Ctx* c, uint32_t size);
 uint32_t* build_grid(Ctx* c, const float4* pos4, const float* pos3, const uint32_t* perm, int64_t n,
                      const GridParams& g, int keep, const unsigned long long* count = nullptr,
                      uint32_t* off_out = nullptr, bool keys_ready = false);
-uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count);
+// The photon records in origin order (photon index, then rank); with
+// starts_done the per-photon record starts (origin_starts, which needs only
+// the photon pass's per-photon counts) are already enqueued on s.
+uint32_t* origin_order(Ctx* c, hipStream_t s, int64_t n, const GridParams& g, const unsigned long long* count,
+                       bool starts_done = false);
+void origin_starts(Ctx* c, hipStream_t s);
 hipError_t ensure_sort_buffers(Ctx* c, int64_t n, uint32_t S);
 hipError_t ensure_hit_sort(Ctx* c, int64_t n);
 void sort_hitpoints(hipStream_t s, DBuf* hk, DBuf& hhist, const float4* hp, const float4* hw, int64_t cap,

@@ -526,6 +526,14 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     for (int i = 0; i < 4; i++) TRY_HIP(ctx, ctx->hk_alt[i].ensure(ctx->hk[i].n));
     TRY_HIP(ctx, ctx->hhist_alt.ensure(ctx->hhist.n));
     TRY_HIP(ctx, ctx->pass_counters.ensure(sizeof(Counters) * (size_t)count));
+    if (ctx->pinned_count < count) {
+        (void)hipHostFree(ctx->pinned_counters);
+        ctx->pinned_counters = nullptr;
+        ctx->pinned_count = 0;
+        TRY_HIP(ctx, hipHostMalloc((void**)&ctx->pinned_counters, sizeof(Counters) * (size_t)count,
+                                   hipHostMallocDefault));
+        ctx->pinned_count = count;
+    }
     hipStream_t A = ctx->stream, Cs = ctx->stream3;
     hipStream_t Bq[2] = {ctx->stream2, ctx->stream4};
     DBuf* hkq[2] = {ctx->hk, ctx->hk_alt};
@@ -535,7 +543,6 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
     const int eye_grid = ctx->cu > 0 ? ctx->cu * EYE_BLOCKS_PER_CU : 2048;
     const int photon_grid = ctx->cu > 0 ? ctx->cu * PHOTON_BLOCKS_PER_CU : 2048;
     const int32_t first = ctx->passes_done + 1;  // pass index of k = 0
-    std::vector<Counters> hc_pass((size_t)count);
     std::vector<float> times((size_t)count * 6, 0.0f);
     std::vector<int32_t> sizes((size_t)count, 0);
     std::vector<double> rads((size_t)count, 0.0);
@@ -618,10 +625,14 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         launch_photon_pass(A, pa, photon_grid);
         TRY_HIP(ctx, hipGetLastError());
         TRY_HIP(ctx, hipEventRecord(E[P1], A));
-        Counters& hc = hc_pass[(size_t)k];
+        // the record starts per photon need only the photon kernel's counts;
+        // the stored count goes to pinned memory, so the copy does not hold
+        // the host, which enqueues the eye side of pass k + 2 while the
+        // photons run (its event set was last used by pass k - 4, whose times
+        // were read in the last iteration)
+        origin_starts(ctx, A);
+        Counters& hc = ctx->pinned_counters[k];
         TRY_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(Counters), hipMemcpyDeviceToHost, A));
-        // while the photons run: the eye side of pass k + 2 (its event set was
-        // last used by pass k - 4, whose times were read in the last iteration)
         if (k >= 3) read_times(k - 3);
         if (k + 2 < count) TRY_HIP(ctx, enqueue_eye(k + 2));
         // the stored count sizes the sorts; meanwhile the eye passes and the
@@ -643,7 +654,7 @@ int pppm_render_passes(void* ctx_raw, int32_t count, pppm_pass_stats* stats) {
         g.size = Sz;
         g.sub_bits = render_sub_bits(ctx, Sz);
         const int keep = 4;
-        uint32_t* perm1 = origin_order(ctx, A, n, g, nullptr);
+        uint32_t* perm1 = origin_order(ctx, A, n, g, nullptr, true);
         ctx->stream = A;
         uint32_t* sorted = build_grid(ctx, nullptr, nullptr, perm1, n, g, keep, nullptr,
                                       S.off->as<uint32_t>(), true);
