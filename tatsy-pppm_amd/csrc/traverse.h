@@ -144,23 +144,30 @@ __device__ inline void trace_resume(const SceneDev& sc, const TraceRay& r, Trace
                 key[c] = (n <= f * SLAB_FAR_SCALE) ? ((__float_as_uint(n) & 0x7FFFFFFCu) | (uint32_t)c)
                                                    : (0xFFFFFFFCu | (uint32_t)c);
             }
-            // sorting network, ascending, on min / max only (near child first)
+            // sorting network, ascending (near child first), on (key, ref)
+            // pairs: each comparator moves the refs with the keys, so no ref
+            // is looked up by slot afterwards (keys are distinct: the slot
+            // bits break ties)
             uint32_t k0 = key[0], k1 = key[1], k2 = key[2], k3 = key[3];
-    #define KSORT(a, b)                  \
-        {                                \
-            const uint32_t lo = min(a, b); \
-            b = max(a, b);               \
-            a = lo;                      \
+            int32_t r0 = (int32_t)ch[0], r1 = (int32_t)ch[1], r2 = (int32_t)ch[2], r3 = (int32_t)ch[3];
+    #define PSORT(ka, kb, ra, rb)               \
+        {                                       \
+            const bool sw = kb < ka;            \
+            const uint32_t kl = sw ? kb : ka;   \
+            kb = sw ? ka : kb;                  \
+            ka = kl;                            \
+            const int32_t rl = sw ? rb : ra;    \
+            rb = sw ? ra : rb;                  \
+            ra = rl;                            \
         }
-            KSORT(k0, k1);
-            KSORT(k2, k3);
-            KSORT(k0, k2);
-            KSORT(k1, k3);
-            KSORT(k1, k2);
-    #undef KSORT
-            const int32_t ref[4] = {(int32_t)ch[0], (int32_t)ch[1], (int32_t)ch[2], (int32_t)ch[3]};
+            PSORT(k0, k1, r0, r1);
+            PSORT(k2, k3, r2, r3);
+            PSORT(k0, k2, r0, r2);
+            PSORT(k1, k3, r1, r3);
+            PSORT(k1, k2, r1, r2);
+    #undef PSORT
             const uint32_t MISS = 0x7F800000u;
-            int32_t push[3] = {ref[k3 & 3u], ref[k2 & 3u], ref[k1 & 3u]};
+            int32_t push[3] = {r3, r2, r1};
             bool pv[3] = {k3 < MISS, k2 < MISS, k1 < MISS};
             if (sp <= STACK_LDS - 3) {
                 // room for three in LDS: every candidate is written to the
@@ -186,7 +193,7 @@ __device__ inline void trace_resume(const SceneDev& sc, const TraceRay& r, Trace
                     }
                 }
             }
-            cur = k0 < MISS ? ref[k0 & 3u] : POP();
+            cur = k0 < MISS ? r0 : POP();
             if (cur < 0 && leaf >= 0) {  // the first leaf waits; traversal goes on
                 leaf = cur;
                 cur = POP();
