@@ -339,9 +339,12 @@ __device__ inline float union_half_area(const float* a, const float* b) {
 }
 
 // Nearest neighbour of every cluster within PLOC_R positions and of its own
-// segment; -1 when it has none.  Ties go to the lower position, which orders
-// all candidate pairs by (distance, lower position, higher position), so
-// following nearest neighbours always ends in a mutual pair.
+// segment; -1 when it has none.  Pairs are ordered by (union area, distance
+// in positions, for adjacent clusters whether the lower one is at an odd
+// position, lower position): a strict order of the pairs themselves, so
+// following nearest neighbours always ends in a mutual pair.  Among equal
+// areas (duplicated triangles) adjacent clusters pair up as (2k, 2k + 1), so
+// a run of duplicates halves every round instead of losing one cluster.
 __global__ __launch_bounds__(PLOC_BLOCK) void k_ploc_nn(const int32_t* __restrict__ cl, int64_t nc,
                                                          const float* __restrict__ boxes,
                                                          int32_t* __restrict__ nn) {
@@ -364,16 +367,25 @@ __global__ __launch_bounds__(PLOC_BLOCK) void k_ploc_nn(const int32_t* __restric
     if (i >= nc) return;
     const int me = threadIdx.x + PLOC_R;
     const float seg = s_box[me][6];
-    float best = __builtin_inff();
+    float best = 0.0f;
     int bd = 0;  // offset of the nearest neighbour, 0 for none
+    int64_t bmn = 0;
+    int bad = 0, btb = 0;
     for (int d = -PLOC_R; d <= PLOC_R; d++) {
         if (d == 0) continue;
         const int o = me + d;
         if (s_box[o][6] != seg) continue;
         const float a = union_half_area(s_box[me], s_box[o]);
-        if (a < best) {
+        const int ad = d < 0 ? -d : d;
+        const int64_t mn = d < 0 ? i + d : i;
+        const int tb = ad == 1 ? (int)(mn & 1) : 0;
+        if (bd == 0 || a < best ||
+            (a == best && (ad < bad || (ad == bad && (tb < btb || (tb == btb && mn < bmn)))))) {
             best = a;
             bd = d;
+            bad = ad;
+            btb = tb;
+            bmn = mn;
         }
     }
     nn[i] = bd == 0 ? -1 : (int32_t)(i + bd);

@@ -259,6 +259,11 @@ int pppm_scene_upload(void* ctx_raw, const float* tri_xyz, const int32_t* bsdf_i
     for (int64_t i = 0; i < n; i++)
         if (bsdf_id[i] < 0 || bsdf_id[i] >= nb)
             return ctx->fail(PPPM_ERR_INVALID, "bsdf id out of range at triangle " + std::to_string(i));
+    // the BVH builders order boxes by Morton codes and surface areas, which a
+    // NaN or infinite coordinate leaves undefined
+    for (int64_t i = 0; i < 9 * n; i++)
+        if (!std::isfinite(tri_xyz[i]))
+            return ctx->fail(PPPM_ERR_INVALID, "non-finite coordinate at triangle " + std::to_string(i / 9));
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->has_scene = false;

@@ -161,6 +161,54 @@ def test_bvh_structure(gpu_ctx, kind, freq, builder):
     _check_bvh(nodes, tris, root, tri, alone)
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_bvh_identical_and_degenerate_triangles(gpu_ctx, builder):
+    """Ties everywhere: 2000 copies of one triangle (every pair of clusters at
+    the same union area), 200 zero-area triangles and a cluster of tiny ones at
+    one point, inside the box's walls; hits and distances against brute force."""
+    say(f"identical / degenerate triangles builder={builder}")
+    base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
+    walls = np.asarray(base.tri, np.float32).reshape(-1, 9)[:12]  # the closed box comes first
+    e = walls.reshape(-1, 3, 3).max(axis=1) - walls.reshape(-1, 3, 3).min(axis=1)
+    assert np.all(e.max(axis=1) > 4.0)
+    one = np.array([[2.0, 1.0, 2.0, 3.0, 1.0, 2.0, 2.5, 2.0, 2.5]], np.float32)
+    rng = np.random.default_rng(5)
+    # three points on a line, on a 1/8 grid so that the sums are exact: zero
+    # area in FP32 and FP64 alike (both tests then miss them)
+    line = np.zeros((200, 9), np.float32)
+    a = rng.integers(8, 33, (200, 3)).astype(np.float32) / 8
+    d = rng.integers(-4, 5, (200, 3)).astype(np.float32) / 8
+    line[:, 0:3], line[:, 3:6], line[:, 6:9] = a, a + d, a + 2 * d
+    tiny = (np.array([3.5, 3.5, 1.0] * 3, np.float32) + rng.uniform(-1e-4, 1e-4, (500, 9))).astype(np.float32)
+    tri = np.concatenate([walls, np.repeat(one, 2000, axis=0), line, tiny]).astype(np.float32)
+    sub = P.Scene(tri, np.zeros(len(tri), np.int32), [base.bsdfs[0]], base.lights, base.camera,
+                  base.bbox_min, base.bbox_max)
+    gpu_ctx.upload(sub)
+    info = gpu_ctx.build_bvh(builder)
+    assert info["builder"] == builder
+    nodes, tris, root = gpu_ctx.bvh_download()
+    _check_bvh(nodes, tris, root, tri, np.arange(12))
+    rays = random_rays(rng, 20000, np.array([0.1] * 3), np.array([4.9] * 3))
+    hits = gpu_ctx.intersect(rays)
+    t_ref, tri_ref = O.OracleScene(sub).intersect(rays.view(np.float32).reshape(-1, 8), brute=True)
+    # identical triangles tie: which copy is reported is open, the distance is not
+    assert np.array_equal(hits["tri"] >= 0, tri_ref >= 0)
+    both = (hits["tri"] >= 0) & (tri_ref >= 0)
+    np.testing.assert_allclose(hits["t"][both], t_ref[both], rtol=2e-5, atol=2e-6)
+
+
+def test_scene_upload_rejects_non_finite_coordinates(gpu_ctx):
+    base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
+    tri = np.array(base.tri, np.float32).reshape(-1, 9).copy()
+    tri[5, 4] = np.nan
+    sub = P.Scene(tri, base.bsdf_id, base.bsdfs, base.lights, base.camera, base.bbox_min, base.bbox_max)
+    with pytest.raises(P.PPPMError, match="non-finite"):
+        gpu_ctx.upload(sub)
+    tri[5, 4] = np.inf
+    with pytest.raises(P.PPPMError, match="non-finite"):
+        gpu_ctx.upload(sub)
+
+
 # ---------------------------------------------------------------------------
 # Hash grid: bit-exact
 # ---------------------------------------------------------------------------
