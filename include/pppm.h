@@ -185,7 +185,8 @@ int pppm_scene_upload(void* ctx, const float* tri_xyz /* [n][9] */, const int32_
 #define PPPM_BVH_DEFAULT 0 /* the builder whose trees traced fastest on gfx950 */
 #define PPPM_BVH_LBVH 1    /* radix tree over the Morton codes (Karras 2012) */
 #define PPPM_BVH_PLOC 2    /* locally-ordered clustering on surface area (Meister and Bittner 2018);
-                            * falls back to LBVH when its tree is deeper than the traversal stack */
+                            * falls back to LBVH when its tree would be deeper than the traversal
+                            * stack allows or its clustering would run past 64 + 8 log2(n) rounds */
 int pppm_bvh_build(void* ctx, pppm_bvh_info* info);
 int pppm_bvh_build_with(void* ctx, int32_t builder, pppm_bvh_info* info);
 /* Batch closest hit: replaces Scene::intersect (sources/scene.cc:40-47) and

@@ -570,6 +570,7 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, int builder, 
     out->nodes = nullptr;
     out->tris4 = nullptr;
     out->root = EMPTY_REF;
+    out->builder_used = BVH_BUILD_LBVH;
     out->n_nodes = 0;
     out->n_leaves = 0;
     out->depth = 0;
@@ -628,6 +629,9 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, int builder, 
         const int64_t big_first = n - (int64_t)n_big;
         uint32_t* codes = alt ? keys2 : keys;
         uint32_t* order = alt ? vals2 : vals;
+        bool ploc_gave_up = false;
+        int32_t* cl_final = nullptr;
+        int64_t segs_final = 0;
         if (builder == BVH_BUILD_PLOC && n > LEAF_MAX) {
             HIPCHK(hipMalloc(&child_b, sizeof(int32_t) * 2 * ni));
             HIPCHK(hipMalloc(&size_b, sizeof(int32_t) * 2 * n));
@@ -644,7 +648,18 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, int builder, 
             const int64_t segs = (n_big > 0 ? 1 : 0) + ((int64_t)n_big < n ? 1 : 0);
             int64_t nc = n;
             int32_t *cin = cl_a, *cout = cl_b;
+            // each round adds at most one level to the tree; a round count
+            // far past log2(n) (clusters merging one pair at a time) means a
+            // chain-like tree: the radix tree is built instead
+            int log2n = 0;
+            while ((1ll << log2n) < n) log2n++;
+            const int max_rounds = 64 + 8 * log2n;
+            int rounds = 0;
             while (nc > segs) {
+                if (++rounds > max_rounds) {
+                    ploc_gave_up = true;
+                    break;
+                }
                 const unsigned nb = grid1(nc, PLOC_BLOCK);
                 hipLaunchKernelGGL(k_ploc_nn, dim3(nb), dim3(PLOC_BLOCK), 0, s, cin, nc, boxes_b, nn);
                 hipLaunchKernelGGL(k_ploc_merge, dim3(nb), dim3(PLOC_BLOCK), 0, s, cin, nc, nn, n, counts + 4,
@@ -661,7 +676,14 @@ std::string bvh_build(hipStream_t s, const float* tri9, int64_t n, int builder, 
                 nc = (int64_t)left;
                 int32_t* t = cin; cin = cout; cout = t;
             }
-            if (segs == 2)
+            cl_final = cin;
+            segs_final = segs;
+        }
+        out->builder_used = builder == BVH_BUILD_PLOC && n > LEAF_MAX && !ploc_gave_up ? BVH_BUILD_PLOC
+                                                                                        : BVH_BUILD_LBVH;
+        if (builder == BVH_BUILD_PLOC && n > LEAF_MAX && !ploc_gave_up) {
+            int32_t* cin = cl_final;
+            if (segs_final == 2)
                 hipLaunchKernelGGL(k_ploc_root, dim3(1), dim3(1), 0, s, cin, n, counts + 4, boxes_b, child_b,
                                    size_b, parent);
             uint32_t merged = 0;

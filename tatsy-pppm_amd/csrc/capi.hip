@@ -359,6 +359,7 @@ int pppm_bvh_build_with(void* ctx_raw, int32_t builder, pppm_bvh_info* info) {
     std::string e = bvh_build(ctx->stream, ctx->tri9.as<float>(), ctx->n_tris,
                               builder == PPPM_BVH_PLOC ? BVH_BUILD_PLOC : BVH_BUILD_LBVH, &b);
     if (!e.empty()) return ctx->fail(PPPM_ERR_HIP, "BVH build: " + e);
+    if (builder == PPPM_BVH_PLOC && b.builder_used != BVH_BUILD_PLOC) builder = PPPM_BVH_LBVH;
     // the traversal stack holds at most 3 entries per level below the root:
     // a clustered tree deeper than that is replaced by the radix tree, whose
     // depth is bounded by the 30 Morton bits

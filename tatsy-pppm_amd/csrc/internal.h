@@ -88,6 +88,7 @@ struct BvhBuild {
     int32_t root;
     int64_t n_nodes, n_leaves;
     int32_t depth;
+    int32_t builder_used;  // BVH_BUILD_LBVH or BVH_BUILD_PLOC (PLOC falls back when its rounds run long)
 };
 // Binary trees the 4-wide BVH is collapsed from.
 #define BVH_BUILD_LBVH 1  // radix tree over the Morton codes (Karras 2012)

@@ -197,6 +197,28 @@ def test_bvh_identical_and_degenerate_triangles(gpu_ctx, builder):
     np.testing.assert_allclose(hits["t"][both], t_ref[both], rtol=2e-5, atol=2e-6)
 
 
+def test_ploc_falls_back_on_chain_like_clustering(gpu_ctx):
+    """Triangles 1.5 times farther apart each step: every round only the two
+    nearest clusters are mutual neighbours, so clustering would take n - 1
+    rounds and give a chain; past 64 + 8 log2(n) rounds the radix tree is
+    built instead (and reported)."""
+    say("PLOC round cap")
+    n = 200
+    x = (1.5 ** (np.arange(n) - 100.0)).astype(np.float64)
+    h = 0.01 * x
+    tri = np.zeros((n, 9), np.float32)
+    tri[:, 0], tri[:, 3], tri[:, 6] = x, x + h, x
+    tri[:, 4], tri[:, 8] = h, h
+    base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
+    sub = P.Scene(tri, np.zeros(n, np.int32), [base.bsdfs[0]], base.lights, base.camera,
+                  base.bbox_min, base.bbox_max)
+    gpu_ctx.upload(sub)
+    info = gpu_ctx.build_bvh(P.BVH_PLOC)
+    assert info["builder"] == P.BVH_LBVH
+    nodes, tris, root = gpu_ctx.bvh_download()
+    _check_bvh(nodes, tris, root, tri, np.array([], np.int64))
+
+
 def test_scene_upload_rejects_non_finite_coordinates(gpu_ctx):
     base = P.scenegen(P.SCENE_BOX, 1, 1, 8, 8)
     tri = np.array(base.tri, np.float32).reshape(-1, 9).copy()
