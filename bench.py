@@ -615,11 +615,47 @@ def batch_study(P, np, n_scenes, passes, seed, branches=4):
     }
 
 
+def whole_job(ctx, photons, passes, r1, ray_eps, seed, mode):
+    """The whole progressive job in one pipelined call, from pass 1: whole-job
+    photons/s (pipeline fill and drain included) and the gather's useful-byte
+    rate over ranges of passes. The radius shrinks as r_i^2 = r_1^2 prod_k
+    (k + alpha)/(k + 1), so the in-radius photons (the useful bytes) fall with
+    it while the per-hit-point work stays: the gather's HBM fraction falls
+    over the job."""
+    t0 = time.perf_counter()
+    ctx.begin(photons, passes, r1, ray_eps=ray_eps, seed=seed, gather_mode=mode)
+    begin_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    stats = ctx.render_passes(passes)
+    elapsed = time.perf_counter() - t0
+    edges = [b for b in (1, 3, 11, 33, 65, 129) if b <= passes] + [passes + 1]
+    ranges = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        part = stats[lo - 1:hi - 1]
+        g_ms = sum(s["gather_ms"] for s in part) / len(part)
+        useful = sum(useful_bytes(s) for s in part) / len(part)
+        ranges.append({"passes": [lo, hi - 1], "gather_avg_launch_ms": g_ms,
+                       "useful_bytes_per_launch": useful,
+                       "accepted_per_hitpoint": sum(s["gather_inradius"] for s in part)
+                       / max(1, sum(s["hitpoints"] for s in part)),
+                       "frac": useful / (g_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS})
+    g_ms = sum(s["gather_ms"] for s in stats)
+    useful = sum(useful_bytes(s) for s in stats)
+    return {"passes": passes, "photons_per_pass": photons, "begin_s": begin_s, "wall_s": elapsed,
+            "photons_per_s": photons * passes / elapsed, "ms_per_pass": 1e3 * elapsed / passes,
+            "gather_frac": useful / (g_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "gather_by_passes": ranges,
+            "note": "one render_passes call over the whole job from pass 1 (fill and drain included); "
+                    "gather fraction = useful bytes / pipelined launch span / HBM peak"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--job-passes", type=int, default=256,
+                    help="passes of the whole-job leg (0 skips it): one pipelined run from pass 1")
     ap.add_argument("--photons", type=int, default=4_000_000)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
@@ -677,12 +713,16 @@ def main():
     elapsed = time.perf_counter() - t0
     img = ctx.image()
     assert np.all(np.isfinite(img))
-    # the gather alone (single passes: nothing runs beside it), untimed; these
-    # passes also count BVH nodes and triangles per ray (counting kernels, same
-    # images), which the gather does not depend on
+    job = whole_job(ctx, args.photons, args.job_passes, r1, ray_eps, seed + 2, mode) if args.job_passes > 0 else None
+    # the gather alone (single passes: nothing runs beside it), untimed, over
+    # the timed pass range (same radii); these passes also count BVH nodes and
+    # triangles per ray (counting kernels, same images), which the gather does
+    # not depend on
+    ctx.begin(args.photons, total, r1, ray_eps=ray_eps, seed=seed + 1, gather_mode=mode)
+    for _ in range(args.warmup):
+        ctx.render_pass()
     ctx.count_traversal(True)
-    ctx.begin(args.photons, 3, r1, ray_eps=ray_eps, seed=seed + 1, gather_mode=mode)
-    iso = [ctx.render_pass() for _ in range(3)][1:]
+    iso = [ctx.render_pass() for _ in range(args.steps)]
     ctx.count_traversal(False)
     probe = ctx.issue_probe()
     cus = max(1, ctx.device_info()["cu"])
@@ -747,7 +787,8 @@ def main():
                              "written, position and flux of every photon the estimate sums",
             "avg_launch_ms": gather_ms,
             "note": "timed region runs passes pipelined: the gather shares the GPU with the next pass's photons",
-            "isolated": {"achieved": iso_useful / (iso_ms * 1e-3) / 1e9, "avg_launch_ms": iso_ms,
+            "isolated": {"passes": [args.warmup + 1, total],
+                         "achieved": iso_useful / (iso_ms * 1e-3) / 1e9, "avg_launch_ms": iso_ms,
                          "frac": iso_useful / (iso_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
             "records_read_rate": {"bytes_per_launch": read_bytes,
                                   "achieved": read_bytes / (gather_ms * 1e-3) / 1e9,
@@ -794,6 +835,7 @@ def main():
         "gather_mode": args.gather_mode,
         "photons_stored_per_pass": sum(s["photons_stored"] for s in stats) / args.steps,
         "hitpoints_per_pass": sum(s["hitpoints"] for s in stats) / args.steps,
+        "whole_job": job,
         "time_to_rmse": None,
         "time_to_rmse_vs_fp64_oracle": None,
     }
