@@ -90,10 +90,14 @@ __device__ inline v3f spawn(v3f p, v3f ng, v3f dir, float eps) {
 
 // Emission of photon i: light by power, uniform point, cosine direction (or
 // the learned proposal's mixture); returns the ray origin, direction, flux.
+// ENV: the scene may have an environment light (sc.env set).  Its FP64 code
+// needs more registers than the rest of a kernel, so kernels of scenes
+// without one are compiled without it; the numbers drawn are the same.
+template <bool ENV>
 __device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, Rng& rng, v3f& o, v3f& dir,
                                    v3f& flux) {
     float ul = rng.at(0);
-    if (sc.env && (sc.n_lights == 0 || (double)ul * sc.pick_total >= sc.pick_area)) {
+    if (ENV && sc.env && (sc.n_lights == 0 || (double)ul * sc.pick_total >= sc.pick_area)) {
         // environment: a direction w by the importance cells (envmap.cc:62-80;
         // FP64, so the cell, the texel and the weight are the host's), a point
         // on a disc of the bounding sphere's radius facing w, travelling along
@@ -155,9 +159,11 @@ __device__ inline void emit_photon(const PhotonPassArgs& a, const SceneDev& sc, 
 // do not idle behind the longest ray of their wave.  Once the counter has run
 // out every ray is traced to its end.  A photon's random numbers are keyed by
 // its index, so its path and records do not depend on the lane that traces it.
-// Five waves per SIMD: the register budget that keeps the traversal unspilled.
-template <bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_photon(PhotonPassArgs a) {
+// Four waves per SIMD: the 101 registers the kernel takes unspilled (without
+// the environment-light code; with it, spills), measured faster in the
+// pipelined passes than five waves at 96 registers.
+template <bool COUNT, bool ENV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_photon(PhotonPassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
     int32_t ovf_stack[STACK_OVF];
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     if (k < (unsigned long long)a.n_photons) {
                         i = (uint32_t)k;
                         rng.init(a.seed, KIND_PHOTON, pass, i, a.scene_id);
-                        emit_photon(a, sc, rng, o, dir, flux);
+                        emit_photon<ENV>(a, sc, rng, o, dir, flux);
                         bounce = 0;
                         stored = 0;
                         live = a.max_bounces > 0 && vmax3(flux) > 0.0f;
@@ -290,7 +296,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 // back to the gather as they end; a persistent eye kernel with path
 // regeneration was faster alone (2.8 against 3.2 ms) but held its registers
 // for the whole pass and slowed the pipelined passes by 1 %.
-template <bool COUNT>
+template <bool COUNT, bool ENV>
 __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
     __shared__ int32_t s_stack[STACK_LDS * 256];
     int32_t* stk = s_stack + threadIdx.x;
@@ -317,17 +323,20 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         v3f direct = mk(0.0f, 0.0f, 0.0f);
         float4 hit_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
+        // leaving the scene, or at the bounce limit: the environment seen along
+        // d, or black (ppm_probabilistic.cc:248-251); looked up after the loop,
+        // where the traversal's registers are free (its FP64 angles would
+        // otherwise set the register count of the whole loop)
+        bool escaped = false;
         for (int bounce = 0;; bounce++) {
-            // leaving the scene, or at the bounce limit: the environment, or
-            // black (ppm_probabilistic.cc:248-251)
             if (bounce >= a.max_bounces) {
-                if (sc.env) direct = thr * env_lookup(sc, d.x, d.y, d.z);
+                escaped = true;
                 break;
             }
             Hit h = trace_closest<256, COUNT>(sc, o, d, 0.0f, TRACE_TMAX, stk, &ovf, &tc);
             segs++;
             if (h.tri < 0) {
-                if (sc.env) direct = thr * env_lookup(sc, d.x, d.y, d.z);
+                escaped = true;
                 break;
             }
             const int dd = EYE_DIM_BOUNCE + 3 * bounce;
@@ -374,6 +383,7 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
             o = spawn(hp, ng, nd, a.ray_eps);
             d = nd;
         }
+        if (ENV && escaped && sc.env) direct = thr * env_lookup(sc, d.x, d.y, d.z);
         // the hit point of pixel p lives in slot p (pixel id all ones: none), so
         // the hit point order entering the Morton sort is the pixel order; a
         // slot without hit point carries the emission the path saw directly
@@ -538,10 +548,15 @@ void launch_photon_pass(hipStream_t s, const PhotonPassArgs& a, int grid_limit) 
     if (a.n_photons <= 0) return;
     unsigned g = grid1(a.n_photons, 256);
     if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
-    if (a.count_traversal)
-        hipLaunchKernelGGL(k_photon<true>, dim3(g), dim3(256), 0, s, a);
+    const bool env = a.sc.env != nullptr;
+    if (a.count_traversal && env)
+        hipLaunchKernelGGL((k_photon<true, true>), dim3(g), dim3(256), 0, s, a);
+    else if (a.count_traversal)
+        hipLaunchKernelGGL((k_photon<true, false>), dim3(g), dim3(256), 0, s, a);
+    else if (env)
+        hipLaunchKernelGGL((k_photon<false, true>), dim3(g), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_photon<false>, dim3(g), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_photon<false, false>), dim3(g), dim3(256), 0, s, a);
 }
 
 void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
@@ -549,10 +564,15 @@ void launch_eye_pass(hipStream_t s, const EyePassArgs& a, int grid_limit) {
     if (np <= 0) return;
     unsigned g = grid1(np, 256);
     if (grid_limit > 0 && g > (unsigned)grid_limit) g = (unsigned)grid_limit;
-    if (a.count_traversal)
-        hipLaunchKernelGGL(k_eye<true>, dim3(g), dim3(256), 0, s, a);
+    const bool env = a.sc.env != nullptr;
+    if (a.count_traversal && env)
+        hipLaunchKernelGGL((k_eye<true, true>), dim3(g), dim3(256), 0, s, a);
+    else if (a.count_traversal)
+        hipLaunchKernelGGL((k_eye<true, false>), dim3(g), dim3(256), 0, s, a);
+    else if (env)
+        hipLaunchKernelGGL((k_eye<false, true>), dim3(g), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_eye<false>, dim3(g), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_eye<false, false>), dim3(g), dim3(256), 0, s, a);
 }
 
 void launch_direct_add(hipStream_t s, const float4* hp_w, int64_t np, float* img_sum, float* img_comp) {
