@@ -817,7 +817,7 @@ def main():
             "unit": "GB/s",
             "frac": trav_gbps / HBM_PEAK_GBPS,
             "rays_per_s": photon_rays / (photon_kernel_ms * 1e-3),
-            "issue": issue_bound("k_photon<true>", probe, cus, iso_photon_ms, None),
+            "issue": issue_bound("k_photon<true, false>", probe, cus, iso_photon_ms, None),
             "note": "bytes = rays (32 + 16) + 128 per node + 36 per triangle; node and triangle counts from "
                     "the counting kernels on untimed passes; the BVH (about 50 MB) lives in the 256 MB "
                     "last-level cache",
