@@ -6,7 +6,9 @@ Same seeds on both sides, so GPU and oracle draw the same numbers:
   positions and flux as the oracle's FP64 restatement;
 * whole renders (environment alone, and with an area light beside it) within
   5 % of the seed-to-seed noise and below 1e-2 relative RMSE;
-* eye rays leaving the scene show their texel.
+* eye rays leaving the scene show their texel;
+* a black environment renders like none, bit for bit (the kernels built with
+  and without the environment-light code).
 """
 import sys
 
@@ -33,7 +35,7 @@ def env_image(w=256, h=128, seed=3):
     return img.astype(np.float32)
 
 
-def open_scene(res=64, with_light=False):
+def open_scene(res=64, with_light=False, light_down=False):
     floor = S.quad((-3, 0, -3), (0, 0, 6), (6, 0, 0))
     ball = S.icosphere(8, 0.7, (0.5, 0.7, 0.0))
     glass = S.icosphere(8, 0.5, (-1.0, 0.5, 0.8))
@@ -43,7 +45,10 @@ def open_scene(res=64, with_light=False):
     lights = []
     if with_light:
         bs.append(S.bsdf(P.EMITTER, (0, 0, 0), emission=(4.0, 4.0, 4.0)))
-        q, L = S.emitter_quad((-0.5, 2.5, -0.5), (0, 0, 1), (1, 0, 0), (4.0, 4.0, 4.0), 3)
+        if light_down:  # emitting along (1, 0, 0) x (0, 0, 1) = -y, onto the objects
+            q, L = S.emitter_quad((-0.5, 2.5, -0.5), (1, 0, 0), (0, 0, 1), (4.0, 4.0, 4.0), 3)
+        else:
+            q, L = S.emitter_quad((-0.5, 2.5, -0.5), (0, 0, 1), (1, 0, 0), (4.0, 4.0, 4.0), 3)
         parts.append(q)
         lights.append(L)
     cam = S.camera((0, 1.2, 4.5), (0, -0.25, -1), (0, 1, 0), 50.0, res, res)
@@ -121,6 +126,35 @@ def test_eye_rays_leaving_the_scene_show_their_texel(gpu_ctx):
     same = np.all(g.reshape(-1, 3) == dr.astype(np.float32).astype(np.float64), axis=1)
     # the FP32 camera ray can cross a texel boundary where the FP64 one does not
     assert same.mean() > 0.99, same.mean()
+
+
+def test_black_environment_renders_like_none(gpu_ctx):
+    """An all-black environment map runs the photon and eye kernels built with
+    the environment-light code, but is never chosen to emit (its power is 0)
+    and shows black: photons and image must equal, bit for bit, those of the
+    same scene without one, which runs the kernels built without that code."""
+    say("black environment vs none (kernels with and without the environment code)")
+    s = open_scene(with_light=True, light_down=True)
+    N, passes = 60000, 3
+    out = []
+    try:
+        for env in (None, np.zeros((64, 128, 3), np.float32)):
+            gpu_ctx.upload(s)
+            gpu_ctx.build_bvh()
+            gpu_ctx.set_envmap(env)
+            gpu_ctx.set_camera(s.camera)
+            gpu_ctx.begin(N, passes, 0.01 * s.diagonal(), seed=0x7A75)
+            gpu_ctx.render_pass()
+            photons = gpu_ctx.last_photons()
+            gpu_ctx.render_passes(passes - 1)  # pipelined passes too
+            out.append((gpu_ctx.image(), photons))
+    finally:
+        gpu_ctx.set_envmap(None)
+    (a, pa), (b, pb) = out
+    assert np.any(a > 0) and len(pa[0]) > 0.3 * N
+    for x, y in zip(pa, pb):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a, b)
 
 
 def test_envmap_arguments_are_checked(gpu_ctx):
