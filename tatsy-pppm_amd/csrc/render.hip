@@ -324,9 +324,10 @@ __global__ __launch_bounds__(256) void k_eye(EyePassArgs a) {
         float4 hit_pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 hit_w = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(NO_HITPOINT));
         // leaving the scene, or at the bounce limit: the environment seen along
-        // d, or black (ppm_probabilistic.cc:248-251); looked up after the loop,
-        // where the traversal's registers are free (its FP64 angles would
-        // otherwise set the register count of the whole loop)
+        // d, or black (ppm_probabilistic.cc:248-251), looked up once after the
+        // loop; kernels built with ENV false (scenes without an environment)
+        // leave the lookup out: its FP64 angles take the kernel from 105 to
+        // 177 registers
         bool escaped = false;
         for (int bounce = 0;; bounce++) {
             if (bounce >= a.max_bounces) {
