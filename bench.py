@@ -11,6 +11,8 @@ Prints one JSON line.  value = emitted photons per second of whole-pass wall
 time with the scene resident in HBM.  The roofline object covers the gather
 kernel, timed with HIP events on the library's stream; cpu_baseline times the
 oracle (FP64 CPU restatement, OpenMP) on one full pass of the same workload.
+whole_job renders all 256 passes of the workload in one pipelined call and
+reports its photons/s and the gather's useful-byte fraction by pass range.
 """
 import argparse
 import json
